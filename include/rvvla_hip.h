@@ -1,0 +1,184 @@
+/*
+ * rvvla_hip.h -- C-ABI of the MI355X (gfx950) SpMV library librvvla_hip.so.
+ *
+ * Plain pointers and sizes only; no exceptions cross this boundary; every
+ * entry point returns an int status (RVVLA_OK == 0, negative on error) that
+ * rvvla_strerror() turns into text.
+ *
+ * The library replaces the reference's multiply path
+ *   template<T, Mtx, simd, stage> sparse_mv(op, alpha, const Mtx<T>&, descr,
+ *                                          const vector<T>& b, beta, vector<T>& y)
+ *   (reference include/spmv_mtx.h:21-29)
+ * and its conversion path
+ *   template<T, MtxDist, simd> convert(MtxDist<T>& dist, const spMtxCRS<T>& src,
+ *                                      const convertParams&)
+ *   (reference include/convert.h:30-31)
+ * for the eight formats CSR, HCSR, SELL-C-sigma, CSR5, LAV, VNEC, VHCC, CVR.
+ * The C++ templates in sparse_matrix.h are implemented on top of this ABI.
+ *
+ * Device pointers passed to rvvla_spmv/rvvla_spmm must be allocated on the
+ * current HIP device; the call is ordered on `stream` (hipStream_t, NULL for
+ * the default stream) and does not synchronise.
+ */
+#ifndef RVVLA_HIP_H
+#define RVVLA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+/* The CSR5 codes mirror reference include/CSR5.h:29-34 (CSR5_* values).     */
+#define RVVLA_OK                    0
+#define RVVLA_ERR_UNKNOWN_FORMAT   -1  /* CSR5_UNKNOWN_FORMAT               */
+#define RVVLA_ERR_UNSUPPORTED_OMEGA -2 /* CSR5_UNSUPPORTED_OMEGA            */
+#define RVVLA_ERR_CONVERSION       -3  /* CSR5_CONVERTION_FAILED            */
+#define RVVLA_ERR_UNSUPPORTED_SPMV -4  /* CSR5_UNSUPPORTED_SPMV             */
+#define RVVLA_ERR_VALUE_TYPE       -5  /* CSR5_UNSUPPORTED_VALUE_TYPE       */
+#define RVVLA_ERR_INVALID_ARG      -10 /* std::invalid_argument (crs_to_hcsr.cpp:38) */
+#define RVVLA_ERR_LENGTH           -11 /* std::length_error (crs_to_hcsr.cpp:55,87) */
+#define RVVLA_ERR_LOGIC            -12 /* std::logic_error (crs_to_hcsr.cpp:91)   */
+#define RVVLA_ERR_HIP              -20 /* a HIP runtime call failed           */
+#define RVVLA_ERR_NO_DEVICE        -21 /* no HIP device is visible            */
+#define RVVLA_ERR_OUT_OF_MEMORY    -22 /* device or host allocation failed    */
+#define RVVLA_ERR_NOT_FOUND        -23 /* unknown layout field / parameter    */
+
+/* ---- value storage types ---------------------------------------------- */
+/* F64 / F32: the drop-in precisions (x, y, alpha, beta and accumulation in
+ * the same type).  BF16 / F16: values stored in 16 bits, x and y in FP32,
+ * accumulation in FP32 (compensated on rows longer than 1024 entries).    */
+#define RVVLA_F64   0
+#define RVVLA_F32   1
+#define RVVLA_BF16  2
+#define RVVLA_F16   3
+
+/* ---- formats ------------------------------------------------------------ */
+#define RVVLA_FMT_CSR   0   /* spMtxCRS          reference include/CRS.h:10   */
+#define RVVLA_FMT_HCSR  1   /* spMtxHCSR         reference include/HCSR.h:39  */
+#define RVVLA_FMT_SELL  2   /* spMtxSELL_C_Sigma reference include/SELL_C_Sigma.h:23 */
+#define RVVLA_FMT_CSR5  3   /* spMtxCSR5         reference include/CSR5.h:57  */
+#define RVVLA_FMT_LAV   4   /* spMtxLAV          reference include/LAV.h:10   */
+#define RVVLA_FMT_VNEC  5   /* spMtxVNEC         reference include/VNEC.h:35  */
+#define RVVLA_FMT_VHCC  6   /* spMtxVHCC         reference include/VHCC.h:43  */
+#define RVVLA_FMT_CVR   7   /* spMtxCVR          reference include/CVR.h:22   */
+#define RVVLA_NUM_FORMATS 8
+
+typedef struct rvvla_layout_s* rvvla_layout_t;  /* host-side converted matrix */
+typedef struct rvvla_mat_s*    rvvla_mat_t;     /* device-resident matrix     */
+
+/* ---- library --------------------------------------------------------- */
+const char* rvvla_strerror(int code);
+int         rvvla_version(void);
+/* Name of a format code ("csr", "hcsr", "sell", "csr5", "lav", "vnec",
+ * "vhcc", "cvr"); NULL for an unknown code. */
+const char* rvvla_format_name(int format);
+
+/* ---- conversion (host) -----------------------------------------------
+ * Replaces convert<T, MtxDist, true>(dist, src, params)
+ * (reference include/convert.h:30-31; the only reference converter is
+ * crs_to_HCSR, src/scalar/crs_to_hcsr.cpp:30-169).
+ * `rst`, `col`, `val` are the spMtxCRS fields Rst/Col/Val
+ * (reference include/CRS.h:18-20); `val` holds doubles for RVVLA_F64 and
+ * floats for every other dtype (BF16/F16 round from FP32 at conversion).
+ * `params` is the convertParams map (reference include/convert.h:21-28)
+ * written as "key=value" pairs separated by ',' or ';', e.g. "R=2000,C=2000,p=2";
+ * NULL or "" keeps every default. */
+int rvvla_convert(int format, int dtype, int64_t m, int64_t n, int64_t nnz,
+                  const int32_t* rst, const int32_t* col, const void* val,
+                  const char* params, rvvla_layout_t* out);
+/* Read-only view of one array of the converted host structure, by the
+ * reference field name (e.g. "block_ptr", "row_offset", "vPerm",
+ * "_csr5_tile_descriptor").  elem_bytes is 1, 2, 4 or 8. */
+int rvvla_layout_array(rvvla_layout_t L, const char* field, const void** data,
+                       int64_t* count, int* elem_bytes);
+/* Scalar field or parameter of the converted structure (e.g. "R", "b_n",
+ * "Sigma", "_p", "_tail_tile_start"), returned as a double. */
+int rvvla_layout_scalar(rvvla_layout_t L, const char* field, double* value);
+int rvvla_layout_free(rvvla_layout_t L);
+
+/* ---- upload (host -> device, once) ------------------------------------ */
+int rvvla_upload(rvvla_layout_t L, rvvla_mat_t* out);
+/* Direct upload of spMtxCRS fields (reference include/CRS.h:13-20). */
+int rvvla_csr_upload(int dtype, int64_t m, int64_t n, int64_t nnz,
+                     const int32_t* Rst, const int32_t* Col, const void* Val,
+                     rvvla_mat_t* out);
+/* Direct upload of spMtxHCSR fields (reference include/HCSR.h:42-53), e.g.
+ * a structure produced by the reference's own crs_to_HCSR. */
+int rvvla_hcsr_upload(int dtype, int64_t m, int64_t n, int64_t nz,
+                      int64_t R, int64_t C, int64_t b_m, int64_t b_n,
+                      const int32_t* block_ptr, const uint32_t* row_offset,
+                      int64_t row_ptr_len, const uint32_t* row_ptr,
+                      const uint16_t* col_idx, const void* values,
+                      rvvla_mat_t* out);
+/* Direct upload of spMtxSELL_C_Sigma fields
+ * (reference include/SELL_C_Sigma.h:26-38); C must be 64. */
+int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz,
+                      int64_t cnt_b, int C, int Sigma,
+                      const int32_t* vCl, const int32_t* vCs, const int32_t* vBs,
+                      const int32_t* vSBs, int64_t slots, const uint32_t* vCol,
+                      const uint32_t* vPerm, const void* vVal, rvvla_mat_t* out);
+
+int rvvla_mat_free(rvvla_mat_t A);
+/* Shape and storage of a device matrix.  device_bytes = bytes of the
+ * format's arrays resident in HBM (what one multiply streams, x/y apart). */
+int rvvla_mat_info(rvvla_mat_t A, int64_t* m, int64_t* n, int64_t* nnz,
+                   int* format, int* dtype, int64_t* device_bytes);
+
+/* ---- multiply ------------------------------------------------------------
+ * y = alpha * A * x + beta * y  (the MULT stage of sparse_mv,
+ * reference include/types.h:33-38).  x and y are device pointers of the
+ * compute type (double for F64, float for F32/BF16/F16).  With beta == 0
+ * y is write-only (BLAS convention); the reference's scalar CSR reads y
+ * there too (spmv_mtx_crs.cpp:37-43), so the two differ only when y already
+ * holds NaN or Inf.  Only SPARSE_OPERATION_NON_TRANSPOSE exists, as in the
+ * reference. */
+int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta,
+               void* y, void* stream);
+/* Y = alpha * A * X + beta * Y for k right-hand sides; X is n x k and Y is
+ * m x k, row-major with leading dimensions ldx >= k and ldy >= k
+ * (CSR, F32/F64).  No reference counterpart: the reference takes one vector. */
+int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx,
+               double beta, void* Y, int64_t ldy, void* stream);
+/* ALL_STAGES on host vectors: upload b (and y), multiply, download y,
+ * synchronously -- the std::vector form of sparse_mv
+ * (reference include/spmv_mtx.h:21-29). */
+int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host,
+                    double beta, void* y_host);
+
+/* ---- synthetic matrices (host, deterministic, OpenMP) ----------------
+ * Generators for the benchmark matrices.  Every array is allocated by the
+ * library and released with rvvla_gen_free.  Values are written as double
+ * (convert them for other dtypes).  Streams: splitmix64 keyed by
+ * (seed, purpose, row), so results do not depend on the thread count. */
+typedef struct {
+  int64_t m, n, nnz;
+  int32_t* rst;
+  int32_t* col;
+  double*  val;
+} rvvla_gen_csr;
+/* 5-point Laplacian on a g x g grid, natural row-major order, 4 on the
+ * diagonal and -1 off it. */
+int rvvla_gen_laplacian5(int64_t g, rvvla_gen_csr* out);
+/* n x n, each row = diagonal + (k-1) distinct uniform off-diagonal columns
+ * (include_diag=1), or k distinct uniform columns (include_diag=0); sorted
+ * columns; values uniform[-1,1] (normal=0) or N(0, stdev^2) (normal=1). */
+int rvvla_gen_random_rows(int64_t n, int k, int include_diag, int normal,
+                          double stdev, uint64_t seed, rvvla_gen_csr* out);
+/* n x n power-law rows: length ~ discrete power law exponent `alpha` on
+ * [1, maxlen], rescaled (stochastic rounding, clipped to [1, maxlen]) so the
+ * expected total is `target_nnz`; half of each row within +-`band` of the
+ * diagonal (capped by the columns available there), the rest uniform;
+ * values N(0,1). */
+int rvvla_gen_powerlaw(int64_t n, double alpha, int64_t maxlen, int64_t target_nnz,
+                       int64_t band, uint64_t seed, rvvla_gen_csr* out);
+/* v[i] = uniform[lo, hi) from (seed, purpose, i). */
+int rvvla_gen_uniform(int64_t count, double lo, double hi, uint64_t seed,
+                      uint64_t purpose, double* v);
+int rvvla_gen_free(rvvla_gen_csr* g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RVVLA_HIP_H */

@@ -1,0 +1,387 @@
+// oracle.cpp -- CPU restatement of the reference SpMV path.  TEST
+// INFRASTRUCTURE ONLY: loaded by tests/, by __graft_entry__.smoke() and by
+// bench.py's CPU-baseline leg, never by the library.
+//
+// Every function names the reference lines it restates.  It is compiled the
+// way the reference scalar library is (g++ -O3 -fopenmp -fno-tree-vectorize,
+// reference src/scalar/CMakeLists.txt:17-18), so its sums round exactly as
+// the reference's scalar kernels do on x86-64.
+//
+// Pinning (what checks each function against something independent):
+//  * orc_csr_spmv_*      : bit-equal to the compiled reference scalar CSR
+//                          (oracle/_ref) and to golden hex vectors; Laplacian
+//                          closed forms; dense brute force on tiny inputs.
+//  * orc_hcsr_convert_*  : bit-equal to the reference converter crs_to_HCSR
+//                          (oracle/_ref), to golden fixtures and to the
+//                          worked 6x6 example.
+//  * orc_hcsr_spmv_*     : bit-equal to the reference scalar HCSR kernel.
+//  * orc_sell_build_*    : no reference builder exists; its layout is fed to
+//                          the reference scalar SELL kernel (oracle/_ref) and
+//                          must reproduce the CSR result; sortedness and
+//                          permutation invariants.
+//  * orc_sell_spmv_*     : bit-equal to the reference scalar SELL kernel.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace {
+
+// A converted structure: named arrays (raw bytes + element size) and scalars.
+struct Obj {
+  std::map<std::string, std::pair<std::vector<char>, int>> arrays;
+  std::map<std::string, double> scalars;
+  template <class T>
+  void put(const std::string& k, const std::vector<T>& v) {
+    std::vector<char> b(v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(b.data(), v.data(), b.size());
+    arrays[k] = {std::move(b), (int)sizeof(T)};
+  }
+  template <class T>
+  const T* get(const std::string& k) const {
+    return (const T*)arrays.at(k).first.data();
+  }
+  int64_t count(const std::string& k) const {
+    const auto& a = arrays.at(k);
+    return (int64_t)a.first.size() / a.second;
+  }
+};
+
+// ---------------------------------------------------------------- CSR
+// reference src/scalar/spmv_mtx_crs.cpp:22-46 (double): tmp summed in
+// entry order, y = tmp * alpha + beta * y, rows in dynamic chunks of 256.
+void csr_f64(int64_t m, const int* rst, const int* col, const double* val, const double* b,
+             double alpha, double beta, double* y) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < m; i++) {
+    double tmp = 0.0;
+    for (int j = rst[i]; j < rst[i + 1]; j++) tmp += val[j] * b[col[j]];
+    y[i] = tmp * alpha + beta * y[i];
+  }
+}
+
+// reference src/scalar/spmv_mtx_crs.cpp:48-71 (float): y = alpha * tmp + beta * y.
+void csr_f32(int64_t m, const int* rst, const int* col, const float* val, const float* b,
+             float alpha, float beta, float* y) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < m; i++) {
+    float tmp = 0.0f;
+    for (int j = rst[i]; j < rst[i + 1]; j++) tmp += val[j] * b[col[j]];
+    y[i] = alpha * tmp + beta * y[i];
+  }
+}
+
+// ---------------------------------------------------------------- HCSR
+// reference src/scalar/crs_to_hcsr.cpp:30-135, written serially.
+template <class T>
+Obj* hcsr_convert(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col, const T* val,
+                  int64_t R, int64_t C, double p, int* err) {
+  *err = 0;
+  if (R <= 0 || C <= 0) { *err = -10; return nullptr; }
+  const int64_t bm = (m + R - 1) / R, bn = (n + C - 1) / C, nb = bm * bn;
+  std::vector<int> block_ptr((size_t)nb + 1, 0);
+  std::vector<uint32_t> row_offset((size_t)nb + 1, 0);
+  std::vector<int> kind((size_t)nb, 0);
+  for (int64_t br = 0; br < bm; ++br) {
+    const int64_t r0 = br * R, r1 = std::min(m, r0 + R);
+    std::vector<int64_t> cnt((size_t)bn, 0);
+    for (int64_t i = r0; i < r1; ++i)
+      for (int k = rst[i]; k < rst[i + 1]; ++k) cnt[(size_t)(col[k] / C)]++;
+    for (int64_t j = 0; j < bn; ++j) {
+      const int64_t b = br * bn + j;
+      block_ptr[(size_t)b + 1] = (int)cnt[(size_t)j];
+      if (cnt[(size_t)j] == 0) kind[(size_t)b] = 0;
+      else if (double(r1 - r0) * p < double(cnt[(size_t)j])) {
+        kind[(size_t)b] = 1;
+        row_offset[(size_t)b + 1] = (uint32_t)(r1 - r0 + 1);
+      } else {
+        kind[(size_t)b] = 2;
+        row_offset[(size_t)b + 1] = (uint32_t)cnt[(size_t)j];
+      }
+    }
+  }
+  for (int64_t b = 1; b <= nb; ++b) {
+    block_ptr[(size_t)b] += block_ptr[(size_t)b - 1];
+    row_offset[(size_t)b] += row_offset[(size_t)b - 1];
+  }
+  const uint64_t total_rp = row_offset[(size_t)nb];
+  if (total_rp > (1u << 30) - 1) { *err = -11; return nullptr; }
+  if (block_ptr[(size_t)nb] != nnz) { *err = -12; return nullptr; }
+  std::vector<uint32_t> row_ptr((size_t)total_rp, 0);
+  std::vector<uint16_t> col_idx((size_t)nnz);
+  std::vector<T> values((size_t)nnz);
+  for (int64_t br = 0; br < bm; ++br) {
+    const int64_t r0 = br * R, r1 = std::min(m, r0 + R);
+    std::vector<int64_t> pos((size_t)bn);
+    for (int64_t j = 0; j < bn; ++j) pos[(size_t)j] = block_ptr[(size_t)(br * bn + j)];
+    for (int64_t i = r0; i < r1; ++i) {
+      for (int k = rst[i]; k < rst[i + 1]; ++k) {
+        const int64_t j = col[k] / C, b = br * bn + j, q = pos[(size_t)j]++;
+        col_idx[(size_t)q] = (uint16_t)(col[k] - j * C);
+        values[(size_t)q] = val[k];
+        if (kind[(size_t)b] == 1) row_ptr[row_offset[(size_t)b] + (size_t)(i - r0) + 1]++;
+        else row_ptr[row_offset[(size_t)b] + (size_t)(q - block_ptr[(size_t)b])] = (uint32_t)(i - r0);
+      }
+    }
+    for (int64_t j = 0; j < bn; ++j) {
+      const int64_t b = br * bn + j;
+      if (kind[(size_t)b] != 1) continue;
+      for (int64_t li = 0; li < r1 - r0; ++li)
+        row_ptr[row_offset[(size_t)b] + (size_t)li + 1] += row_ptr[row_offset[(size_t)b] + (size_t)li];
+    }
+  }
+  for (int64_t b = 0; b < nb; ++b) row_offset[(size_t)b] |= (uint32_t)kind[(size_t)b] << 30;
+  Obj* o = new Obj;
+  o->put("block_ptr", block_ptr);
+  o->put("row_offset", row_offset);
+  o->put("row_ptr", row_ptr);
+  o->put("col_idx", col_idx);
+  o->put("values", values);
+  o->scalars = {{"R", (double)R}, {"C", (double)C}, {"m", (double)m}, {"n", (double)n},
+                {"nz", (double)nnz}, {"b_m", (double)bm}, {"b_n", (double)bn}};
+  return o;
+}
+
+// reference src/scalar/spmv_mtx_hcsr.cpp:22-102: per block-row, y *= beta,
+// then every block column in order (CSR kind: y[i] += alpha * row dot;
+// any other kind: COO scatter y[row] += alpha * v * b, also for empty blocks
+// where it adds nothing).
+template <class T>
+void hcsr_spmv(const Obj* o, const T* b, T alpha, T beta, T* y) {
+  const int64_t m = (int64_t)o->scalars.at("m"), n = (int64_t)o->scalars.at("n");
+  const int64_t R = (int64_t)o->scalars.at("R"), C = (int64_t)o->scalars.at("C");
+  const int64_t bn = (int64_t)o->scalars.at("b_n");
+  const int* bp = o->get<int>("block_ptr");
+  const uint32_t* ro = o->get<uint32_t>("row_offset");
+  const uint32_t* rp = o->get<uint32_t>("row_ptr");
+  const uint16_t* ci = o->get<uint16_t>("col_idx");
+  const T* v = o->get<T>("values");
+#pragma omp parallel for schedule(dynamic)
+  for (int64_t i = 0; i < m; i += R) {
+    const int64_t maxr = std::min(R, m - i);
+    for (int64_t r = 0; r < maxr; ++r) y[i + r] *= beta;
+    for (int64_t j = 0; j < n; j += C) {
+      const int64_t blk = j / C + (i / R) * bn;
+      const int start = bp[blk];
+      const uint32_t key = ro[blk] >> 30, off = ro[blk] & 0x3FFFFFFFu;
+      const int nz = bp[blk + 1] - start;
+      const uint32_t* lrp = rp + off;
+      const uint16_t* lc = ci + start;
+      const T* lv = v + start;
+      if (key == 1) {
+        for (int64_t r = 0; r < maxr; ++r) {
+          T tmp = 0;
+          for (uint32_t k = lrp[r]; k < lrp[r + 1]; ++k) tmp += lv[k] * b[j + lc[k]];
+          y[i + r] += alpha * tmp;
+        }
+      } else {
+        for (int k = 0; k < nz; ++k) y[i + lrp[k]] += alpha * lv[k] * b[j + lc[k]];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- SELL-C-sigma
+// Builder (no reference builder exists): stable descending sort by row
+// length inside windows of sigma rows (Kreutzer et al. 2014); slices of C
+// sorted rows; slot (j, k) at Cs + j*Bs + k; padding = value 0 with the
+// row's last column (0 for an empty row).
+template <class T>
+Obj* sell_build(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col, const T* val,
+                int C, int sigma) {
+  std::vector<uint32_t> perm((size_t)m);
+  for (int64_t i = 0; i < m; ++i) perm[(size_t)i] = (uint32_t)i;
+  for (int64_t w = 0; w < m; w += sigma) {
+    const int64_t e = std::min<int64_t>(m, w + sigma);
+    std::stable_sort(perm.begin() + w, perm.begin() + e, [&](uint32_t a, uint32_t b) {
+      return (rst[a + 1] - rst[a]) > (rst[b + 1] - rst[b]);
+    });
+  }
+  const int64_t nb = (m + C - 1) / C;
+  std::vector<int> Cl((size_t)nb), Cs((size_t)nb), Bs((size_t)nb), SBs((size_t)nb);
+  int64_t slots = 0;
+  for (int64_t i = 0; i < nb; ++i) {
+    SBs[(size_t)i] = (int)(i * C);
+    Bs[(size_t)i] = (int)std::min<int64_t>(C, m - i * C);
+    int w = 0;
+    for (int k = 0; k < Bs[(size_t)i]; ++k) {
+      const uint32_t r = perm[(size_t)(i * C + k)];
+      w = std::max(w, rst[r + 1] - rst[r]);
+    }
+    Cl[(size_t)i] = w;
+    Cs[(size_t)i] = (int)slots;
+    slots += (int64_t)w * Bs[(size_t)i];
+  }
+  std::vector<uint32_t> vcol((size_t)slots, 0);
+  std::vector<T> vval((size_t)slots, T(0));
+  for (int64_t i = 0; i < nb; ++i)
+    for (int k = 0; k < Bs[(size_t)i]; ++k) {
+      const uint32_t r = perm[(size_t)(SBs[(size_t)i] + k)];
+      const int len = rst[r + 1] - rst[r];
+      for (int j = 0; j < Cl[(size_t)i]; ++j) {
+        const size_t s = (size_t)Cs[(size_t)i] + (size_t)j * Bs[(size_t)i] + k;
+        if (j < len) {
+          vcol[s] = (uint32_t)col[rst[r] + j];
+          vval[s] = val[rst[r] + j];
+        } else {
+          vcol[s] = len > 0 ? (uint32_t)col[rst[r] + len - 1] : 0u;
+        }
+      }
+    }
+  Obj* o = new Obj;
+  o->put("vCl", Cl);
+  o->put("vCs", Cs);
+  o->put("vBs", Bs);
+  o->put("vSBs", SBs);
+  o->put("vCol", vcol);
+  o->put("vPerm", perm);
+  o->put("vVal", vval);
+  o->scalars = {{"m", (double)m}, {"n", (double)n}, {"nz", (double)nnz}, {"cnt_b", (double)nb},
+                {"C", (double)C}, {"Sigma", (double)sigma}};
+  return o;
+}
+
+// reference src/scalar/spmv_mtx_sell_c_sigma.cpp:22-149 (all stages):
+// y_s[i] = y[perm[i]]; per slice y_tmp[k] += Val*b[Col] over slots j;
+// y_s = y_tmp*alpha + beta*y_s; y[perm[i]] = y_s[i].  Takes the arrays
+// directly so any SELL layout (the library's or this oracle's) can be fed.
+template <class T>
+void sell_spmv(int64_t m, int64_t nb, const int* Cl, const int* Cs, const int* Bs, const int* SBs,
+               const uint32_t* Col, const uint32_t* Perm, const T* Val, const T* b, T alpha,
+               T beta, T* y) {
+  std::vector<T> ys((size_t)m);
+  for (int64_t i = 0; i < m; ++i) ys[(size_t)i] = y[Perm[i]];
+  std::vector<T> acc(64 > (nb ? Bs[0] : 0) ? 64 : Bs[0]);
+  for (int64_t i = 0; i < nb; ++i) {
+    if ((int)acc.size() < Bs[i]) acc.resize((size_t)Bs[i]);
+    for (int k = 0; k < Bs[i]; ++k) acc[(size_t)k] = 0;
+    for (int j = 0; j < Cl[i]; ++j)
+      for (int k = 0; k < Bs[i]; ++k) {
+        const int64_t s = (int64_t)Cs[i] + (int64_t)j * Bs[i] + k;
+        acc[(size_t)k] += Val[s] * b[Col[s]];
+      }
+    for (int k = 0; k < Bs[i]; ++k) {
+      T& yy = ys[(size_t)(SBs[i] + k)];
+      yy = acc[(size_t)k] * alpha + beta * yy;
+    }
+  }
+  for (int64_t i = 0; i < m; ++i) y[Perm[i]] = ys[(size_t)i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_csr_spmv_f64(int64_t m, const int* rst, const int* col, const double* val, const double* b,
+                     double alpha, double beta, double* y) {
+  csr_f64(m, rst, col, val, b, alpha, beta, y);
+  return 0;
+}
+
+int orc_csr_spmv_f32(int64_t m, const int* rst, const int* col, const float* val, const float* b,
+                     float alpha, float beta, float* y) {
+  csr_f32(m, rst, col, val, b, alpha, beta, y);
+  return 0;
+}
+
+// Per-row sum |a_ij| * |x_j| (the scale of the error bound), in double.
+int orc_csr_absdot(int64_t m, const int* rst, const int* col, const double* val, const double* b,
+                   double* out) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < m; i++) {
+    double t = 0.0;
+    for (int j = rst[i]; j < rst[i + 1]; j++) t += std::fabs(val[j]) * std::fabs(b[col[j]]);
+    out[i] = t;
+  }
+  return 0;
+}
+
+// Row dot products accumulated in double-double (TwoProd via fma + TwoSum),
+// rounded once: a reference accurate to ~1e-30 relative for the error
+// statistics of reduced-precision results.
+int orc_csr_dot_dd(int64_t m, const int* rst, const int* col, const double* val, const double* b,
+                   double* out) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t i = 0; i < m; i++) {
+    double hi = 0.0, lo = 0.0;
+    for (int j = rst[i]; j < rst[i + 1]; j++) {
+      const double p = val[j] * b[col[j]];
+      const double pe = std::fma(val[j], b[col[j]], -p);
+      const double s = hi + p;
+      const double z = s - hi;
+      const double e = (hi - (s - z)) + (p - z);
+      hi = s;
+      lo += e + pe;
+    }
+    out[i] = hi + lo;
+  }
+  return 0;
+}
+
+void* orc_hcsr_convert_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                           const double* val, int64_t R, int64_t C, double p, int* err) {
+  return hcsr_convert<double>(m, n, nnz, rst, col, val, R, C, p, err);
+}
+void* orc_hcsr_convert_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                           const float* val, int64_t R, int64_t C, double p, int* err) {
+  return hcsr_convert<float>(m, n, nnz, rst, col, val, R, C, p, err);
+}
+int orc_hcsr_spmv_f64(void* h, const double* b, double alpha, double beta, double* y) {
+  hcsr_spmv<double>((Obj*)h, b, alpha, beta, y);
+  return 0;
+}
+int orc_hcsr_spmv_f32(void* h, const float* b, float alpha, float beta, float* y) {
+  hcsr_spmv<float>((Obj*)h, b, alpha, beta, y);
+  return 0;
+}
+
+void* orc_sell_build_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                         const double* val, int C, int sigma) {
+  return sell_build<double>(m, n, nnz, rst, col, val, C, sigma);
+}
+void* orc_sell_build_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                         const float* val, int C, int sigma) {
+  return sell_build<float>(m, n, nnz, rst, col, val, C, sigma);
+}
+int orc_sell_spmv_f64(int64_t m, int64_t nb, const int* Cl, const int* Cs, const int* Bs,
+                      const int* SBs, const uint32_t* Col, const uint32_t* Perm, const double* Val,
+                      const double* b, double alpha, double beta, double* y) {
+  sell_spmv<double>(m, nb, Cl, Cs, Bs, SBs, Col, Perm, Val, b, alpha, beta, y);
+  return 0;
+}
+int orc_sell_spmv_f32(int64_t m, int64_t nb, const int* Cl, const int* Cs, const int* Bs,
+                      const int* SBs, const uint32_t* Col, const uint32_t* Perm, const float* Val,
+                      const float* b, float alpha, float beta, float* y) {
+  sell_spmv<float>(m, nb, Cl, Cs, Bs, SBs, Col, Perm, Val, b, alpha, beta, y);
+  return 0;
+}
+
+// Generic access to a converted structure.
+int orc_obj_array(void* h, const char* name, const void** data, int64_t* count, int* elem) {
+  Obj* o = (Obj*)h;
+  auto it = o->arrays.find(name);
+  if (it == o->arrays.end()) return -23;
+  *data = it->second.first.data();
+  *elem = it->second.second;
+  *count = (int64_t)it->second.first.size() / it->second.second;
+  return 0;
+}
+int orc_obj_scalar(void* h, const char* name, double* v) {
+  Obj* o = (Obj*)h;
+  auto it = o->scalars.find(name);
+  if (it == o->scalars.end()) return -23;
+  *v = it->second;
+  return 0;
+}
+void orc_obj_free(void* h) { delete (Obj*)h; }
+
+int orc_num_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+"""rvvlasparse_amd -- MI355X (gfx950) SpMV in the eight RVVLASparse formats.
+
+Python view of the C-ABI in include/rvvla_hip.h (librvvla_hip.so, built
+in-tree by ``rvvlasparse_amd._build``).  Importing fails loudly when the
+library is missing: there is no CPU fallback.
+
+Device vectors are passed as torch tensors on a HIP device (or raw device
+pointers); torch only provides memory and streams here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "librvvla_hip.so"
+
+F64, F32, BF16, F16 = 0, 1, 2, 3
+DTYPES = {"f64": F64, "f32": F32, "bf16": BF16, "f16": F16}
+FORMATS = {"csr": 0, "hcsr": 1, "sell": 2, "csr5": 3, "lav": 4, "vnec": 5, "vhcc": 6, "cvr": 7}
+FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
+
+
+class RvvlaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rvvla error {code}: {msg}")
+        self.code = code
+
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m rvvlasparse_amd._build` "
+        "(the HIP library is required: there is no CPU fallback)")
+
+lib = C.CDLL(str(LIB_PATH))
+
+
+class GenCSR(C.Structure):
+    _fields_ = [("m", C.c_int64), ("n", C.c_int64), ("nnz", C.c_int64),
+                ("rst", C.POINTER(C.c_int32)), ("col", C.POINTER(C.c_int32)),
+                ("val", C.POINTER(C.c_double))]
+
+
+_i64, _i32, _d, _vp, _cp = C.c_int64, C.c_int, C.c_double, C.c_void_p, C.c_char_p
+_sigs = {
+    "rvvla_strerror": (_cp, [_i32]),
+    "rvvla_version": (_i32, []),
+    "rvvla_format_name": (_cp, [_i32]),
+    "rvvla_convert": (_i32, [_i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _cp, C.POINTER(_vp)]),
+    "rvvla_layout_array": (_i32, [_vp, _cp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
+    "rvvla_layout_scalar": (_i32, [_vp, _cp, C.POINTER(_d)]),
+    "rvvla_layout_free": (_i32, [_vp]),
+    "rvvla_upload": (_i32, [_vp, C.POINTER(_vp)]),
+    "rvvla_csr_upload": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "rvvla_hcsr_upload": (_i32, [_i32, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _vp, _i64,
+                                 _vp, _vp, _vp, C.POINTER(_vp)]),
+    "rvvla_sell_upload": (_i32, [_i32, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp,
+                                 _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "rvvla_mat_free": (_i32, [_vp]),
+    "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                              C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
+    "rvvla_spmv": (_i32, [_vp, _d, _vp, _d, _vp, _vp]),
+    "rvvla_spmm": (_i32, [_vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp]),
+    "rvvla_spmv_host": (_i32, [_vp, _d, _vp, _d, _vp]),
+    "rvvla_gen_laplacian5": (_i32, [_i64, C.POINTER(GenCSR)]),
+    "rvvla_gen_random_rows": (_i32, [_i64, _i32, _i32, _i32, _d, C.c_uint64, C.POINTER(GenCSR)]),
+    "rvvla_gen_powerlaw": (_i32, [_i64, _d, _i64, _i64, _i64, C.c_uint64, C.POINTER(GenCSR)]),
+    "rvvla_gen_uniform": (_i32, [_i64, _d, _d, C.c_uint64, C.c_uint64, _vp]),
+    "rvvla_gen_free": (_i32, [C.POINTER(GenCSR)]),
+}
+for _name, (_res, _args) in _sigs.items():
+    _f = getattr(lib, _name)
+    _f.restype, _f.argtypes = _res, _args
+
+
+def check(code: int) -> None:
+    if code != 0:
+        raise RvvlaError(code, lib.rvvla_strerror(code).decode())
+
+
+def _ptr(a) -> int | None:
+    """Address of a numpy array, a torch tensor or an int."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+# --------------------------------------------------------------- matrices
+class CSR:
+    """Host CSR matrix (int32 Rst/Col, float64 values) -- spMtxCRS fields."""
+
+    def __init__(self, m: int, n: int, rst: np.ndarray, col: np.ndarray, val: np.ndarray):
+        self.m, self.n = int(m), int(n)
+        self.rst = np.ascontiguousarray(rst, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.shape[0])
+
+    def row_lengths(self) -> np.ndarray:
+        return np.diff(self.rst)
+
+    def values_for(self, dtype: int) -> np.ndarray:
+        return self.val if dtype == F64 else self.val.astype(np.float32)
+
+
+def _take(g: GenCSR) -> CSR:
+    try:
+        m, n, nnz = g.m, g.n, g.nnz
+        rst = np.ctypeslib.as_array(g.rst, shape=(m + 1,)).copy()
+        col = np.ctypeslib.as_array(g.col, shape=(max(nnz, 1),))[:nnz].copy()
+        val = np.ctypeslib.as_array(g.val, shape=(max(nnz, 1),))[:nnz].copy()
+    finally:
+        lib.rvvla_gen_free(C.byref(g))
+    return CSR(m, n, rst, col, val)
+
+
+def gen_laplacian5(grid: int) -> CSR:
+    g = GenCSR()
+    check(lib.rvvla_gen_laplacian5(grid, C.byref(g)))
+    return _take(g)
+
+
+def gen_random_rows(n: int, k: int, seed: int = 0, include_diag: bool = True,
+                    normal: bool = False, stdev: float = 1.0) -> CSR:
+    g = GenCSR()
+    check(lib.rvvla_gen_random_rows(n, k, int(include_diag), int(normal), stdev, seed, C.byref(g)))
+    return _take(g)
+
+
+def gen_powerlaw(n: int, alpha: float = 1.8, maxlen: int = 50000, target_nnz: int = 64_000_000,
+                 band: int = 1024, seed: int = 0) -> CSR:
+    g = GenCSR()
+    check(lib.rvvla_gen_powerlaw(n, alpha, maxlen, target_nnz, band, seed, C.byref(g)))
+    return _take(g)
+
+
+def gen_uniform(count: int, lo: float = -1.0, hi: float = 1.0, seed: int = 0,
+                purpose: int = 3) -> np.ndarray:
+    v = np.empty(count, dtype=np.float64)
+    check(lib.rvvla_gen_uniform(count, lo, hi, seed, purpose, _ptr(v)))
+    return v
+
+
+_ELEM = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+class Layout:
+    """Host-side converted matrix: convert<T, Fmt, true>(dist, src, params)."""
+
+    def __init__(self, fmt: str | int, a: CSR, dtype: int = F64, params: str = ""):
+        self.format = FORMATS[fmt] if isinstance(fmt, str) else int(fmt)
+        self.dtype = dtype
+        self.csr = a
+        self._vals = a.values_for(dtype)
+        h = C.c_void_p()
+        check(lib.rvvla_convert(self.format, dtype, a.m, a.n, a.nnz, _ptr(a.rst), _ptr(a.col),
+                                _ptr(self._vals), params.encode(), C.byref(h)))
+        self._h = h
+
+    def array(self, name: str, dtype=None) -> np.ndarray:
+        """Copy of one array of the converted structure (raw unsigned view
+        unless `dtype` is given)."""
+        d, cnt, eb = C.c_void_p(), C.c_int64(), C.c_int()
+        check(lib.rvvla_layout_array(self._h, name.encode(), C.byref(d), C.byref(cnt), C.byref(eb)))
+        raw = _ELEM[eb.value]
+        if cnt.value == 0:
+            out = np.empty(0, dtype=raw)
+        else:
+            buf = (C.c_char * (cnt.value * eb.value)).from_address(d.value)
+            out = np.frombuffer(buf, dtype=raw).copy()
+        return out.view(dtype) if dtype is not None else out
+
+    def scalar(self, name: str) -> float:
+        v = C.c_double()
+        check(lib.rvvla_layout_scalar(self._h, name.encode(), C.byref(v)))
+        return v.value
+
+    def upload(self) -> "DeviceMatrix":
+        h = C.c_void_p()
+        check(lib.rvvla_upload(self._h, C.byref(h)))
+        return DeviceMatrix(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.rvvla_layout_free(self._h)
+            self._h = None
+
+
+class DeviceMatrix:
+    """A device-resident matrix (rvvla_mat_t)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        m, n, nnz, b = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        f, d = C.c_int(), C.c_int()
+        check(lib.rvvla_mat_info(handle, C.byref(m), C.byref(n), C.byref(nnz), C.byref(f),
+                                 C.byref(d), C.byref(b)))
+        self.m, self.n, self.nnz = m.value, n.value, nnz.value
+        self.format, self.dtype, self.device_bytes = f.value, d.value, b.value
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    def spmv(self, x, y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        """y = alpha*A*x + beta*y on device vectors, ordered on `stream`
+        (a torch.cuda.Stream, a raw hipStream_t int, or None = torch's
+        current stream when torch tensors are passed)."""
+        check(lib.rvvla_spmv(self._h, alpha, _ptr(x), beta, _ptr(y), _stream(stream, y)))
+
+    def spmm(self, X, Y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        k = int(X.shape[1])
+        check(lib.rvvla_spmm(self._h, k, alpha, _ptr(X), int(X.stride(0)), beta, _ptr(Y),
+                             int(Y.stride(0)), _stream(stream, Y)))
+
+    def spmv_host(self, b: np.ndarray, y: np.ndarray, alpha: float = 1.0, beta: float = 0.0):
+        check(lib.rvvla_spmv_host(self._h, alpha, _ptr(b), beta, _ptr(y)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.rvvla_mat_free(self._h)
+            self._h = None
+
+
+def _stream(stream, like):
+    if stream is None:
+        if hasattr(like, "device") and getattr(like, "is_cuda", False):
+            import torch
+            return torch.cuda.current_stream(like.device).cuda_stream
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def csr_upload(a: CSR, dtype: int = F64) -> DeviceMatrix:
+    h = C.c_void_p()
+    vals = a.values_for(dtype)
+    check(lib.rvvla_csr_upload(dtype, a.m, a.n, a.nnz, _ptr(a.rst), _ptr(a.col), _ptr(vals),
+                               C.byref(h)))
+    return DeviceMatrix(h)
+
+
+def convert(fmt: str | int, a: CSR, dtype: int = F64, params: str = "") -> Layout:
+    return Layout(fmt, a, dtype, params)
+
+
+def io_dtype(dtype: int):
+    """numpy dtype of x and y for a value dtype (FP32 for the 16-bit forms)."""
+    return np.float64 if dtype == F64 else np.float32

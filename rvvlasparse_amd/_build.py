@@ -1,0 +1,80 @@
+"""Build librvvla_hip.so in-tree (gfx950 device code, OpenMP host code).
+
+HIP sources (*.hip and capi.cpp) go through hipcc with --offload-arch=gfx950;
+the host converters and generators go through g++ -fopenmp; g++ links the
+shared library against libamdhip64.  Objects land in _build/ at the repo
+root; the library lands next to this file.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+INCLUDE = ROOT / "include"
+BUILD = ROOT / "_build"
+LIB = PKG / "librvvla_hip.so"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+
+HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / "capi.cpp", CSRC / "cxx_api.cpp"]
+HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name not in ("capi.cpp", "cxx_api.cpp"))
+
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    # packed FP32 instructions are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd: list[str]) -> None:
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l
+                    and "is not a recognized feature" not in l)
+    if res.returncode != 0:
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def _compile(src: Path) -> Path:
+    obj = BUILD / (src.name + ".o")
+    inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
+    if src in HIP_SOURCES:
+        hipcc = str(ROCM / "bin" / "hipcc")
+        lang = ["-x", "hip"] if src.suffix == ".cpp" else []
+        _run([hipcc, *HIPCC_FLAGS, *inc, *lang, "-c", str(src), "-o", str(obj)])
+    else:
+        _run(["g++", *GXX_FLAGS, *inc, "-c", str(src), "-o", str(obj)])
+    return obj
+
+
+def build(verbose: bool = True, jobs: int | None = None) -> Path:
+    """Compile every source and link librvvla_hip.so; returns its path."""
+    BUILD.mkdir(exist_ok=True)
+    sources = [s for s in HIP_SOURCES if s.exists()] + HOST_SOURCES
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, sources))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run(["g++", "-shared", "-fopenmp", "-o", str(tmp), *map(str, objs),
+          f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} from {len(sources)} sources", flush=True)
+    return LIB
+
+
+def clean() -> None:
+    shutil.rmtree(BUILD, ignore_errors=True)
+    LIB.unlink(missing_ok=True)
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,84 @@
+// internal.h -- shared declarations of librvvla_hip.so (not installed).
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rvvla_hip.h"
+#include "sparse_matrix.h"
+
+namespace rvvla {
+
+// Error carrying an RVVLA_* code; converted to the return value at the ABI.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+// Last error text of the calling thread (rvvla_strerror appends it).
+void set_last_error(const std::string& s);
+const std::string& last_error();
+
+// convertParams from "k=v,k=v" text.
+SparseMatrixLib::convertParams parse_params(const char* text);
+int param_int(const SparseMatrixLib::convertParams& p, const char* key, int dflt);
+double param_double(const SparseMatrixLib::convertParams& p, const char* key, double dflt);
+
+// ---- host converters (CRS -> reference container fields) -----------------
+template <class V>
+void build_hcsr(SparseMatrixLib::spMtxHCSR<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+                size_t R, size_t C, double p);
+template <class V>
+void build_sell(SparseMatrixLib::spMtxSELL_C_Sigma<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+                int C, int sigma);
+template <class V>
+void build_csr5(SparseMatrixLib::spMtxCSR5<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+                int omega, int sigma);
+template <class V>
+void build_lav(SparseMatrixLib::spMtxLAV<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+               int lanes, double T, int segment_len);
+template <class V>
+void build_cvr(SparseMatrixLib::spMtxCVR<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+               int simd_len, int nstreams);
+template <class V>
+void build_vhcc(SparseMatrixLib::spMtxVHCC<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+                int vec_size, int nparts);
+template <class V>
+void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
+                int nparts);
+
+// Default partition counts and parameters chosen for gfx950 (256 CUs).
+int default_sell_sigma(int64_t m, int64_t nnz);
+int default_csr5_sigma(int64_t m, int64_t nnz);
+int default_parts(int64_t m, int64_t nnz, int64_t items_per_part);
+
+// ---- device matrices ----------------------------------------------------
+// Upload of each format's host container; value arrays are given as V
+// (double for F64, float otherwise) and narrowed for BF16/F16.
+template <class V>
+rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a);
+template <class V>
+rvvla_mat_t upload_hcsr(int dtype, const SparseMatrixLib::spMtxHCSR<V>& a);
+template <class V>
+rvvla_mat_t upload_sell(int dtype, const SparseMatrixLib::spMtxSELL_C_Sigma<V>& a);
+template <class V>
+rvvla_mat_t upload_csr5(int dtype, const SparseMatrixLib::spMtxCSR5<V>& a);
+template <class V>
+rvvla_mat_t upload_lav(int dtype, const SparseMatrixLib::spMtxLAV<V>& a);
+template <class V>
+rvvla_mat_t upload_cvr(int dtype, const SparseMatrixLib::spMtxCVR<V>& a);
+template <class V>
+rvvla_mat_t upload_vhcc(int dtype, const SparseMatrixLib::spMtxVHCC<V>& a);
+template <class V>
+rvvla_mat_t upload_vnec(int dtype, const SparseMatrixLib::spMtxVNEC<V>& a,
+                        const SparseMatrixLib::spMtxCRS<V>& src);
+
+// 16-bit value encodings (round to nearest even).
+uint16_t f32_to_bf16(float f);
+uint16_t f32_to_f16(float f);
+
+}  // namespace rvvla

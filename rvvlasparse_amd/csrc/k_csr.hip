@@ -1,0 +1,189 @@
+// k_csr.hip -- CSR SpMV / SpMM for gfx950.
+//
+// Semantics: y_i = alpha * sum_j Val_j * x[Col_j] + beta * y_i, as the
+// reference scalar CSR (reference src/scalar/spmv_mtx_crs.cpp:37-43), except
+// that y is not read when beta == 0.
+//
+// One workgroup (256 threads = 4 wavefronts) per row block, built once at
+// upload:
+//  * stream block: up to 256 consecutive rows with <= 2048 entries in all.
+//    The entries are read fully coalesced, multiplied by x and staged in
+//    LDS; then each row is summed out of LDS by a group of 1..64 lanes
+//    (group width = largest power of two <= 256 / rows), in entry order
+//    within a lane and by a butterfly across lanes.
+//  * long block: one row with more than 1024 entries; all 256 threads
+//    stride over it, then wave and workgroup reductions.  In the 16-bit
+//    value forms the partial sums are compensated (TwoSum) there.
+#include "kcommon.h"
+
+namespace rvvla {
+
+template <class VT, bool COMP>
+__global__ __launch_bounds__(kWG) void k_csr_adaptive(
+    const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
+    const VT* __restrict__ val, const int32_t* __restrict__ blocks,
+    const typename Val<VT>::io* __restrict__ x, typename Val<VT>::io* __restrict__ y,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  __shared__ A prod[kStreamCap];
+  __shared__ A wsum[2 * (kWG / 64)];
+  const int tid = threadIdx.x;
+  const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+  const int e0 = rst[r0], e1 = rst[r1];
+  const int nnz = e1 - e0, nrows = r1 - r0;
+
+  if (nrows == 1 && nnz > kLongRow) {
+    Acc<A, COMP> a;
+    int e = e0 + tid;
+    for (; e + 3 * kWG < e1; e += 4 * kWG) {
+      const int c0 = col[e], c1 = col[e + kWG], c2 = col[e + 2 * kWG], c3 = col[e + 3 * kWG];
+      const A v0 = Val<VT>::get(val, e), v1 = Val<VT>::get(val, e + kWG);
+      const A v2 = Val<VT>::get(val, e + 2 * kWG), v3 = Val<VT>::get(val, e + 3 * kWG);
+      a.add(v0 * (A)x[c0]);
+      a.add(v1 * (A)x[c1]);
+      a.add(v2 * (A)x[c2]);
+      a.add(v3 * (A)x[c3]);
+    }
+    for (; e < e1; e += kWG) a.add(Val<VT>::get(val, e) * (A)x[col[e]]);
+    a = group_reduce(a, 64);
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) {
+      wsum[w] = a.s;
+      wsum[kWG / 64 + w] = a.c;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      Acc<A, COMP> t;
+      t.s = wsum[0];
+      t.c = wsum[kWG / 64];
+      for (int i = 1; i < kWG / 64; ++i) {
+        Acc<A, COMP> o;
+        o.s = wsum[i];
+        o.c = wsum[kWG / 64 + i];
+        t.merge(o);
+      }
+      y[r0] = alpha * t.value() + scaled_y(beta, y, r0);
+    }
+    return;
+  }
+
+  // stream block: coalesced products into LDS
+  {
+    int k = tid;
+    for (; k + 3 * kWG < nnz; k += 4 * kWG) {
+      const int c0 = col[e0 + k], c1 = col[e0 + k + kWG];
+      const int c2 = col[e0 + k + 2 * kWG], c3 = col[e0 + k + 3 * kWG];
+      const A v0 = Val<VT>::get(val, e0 + k), v1 = Val<VT>::get(val, e0 + k + kWG);
+      const A v2 = Val<VT>::get(val, e0 + k + 2 * kWG), v3 = Val<VT>::get(val, e0 + k + 3 * kWG);
+      prod[k] = v0 * (A)x[c0];
+      prod[k + kWG] = v1 * (A)x[c1];
+      prod[k + 2 * kWG] = v2 * (A)x[c2];
+      prod[k + 3 * kWG] = v3 * (A)x[c3];
+    }
+    for (; k < nnz; k += kWG) prod[k] = Val<VT>::get(val, e0 + k) * (A)x[col[e0 + k]];
+  }
+  __syncthreads();
+  if (nrows > kWG / 2) {
+    if (tid < nrows) {
+      const int r = r0 + tid;
+      const int lo = rst[r] - e0, hi = rst[r + 1] - e0;
+      A s = 0;
+      for (int k = lo; k < hi; ++k) s += prod[k];
+      y[r] = alpha * s + scaled_y(beta, y, r);
+    }
+    return;
+  }
+  int width = 64;
+  while (width * nrows > kWG) width >>= 1;
+  const int g = tid / width, l = tid & (width - 1);
+  Acc<A, false> a;
+  if (g < nrows) {
+    const int lo = rst[r0 + g] - e0, hi = rst[r0 + g + 1] - e0;
+    for (int k = lo + l; k < hi; k += width) a.add(prod[k]);
+  }
+  a = group_reduce(a, width);
+  if (g < nrows && l == 0) {
+    const int r = r0 + g;
+    y[r] = alpha * a.value() + scaled_y(beta, y, r);
+  }
+}
+
+// SpMM: Y = alpha*A*X + beta*Y, X n x k, Y m x k (row-major).  One
+// wavefront per row; lanes cover the k right-hand sides (k <= 64 per pass),
+// so X rows are read as contiguous k-vectors.
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_csr_spmm(
+    const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
+    const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ X, int64_t ldx,
+    typename Val<VT>::io* __restrict__ Y, int64_t ldy, int64_t m, int k,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t row = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= m) return;
+  const int e0 = rst[row], e1 = rst[row + 1];
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int c = c0 + lane;
+    if (c >= k) break;
+    A s = 0;
+    for (int e = e0; e < e1; ++e) s += Val<VT>::get(val, e) * (A)X[(int64_t)col[e] * ldx + c];
+    A* yp = &Y[row * ldy + c];
+    *yp = alpha * s + scaled_y(beta, yp, 0);
+  }
+}
+
+template <class VT>
+static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                   hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevCSR& d = A.csr;
+  if (d.nblocks == 0) return RVVLA_OK;
+  const bool comp = sizeof(VT) == 2;
+  if (comp)
+    hipLaunchKernelGGL((k_csr_adaptive<VT, true>), dim3((unsigned)d.nblocks), dim3(kWG), 0, s,
+                       d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y, (AC)alpha,
+                       (AC)beta);
+  else
+    hipLaunchKernelGGL((k_csr_adaptive<VT, false>), dim3((unsigned)d.nblocks), dim3(kWG), 0, s,
+                       d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y, (AC)alpha,
+                       (AC)beta);
+  return hip_check(hipGetLastError(), "k_csr_adaptive launch");
+}
+
+int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+               hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_csr<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_csr<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_csr<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_csr<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+template <class VT>
+static int run_spmm(const rvvla_mat_s& A, int k, double alpha, const void* X, int64_t ldx,
+                    double beta, void* Y, int64_t ldy, hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  if (A.m == 0 || k == 0) return RVVLA_OK;
+  const unsigned grid = (unsigned)((A.m + kWG / 64 - 1) / (kWG / 64));
+  hipLaunchKernelGGL((k_csr_spmm<VT>), dim3(grid), dim3(kWG), 0, s, A.csr.rst, A.csr.col,
+                     (const VT*)A.csr.val, (const IO*)X, ldx, (IO*)Y, ldy, A.m, k, (AC)alpha,
+                     (AC)beta);
+  return hip_check(hipGetLastError(), "k_csr_spmm launch");
+}
+
+int launch_csr_spmm(const rvvla_mat_s& A, int k, double alpha, const void* X, int64_t ldx,
+                    double beta, void* Y, int64_t ldy, hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_spmm<double>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+    case RVVLA_F32: return run_spmm<float>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+    case RVVLA_BF16: return run_spmm<bf16_t>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+    case RVVLA_F16: return run_spmm<_Float16>(A, k, alpha, X, ldx, beta, Y, ldy, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

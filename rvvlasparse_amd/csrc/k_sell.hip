@@ -1,0 +1,69 @@
+// k_sell.hip -- SELL-C-sigma SpMV for gfx950, C = 64.
+//
+// One wavefront per slice: lane k owns sorted row SBs[i] + k and walks the
+// slice's Cl[i] slots; slot (j, k) is at Cs[i] + j*Bs[i] + k, so every load
+// of values and columns is one contiguous 64-lane access.  The result goes
+// straight to the original row through vPerm, fusing the reference's
+// PREPARATION / GET_RESULTS permutations into the multiply as its vector
+// path does (reference src/rvv/spmv_mtx_sell_c_sigma.cpp:60-85):
+// y[perm[s]] = alpha * acc + beta * y[perm[s]].
+#include "kcommon.h"
+
+namespace rvvla {
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_sell64(
+    const int32_t* __restrict__ cl, const int32_t* __restrict__ cs,
+    const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
+    const uint32_t* __restrict__ col, const VT* __restrict__ val,
+    const uint32_t* __restrict__ perm, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, int64_t nslices, typename Val<VT>::acc alpha,
+    typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t slice = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (slice >= nslices) return;
+  const int height = bs[slice];
+  if (lane >= height) return;
+  const int width = cl[slice];
+  int64_t idx = (int64_t)cs[slice] + lane;
+  A a0 = 0, a1 = 0;
+  int j = 0;
+  for (; j + 1 < width; j += 2) {
+    const uint32_t c0 = col[idx], c1 = col[idx + height];
+    const A v0 = Val<VT>::get(val, idx), v1 = Val<VT>::get(val, idx + height);
+    a0 += v0 * (A)x[c0];
+    a1 += v1 * (A)x[c1];
+    idx += 2 * (int64_t)height;
+  }
+  if (j < width) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
+  const uint32_t r = perm[sbs[slice] + lane];
+  y[r] = alpha * (a0 + a1) + scaled_y(beta, y, r);
+}
+
+template <class VT>
+static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                    hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevSELL& d = A.sell;
+  if (d.nslices == 0) return RVVLA_OK;
+  const unsigned grid = (unsigned)((d.nslices + kWG / 64 - 1) / (kWG / 64));
+  hipLaunchKernelGGL((k_sell64<VT>), dim3(grid), dim3(kWG), 0, s, d.cl, d.cs, d.bs, d.sbs, d.col,
+                     (const VT*)d.val, d.perm, (const IO*)x, (IO*)y, d.nslices, (AC)alpha,
+                     (AC)beta);
+  return hip_check(hipGetLastError(), "k_sell64 launch");
+}
+
+int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_sell<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_sell<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_sell<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_sell<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

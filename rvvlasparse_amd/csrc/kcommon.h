@@ -1,0 +1,107 @@
+// kcommon.h -- device helpers shared by the gfx950 SpMV kernels.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "devmat.h"
+
+namespace rvvla {
+
+// 16-bit value storage: bf16 is kept as raw bits, fp16 as _Float16.
+struct bf16_t {
+  uint16_t bits;
+};
+
+template <class VT>
+struct Val;
+template <>
+struct Val<double> {
+  using acc = double;
+  using io = double;
+  static __device__ __forceinline__ double get(const double* p, int64_t i) { return p[i]; }
+};
+template <>
+struct Val<float> {
+  using acc = float;
+  using io = float;
+  static __device__ __forceinline__ float get(const float* p, int64_t i) { return p[i]; }
+};
+template <>
+struct Val<bf16_t> {
+  using acc = float;
+  using io = float;
+  static __device__ __forceinline__ float get(const bf16_t* p, int64_t i) {
+    return __uint_as_float((uint32_t)p[i].bits << 16);
+  }
+};
+template <>
+struct Val<_Float16> {
+  using acc = float;
+  using io = float;
+  static __device__ __forceinline__ float get(const _Float16* p, int64_t i) { return (float)p[i]; }
+};
+
+// beta * y[i], with y not read at all when beta == 0 (BLAS convention: y
+// is then write-only, so NaN/Inf already in y do not propagate).
+template <class A, class IO>
+__device__ __forceinline__ A scaled_y(A beta, const IO* y, int64_t i) {
+  return beta != A(0) ? beta * (A)y[i] : A(0);
+}
+
+// Exact sum of two floats (Knuth's TwoSum): s + e == a + b.
+template <class T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+#pragma clang fp contract(off)
+  const T t = a + b;
+  const T z = t - a;
+  e = (a - (t - z)) + (b - z);
+  s = t;
+}
+
+// Accumulator: plain, or compensated (value + running error) for long rows
+// in reduced precision.
+template <class T, bool COMP>
+struct Acc {
+  T s = 0, c = 0;
+  __device__ __forceinline__ void add(T p) {
+    if constexpr (COMP) {
+      T e;
+      two_sum(s, p, s, e);
+      c += e;
+    } else {
+      s += p;
+    }
+  }
+  __device__ __forceinline__ void merge(const Acc& o) {
+    if constexpr (COMP) {
+      T e;
+      two_sum(s, o.s, s, e);
+      c += o.c + e;
+    } else {
+      s += o.s;
+    }
+  }
+  __device__ __forceinline__ T value() const { return COMP ? s + c : s; }
+};
+
+template <class T>
+__device__ __forceinline__ T shfl_xor(T v, int m) {
+  return __shfl_xor(v, m, 64);
+}
+
+// Sum over aligned groups of `width` lanes (power of two <= 64); every lane
+// of the wavefront must call it.
+template <class T, bool COMP>
+__device__ __forceinline__ Acc<T, COMP> group_reduce(Acc<T, COMP> a, int width) {
+  for (int off = width >> 1; off > 0; off >>= 1) {
+    Acc<T, COMP> o;
+    o.s = shfl_xor(a.s, off);
+    if constexpr (COMP) o.c = shfl_xor(a.c, off);
+    a.merge(o);
+  }
+  return a;
+}
+
+}  // namespace rvvla

@@ -1,0 +1,190 @@
+"""GPU parity: every format and precision against the CPU oracle on the same
+seeded inputs, through the C-ABI (librvvla_hip.so).
+
+FP64 / FP32: |y - y_oracle| <= 4*(len+2)*u*(|alpha|*sum|a||x| + |beta|*|y0|)
+per row (u = 2^-53 / 2^-24).  16-bit values (FP32 x, y and accumulation):
+per row |y - y_fp64| <= (u_v + (4*len+10)*2^-24) * (|alpha|*sum|a||x| +
+|beta|*|y0|) against the FP64 oracle on the unrounded matrix, u_v = 2^-9
+(bf16) or 2^-12 (fp16) -- the value rounding plus FP32 arithmetic.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import rvvlasparse_amd as R
+from helpers import assert_within_bound, laplacian, rand_csr
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+FORMATS_READY = ["csr", "hcsr", "sell"]
+
+
+def _device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda:0")
+
+
+def _csr(t, n=None):
+    rst, col, val = t
+    return R.CSR(len(rst) - 1, n if n is not None else int(col.max()) + 1 if len(col) else 1,
+                 rst, col, val)
+
+
+_CACHE: dict = {}
+
+
+def case(name) -> R.CSR:
+    if name not in _CACHE:
+        if name == "rand_long":
+            _CACHE[name] = _csr(rand_csr(3000, 2600, 0.004, 1, empty_every=13,
+                                         long_rows=(5, 1700, 1701)), 2600)
+        elif name == "lap64":
+            _CACHE[name] = _csr(laplacian(64), 4096)
+        elif name == "powerlaw":
+            _CACHE[name] = R.gen_powerlaw(30000, 1.8, 5000, 450000, 256, seed=2)
+        elif name == "random16_c1":
+            _CACHE[name] = R.gen_random_rows(65536, 16, seed=0)
+        elif name == "empty":
+            _CACHE[name] = R.CSR(5, 7, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0))
+        elif name == "one_long_row":
+            rng = np.random.default_rng(4)
+            col = np.sort(rng.choice(100000, 60000, replace=False)).astype(np.int32)
+            _CACHE[name] = R.CSR(1, 100000, np.array([0, 60000], np.int32), col,
+                                 rng.normal(size=60000))
+        else:
+            raise KeyError(name)
+    return _CACHE[name]
+
+
+CASES = ["rand_long", "lap64", "powerlaw", "random16_c1", "empty", "one_long_row"]
+
+
+def run_gpu(fmt, dtype, a, x64, y064, alpha, beta, params=""):
+    dev = _device()
+    L = R.convert(fmt, a, dtype, params)
+    M = L.upload()
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    M.spmv(x, y, alpha, beta)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def inputs(a, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1, 1, a.n), rng.uniform(-1, 1, a.m)
+
+
+@pytest.mark.parametrize("fmt", FORMATS_READY)
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("name", CASES)
+def test_spmv_drop_in_precisions(fmt, dtype, name):
+    a = case(name)
+    x64, y064 = inputs(a, 1)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    for alpha, beta in ((1.0, 0.0), (0.5, -1.5)):
+        y = run_gpu(fmt, dtype, a, x64, y064, alpha, beta)
+        v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        y_ref = O.csr_spmv(a.rst, a.col, v, x, alpha, beta, y0)
+        assert y.dtype == dt
+        assert_within_bound(y, y_ref, a.rst, a.col, v, x, alpha, beta, y0, dt,
+                            f"{fmt}/{name}/{dt.__name__}/a={alpha},b={beta}")
+
+
+@pytest.mark.parametrize("fmt", FORMATS_READY)
+@pytest.mark.parametrize("dtype", [R.BF16, R.F16])
+@pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64", "one_long_row"])
+def test_spmv_16bit_values(fmt, dtype, name):
+    a = case(name)
+    x64, y064 = inputs(a, 2)
+    alpha, beta = 1.0, 0.5
+    y = run_gpu(fmt, dtype, a, x64, y064, alpha, beta)
+    # the FP64 oracle on the unrounded FP64 matrix and the FP32-rounded x, y0
+    x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+    y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), alpha, beta,
+                     y032.astype(np.float64))
+    uv = 2.0 ** -9 if dtype == R.BF16 else 2.0 ** -12
+    ln = np.diff(a.rst).astype(np.float64)
+    scale = abs(alpha) * O.csr_absdot(a.rst, a.col, a.val, x32.astype(np.float64)) \
+        + abs(beta) * np.abs(y032.astype(np.float64))
+    bound = (uv + (4 * ln + 10) * 2.0 ** -24) * scale
+    err = np.abs(y.astype(np.float64) - y64)
+    assert np.all(err <= bound), f"{fmt}/{name}: max err/bound {np.max(err / np.maximum(bound, 1e-300)):.3f}"
+    if np.linalg.norm(y64) > 0:
+        rel = np.linalg.norm(y - y64) / np.linalg.norm(y64)
+        assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
+
+
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_direct_uploads_match_converted(dtype):
+    """rvvla_csr_upload / rvvla_hcsr_upload / rvvla_sell_upload (reference
+    container fields) give the same result as convert + upload."""
+    import ctypes as C
+    a = case("rand_long")
+    x64, y064 = inputs(a, 3)
+    dev = _device()
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    outs = []
+    M = R.csr_upload(a, dtype)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    M.spmv(x, y, 1.0, 0.25)
+    outs.append(y.cpu().numpy())
+    L = R.convert("hcsr", a, dtype, "R=256,C=512,p=2")
+    vals = L.array("values", np.float64 if dtype == R.F64 else np.float32)
+    arr = {f: L.array(f) for f in ("block_ptr", "row_offset", "row_ptr", "col_idx")}
+    h = C.c_void_p()
+    R.check(R.lib.rvvla_hcsr_upload(dtype, a.m, a.n, a.nnz, 256, 512, int(L.scalar("b_m")),
+                                    int(L.scalar("b_n")), arr["block_ptr"].ctypes.data,
+                                    arr["row_offset"].ctypes.data, arr["row_ptr"].shape[0],
+                                    arr["row_ptr"].ctypes.data, arr["col_idx"].ctypes.data,
+                                    vals.ctypes.data, C.byref(h)))
+    Mh = R.DeviceMatrix(h)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    Mh.spmv(x, y, 1.0, 0.25)
+    outs.append(y.cpu().numpy())
+    dt = np.float64 if dtype == R.F64 else np.float32
+    y_ref = O.csr_spmv(a.rst, a.col, a.values_for(dtype), x64.astype(dt), 1.0, 0.25,
+                       y064.astype(dt))
+    for y in outs:
+        assert_within_bound(y, y_ref, a.rst, a.col, a.values_for(dtype), x64.astype(dt), 1.0,
+                            0.25, y064.astype(dt), dt, "direct upload")
+
+
+def test_spmv_host_path():
+    """rvvla_spmv_host: the std::vector ALL_STAGES form."""
+    _device()
+    a = case("lap64")
+    x64, y064 = inputs(a, 4)
+    M = R.convert("sell", a, R.F64).upload()
+    y = y064.copy()
+    M.spmv_host(x64, y, 2.0, -1.0)
+    y_ref = O.csr_spmv(a.rst, a.col, a.val, x64, 2.0, -1.0, y064)
+    assert_within_bound(y, y_ref, a.rst, a.col, a.val, x64, 2.0, -1.0, y064, np.float64, "host")
+
+
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("k", [1, 5, 16, 64, 70])
+def test_spmm_csr(dtype, k):
+    a = case("rand_long")
+    dev = _device()
+    rng = np.random.default_rng(k)
+    X64 = rng.uniform(-1, 1, (a.n, k))
+    Y064 = rng.uniform(-1, 1, (a.m, k))
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    dt = np.float64 if dtype == R.F64 else np.float32
+    M = R.csr_upload(a, dtype)
+    X = torch.from_numpy(X64).to(io).to(dev)
+    Y = torch.from_numpy(Y064).to(io).to(dev)
+    M.spmm(X, Y, 0.5, 2.0)
+    Y = Y.cpu().numpy()
+    v = a.values_for(dtype)
+    for j in range(k):
+        y_ref = O.csr_spmv(a.rst, a.col, v, X64[:, j].astype(dt), 0.5, 2.0, Y064[:, j].astype(dt))
+        assert_within_bound(Y[:, j], y_ref, a.rst, a.col, v, X64[:, j].astype(dt), 0.5, 2.0,
+                            Y064[:, j].astype(dt), dt, f"spmm col {j}")
