@@ -164,7 +164,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-READY = ("csr", "hcsr", "sell")
+READY = ("csr", "hcsr", "sell", "csr5", "lav", "vhcc", "cvr")
 
 if __name__ == "__main__":
     main()

@@ -274,9 +274,327 @@ void sell_spmv(int64_t m, int64_t nb, const int* Cl, const int* Cs, const int* B
   for (int64_t i = 0; i < m; ++i) y[Perm[i]] = ys[(size_t)i];
 }
 
+// ---------------------------------------------------------------- CSR5
+// Builder (the reference declares spMtxCSR5::inputCSR/asCSR5 but ships no
+// definition, reference include/CSR5.h:61-65): Liu & Vinter's tile layout as
+// the reference vector kernel reads it (src/rvv/spmv_mtx_csr5.cpp:145-294).
+// Written element by element from a row-of-entry table.
+template <class T>
+Obj* csr5_build(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col, const T* val,
+                int omega, int sigma) {
+  const int64_t W = (int64_t)omega * sigma;
+  const int64_t p = nnz > 0 ? (nnz + W - 1) / W : 1;
+  auto clog2 = [](int64_t v) { int b = 0; while ((int64_t(1) << b) < v) ++b; return b; };
+  const int by = clog2(W), bs = clog2(omega), dl = (by + bs + sigma + 31) / 32;
+  const int f0 = 32 - by - bs;
+  std::vector<int64_t> row_of((size_t)nnz);
+  for (int64_t r = 0; r < m; ++r)
+    for (int e = rst[r]; e < rst[r + 1]; ++e) row_of[(size_t)e] = r;
+  auto starts = [&](int64_t e) { return rst[row_of[(size_t)e]] == e; };
+  std::vector<uint32_t> tp((size_t)p + 1);
+  for (int64_t t = 0; t <= p; ++t) tp[(size_t)t] = t * W < nnz ? (uint32_t)row_of[(size_t)(t * W)] : (uint32_t)m;
+  const int64_t nt = p - 1;
+  std::vector<uint32_t> desc((size_t)std::max<int64_t>(nt, 0) * omega * dl, 0);
+  std::vector<uint32_t> eptr((size_t)p + 1, 0), eoff;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t r0 = tp[(size_t)t], r1 = tp[(size_t)t + 1];
+    bool has_empty = false;
+    for (int64_t r = r0 + 1; r < r1; ++r) has_empty |= rst[r] == rst[r + 1];
+    if (has_empty) tp[(size_t)t] |= 0x80000000u;
+    eptr[(size_t)t] = (uint32_t)eoff.size();
+    // flags and counts
+    std::vector<int> cntl((size_t)omega, 0);
+    for (int l = 0; l < omega; ++l)
+      for (int i = 0; i < sigma; ++i) {
+        const int64_t e = t * W + (int64_t)l * sigma + i;
+        if (!starts(e)) continue;
+        const int64_t w = i < f0 ? 0 : 1 + (i - f0) / 32;
+        const int bit = i < f0 ? 31 - (32 - f0) - i : 31 - (i - f0) % 32;
+        desc[(size_t)(t * omega * dl + w * omega + l)] |= 1u << bit;
+        if (l == 0 && i == 0) continue;
+        cntl[(size_t)l]++;
+        if (has_empty) eoff.push_back((uint32_t)(row_of[(size_t)e] - r0 - 1));
+      }
+    uint32_t before = 0;
+    for (int l = 0; l < omega; ++l) {
+      int nx = l + 1;
+      while (nx < omega && cntl[(size_t)nx] == 0) ++nx;
+      const uint32_t seg = (uint32_t)(nx - l - 1);
+      desc[(size_t)(t * omega * dl + l)] |= (before << (32 - by)) | (seg << (32 - by - bs));
+      before += (uint32_t)cntl[(size_t)l];
+    }
+  }
+  if (nt > 0) eptr[(size_t)nt] = (uint32_t)eoff.size();
+  std::vector<uint32_t> c2((size_t)nnz);
+  std::vector<T> v2((size_t)nnz);
+  for (int64_t e = 0; e < nnz; ++e) {
+    int64_t d = e;
+    if (e < nt * W) {
+      const int64_t t = e / W, l = (e % W) / sigma, i = (e % W) % sigma;
+      d = t * W + i * omega + l;
+    }
+    c2[(size_t)d] = (uint32_t)col[e];
+    v2[(size_t)d] = val[e];
+  }
+  Obj* o = new Obj;
+  o->put("_csr_row_pointer", std::vector<int>(rst, rst + m + 1));
+  o->put("_csr_column_index", c2);
+  o->put("_csr_value", v2);
+  o->put("_csr5_tile_pointer", tp);
+  o->put("_csr5_tile_descriptor", desc);
+  o->put("_csr5_tile_empty_offset_pointer", eptr);
+  o->put("_csr5_tile_empty_offset", eoff);
+  o->scalars = {{"_m", (double)m}, {"_n", (double)n}, {"_nnz", (double)nnz}, {"_p", (double)p},
+                {"_omega", (double)omega}, {"_sigma", (double)sigma}, {"_bit_y_offset", (double)by},
+                {"_bit_seg_offset", (double)bs}, {"_tile_desc_len", (double)dl},
+                {"_tail_tile_start", (double)(tp[(size_t)p - 1] & 0x7FFFFFFFu)},
+                {"_num_offsets", (double)eoff.size()}};
+  return o;
+}
+
+// The reference vector CSR5 multiply restated lane by lane for one thread
+// (reference src/rvv/spmv_mtx_csr5.cpp:498-517 entry: y *= beta;
+// :85-296 compute; :318-335 calibrate; :337-365 tail).  Lane vectors are
+// arrays of omega; v_incolumn starts false (the reference reads it before
+// its first assignment); the simple-loop tile sums its omega*sigma entries
+// in storage order.
+template <class T>
+void csr5_spmv_refsem(int64_t m, int64_t p, int omega, int sigma, int by, int bs, int dl,
+                      int tail_start, const int* rst, const uint32_t* col, const T* val,
+                      const uint32_t* tp, const uint32_t* desc, const uint32_t* eptr,
+                      const uint32_t* eoff, const T* x, T alpha, T beta, T* y) {
+  for (int64_t i = 0; i < m; ++i) y[i] *= beta;
+  const int64_t W = (int64_t)omega * sigma;
+  T calibrator = 0;
+  const uint32_t thread_first_row = p - 1 > 0 ? (tp[0] & 0x7FFFFFFFu) : 0;
+  std::vector<char> incolumn((size_t)omega, 0), local((size_t)omega), red((size_t)omega);
+  std::vector<uint32_t> yoff((size_t)omega), segoff((size_t)omega), dsc((size_t)omega),
+      start((size_t)omega), stop((size_t)omega);
+  std::vector<T> sum((size_t)omega), first((size_t)omega), last((size_t)omega), tmp((size_t)omega),
+      scan((size_t)omega);
+  auto yidx = [&](bool empty, uint32_t off_ptr, uint32_t yo) { return empty ? eoff[off_ptr + yo] : yo; };
+  for (int64_t t = 0; t < p - 1; ++t) {
+    const int64_t base = t * W;
+    uint32_t first_row = tp[t];
+    const uint32_t last_row = tp[t + 1] & 0x7FFFFFFFu;
+    if (first_row == last_row) {
+      const bool row_starts_here = (desc[(size_t)(t * omega * dl)] >> (31 - (by + bs))) & 1u;
+      T s = 0;
+      for (int64_t k = 0; k < W; ++k) s += val[base + k] * x[col[base + k]];
+      s *= alpha;
+      if (first_row == thread_first_row && !row_starts_here) calibrator += s;
+      else y[first_row] += s;
+      continue;
+    }
+    const bool empty = (first_row >> 31) & 1u;
+    first_row &= 0x7FFFFFFFu;
+    T* yl = y + first_row + 1;
+    const uint32_t off_ptr = empty ? eptr[t] : 0;
+    const uint32_t* d = desc + t * omega * dl;
+    for (int l = 0; l < omega; ++l) {
+      first[(size_t)l] = 0;
+      stop[(size_t)l] = 0;
+      dsc[(size_t)l] = d[l];
+      yoff[(size_t)l] = dsc[(size_t)l] >> (32 - by);
+      segoff[(size_t)l] = (dsc[(size_t)l] << by) >> (32 - bs);
+      dsc[(size_t)l] <<= (by + bs);
+      local[(size_t)l] = (dsc[(size_t)l] >> 31) == 1u;
+    }
+    const bool first_incolumn = incolumn[0] != 0;
+    const bool first_all_incolumn = (t == 0) ? first_incolumn : false;
+    local[0] = 1;
+    for (int l = 0; l < omega; ++l) {
+      start[(size_t)l] = local[(size_t)l] ? 0u : 1u;
+      incolumn[(size_t)l] = local[(size_t)l] && l != 0;
+      sum[(size_t)l] = val[base + l] * x[col[base + l]];
+    }
+    int ly = 0;
+    for (int i = 1; i < sigma; ++i) {
+      int norm_i = i - (32 - by - bs);
+      if (norm_i % 32 == 0) {
+        ++ly;
+        for (int l = 0; l < omega; ++l) dsc[(size_t)l] = d[ly * omega + l];
+      }
+      norm_i = (ly == 0) ? i : norm_i % 32;
+      const int off = 31 - norm_i;
+      bool any = false;
+      for (int l = 0; l < omega; ++l) {
+        local[(size_t)l] = (dsc[(size_t)l] >> off) & 1u;
+        any |= local[(size_t)l] != 0;
+      }
+      if (any) {
+        for (int l = 0; l < omega; ++l) {
+          const bool green = incolumn[(size_t)l] && local[(size_t)l];
+          if (green) {
+            yl[yidx(empty, off_ptr, yoff[(size_t)l])] += alpha * sum[(size_t)l];
+            yoff[(size_t)l] += 1;
+          }
+          red[(size_t)l] = local[(size_t)l] && !incolumn[(size_t)l];
+          if (red[(size_t)l]) first[(size_t)l] = sum[(size_t)l];
+          if (local[(size_t)l]) sum[(size_t)l] = 0;
+          incolumn[(size_t)l] |= local[(size_t)l];
+          stop[(size_t)l] += local[(size_t)l] ? 1u : 0u;
+        }
+      }
+      for (int l = 0; l < omega; ++l) sum[(size_t)l] += val[base + (int64_t)i * omega + l] * x[col[base + (int64_t)i * omega + l]];
+    }
+    for (int l = 0; l < omega; ++l) {
+      first[(size_t)l] = incolumn[(size_t)l] ? first[(size_t)l] : sum[(size_t)l];
+      last[(size_t)l] = sum[(size_t)l];
+    }
+    for (int l = 0; l < omega; ++l) tmp[(size_t)l] = start[(size_t)l] == 0 ? T(0) : first[(size_t)l];
+    for (int l = 0; l < omega; ++l) scan[(size_t)l] = l + 1 < omega ? tmp[(size_t)l + 1] : T(0);
+    const std::vector<T> shifted = scan;
+    for (int off = 1; off < omega; off <<= 1) {
+      std::vector<T> prev = scan;
+      for (int l = off; l < omega; ++l) scan[(size_t)l] = prev[(size_t)l] + prev[(size_t)(l - off)];
+    }
+    for (int l = 0; l < omega; ++l) {
+      const uint32_t g = segoff[(size_t)l] + (uint32_t)l;
+      const T s = (scan[g] - scan[(size_t)l]) + shifted[(size_t)l];
+      if (start[(size_t)l] <= stop[(size_t)l]) last[(size_t)l] = last[(size_t)l] + s;
+    }
+    for (int l = 0; l < omega; ++l)
+      if (incolumn[(size_t)l]) yl[yidx(empty, off_ptr, yoff[(size_t)l])] += alpha * last[(size_t)l];
+    T add = incolumn[0] ? first[0] : last[0];
+    add *= alpha;
+    if (first_row == thread_first_row && !first_all_incolumn) calibrator += add;
+    else y[first_row] += add;
+  }
+  if (p - 1 > 0) y[tp[0] & 0x7FFFFFFFu] += calibrator;
+  const int64_t tail0 = (p - 1) * W;
+  for (int64_t r = tail_start; r < m; ++r) {
+    const int64_t e0 = r == tail_start ? tail0 : rst[r];
+    T s = 0;
+    for (int64_t e = e0; e < rst[r + 1]; ++e) s += val[e] * x[col[e]];
+    s *= alpha;
+    y[r] += s;
+  }
+}
+
+// The reference vector VHCC multiply restated lane by lane, for any vector
+// width (reference src/rvv/spmv_vhcc.cpp:73-162 kernel, :340-394 MULT stage:
+// tmp_y of pad_rows zeroed, partitions in order, then
+// result[last_row] += result[overflow_row] per partition, then
+// y = y*beta + alpha*tmp_y).  thr holds 10 ints per partition in
+// thr_info_t order (vbase, rbase, sbase, last_row, overflow_row, start_vec,
+// end_vec, panel_id, merge_start, merge_end); scan masks take vec/8 bytes.
+template <class T>
+void vhcc_spmv_refsem(int64_t m, int vec, int nparts, int64_t pad_rows, const int* thr,
+                      const uint32_t* veceor, const uint8_t* masks, const uint32_t* row_arr,
+                      const uint32_t* col, const T* val, const T* x, T alpha, T beta, T* y) {
+  int nm = 0;
+  while ((1 << nm) < vec) ++nm;
+  const int mb = vec / 8;
+  auto bit = [&](int64_t mask_idx, int j) { return (masks[mask_idx * mb + j / 8] >> (j % 8)) & 1; };
+  std::vector<T> tmp((size_t)pad_rows, T(0)), res((size_t)vec), sh((size_t)vec);
+  for (int t = 0; t < nparts; ++t) {
+    const int* I = thr + 10 * t;
+    int64_t vidx = I[0], sidx = (int64_t)I[0] * (nm + 1), ridx = I[1], e = (int64_t)I[5] * vec;
+    uint32_t eor = veceor[vidx++];
+    std::fill(res.begin(), res.end(), T(0));
+    for (int64_t v = I[5]; v < I[6]; ++v) {
+      for (int j = 0; j < vec; ++j) res[(size_t)j] = val[e + j] * x[col[e + j]] + res[(size_t)j];
+      e += vec;
+      if ((uint32_t)v == eor) {
+        for (int s = 0; s < nm; ++s, ++sidx) {
+          const int d = 1 << s;
+          for (int j = 0; j < vec; ++j) sh[(size_t)j] = j >= d ? res[(size_t)(j - d)] : T(0);
+          for (int j = 0; j < vec; ++j)
+            if (bit(sidx, j)) res[(size_t)j] = res[(size_t)j] + sh[(size_t)j];
+        }
+        const int64_t wr = sidx++;
+        for (int j = 0; j < vec; ++j)
+          if (bit(wr, j)) tmp[row_arr[ridx++]] += res[(size_t)j];
+        eor = veceor[vidx++];
+        const bool carry = !bit(wr, vec - 1);
+        const T last = res[(size_t)vec - 1];
+        std::fill(res.begin(), res.end(), T(0));
+        if (carry) res[0] = last;
+      } else {
+        T s = 0;
+        for (int j = 0; j < vec; ++j) s += res[(size_t)j];
+        std::fill(res.begin(), res.end(), T(0));
+        res[0] = s;
+      }
+    }
+  }
+  for (int t = 0; t < nparts; ++t) tmp[(size_t)thr[10 * t + 3]] += tmp[(size_t)thr[10 * t + 4]];
+  for (int64_t i = 0; i < m; ++i) y[i] = y[i] * beta + alpha * tmp[(size_t)i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_vhcc_spmv_f64(int64_t m, int vec, int nparts, int64_t pad_rows, const int* thr,
+                      const uint32_t* veceor, const uint8_t* masks, const uint32_t* row_arr,
+                      const uint32_t* col, const double* val, const double* x, double alpha,
+                      double beta, double* y) {
+  vhcc_spmv_refsem<double>(m, vec, nparts, pad_rows, thr, veceor, masks, row_arr, col, val, x,
+                           alpha, beta, y);
+  return 0;
+}
+int orc_vhcc_spmv_f32(int64_t m, int vec, int nparts, int64_t pad_rows, const int* thr,
+                      const uint32_t* veceor, const uint8_t* masks, const uint32_t* row_arr,
+                      const uint32_t* col, const float* val, const float* x, float alpha,
+                      float beta, float* y) {
+  vhcc_spmv_refsem<float>(m, vec, nparts, pad_rows, thr, veceor, masks, row_arr, col, val, x,
+                          alpha, beta, y);
+  return 0;
+}
+
+void* orc_csr5_build_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                         const double* val, int omega, int sigma) {
+  return csr5_build<double>(m, n, nnz, rst, col, val, omega, sigma);
+}
+void* orc_csr5_build_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                         const float* val, int omega, int sigma) {
+  return csr5_build<float>(m, n, nnz, rst, col, val, omega, sigma);
+}
+// Reference-semantics CSR5 multiply over any CSR5 structure held in an Obj
+// (the oracle's own or one copied in by orc_obj_from_arrays).
+int orc_csr5_spmv_f64(void* h, const double* x, double alpha, double beta, double* y) {
+  const Obj* o = (const Obj*)h;
+  auto S = [&](const char* k) { return (int64_t)o->scalars.at(k); };
+  csr5_spmv_refsem<double>(S("_m"), S("_p"), (int)S("_omega"), (int)S("_sigma"),
+                           (int)S("_bit_y_offset"), (int)S("_bit_seg_offset"),
+                           (int)S("_tile_desc_len"), (int)S("_tail_tile_start"),
+                           o->get<int>("_csr_row_pointer"), o->get<uint32_t>("_csr_column_index"),
+                           o->get<double>("_csr_value"), o->get<uint32_t>("_csr5_tile_pointer"),
+                           o->get<uint32_t>("_csr5_tile_descriptor"),
+                           o->get<uint32_t>("_csr5_tile_empty_offset_pointer"),
+                           o->get<uint32_t>("_csr5_tile_empty_offset"), x, alpha, beta, y);
+  return 0;
+}
+int orc_csr5_spmv_f32(void* h, const float* x, float alpha, float beta, float* y) {
+  const Obj* o = (const Obj*)h;
+  auto S = [&](const char* k) { return (int64_t)o->scalars.at(k); };
+  csr5_spmv_refsem<float>(S("_m"), S("_p"), (int)S("_omega"), (int)S("_sigma"),
+                          (int)S("_bit_y_offset"), (int)S("_bit_seg_offset"),
+                          (int)S("_tile_desc_len"), (int)S("_tail_tile_start"),
+                          o->get<int>("_csr_row_pointer"), o->get<uint32_t>("_csr_column_index"),
+                          o->get<float>("_csr_value"), o->get<uint32_t>("_csr5_tile_pointer"),
+                          o->get<uint32_t>("_csr5_tile_descriptor"),
+                          o->get<uint32_t>("_csr5_tile_empty_offset_pointer"),
+                          o->get<uint32_t>("_csr5_tile_empty_offset"), x, alpha, beta, y);
+  return 0;
+}
+
+// An empty structure to be filled with arrays and scalars from outside (so
+// the library's layouts can be run through the restated kernels).
+void* orc_obj_new(void) { return new Obj; }
+int orc_obj_set_array(void* h, const char* name, const void* data, int64_t count, int elem) {
+  Obj* o = (Obj*)h;
+  std::vector<char> b((size_t)(count * elem));
+  if (count) std::memcpy(b.data(), data, b.size());
+  o->arrays[name] = {std::move(b), elem};
+  return 0;
+}
+int orc_obj_set_scalar(void* h, const char* name, double v) {
+  ((Obj*)h)->scalars[name] = v;
+  return 0;
+}
 
 int orc_csr_spmv_f64(int64_t m, const int* rst, const int* col, const double* val, const double* b,
                      double alpha, double beta, double* y) {

@@ -55,6 +55,15 @@ _ORC_SIGS = {
     "orc_obj_array": (_i32, [_vp, _cp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
     "orc_obj_scalar": (_i32, [_vp, _cp, C.POINTER(_d)]),
     "orc_obj_free": (None, [_vp]),
+    "orc_csr5_build_f64": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _i32, _i32]),
+    "orc_csr5_build_f32": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _i32, _i32]),
+    "orc_csr5_spmv_f64": (_i32, [_vp, _vp, _d, _d, _vp]),
+    "orc_csr5_spmv_f32": (_i32, [_vp, _vp, _f, _f, _vp]),
+    "orc_obj_new": (_vp, []),
+    "orc_obj_set_array": (_i32, [_vp, _cp, _vp, _i64, _i32]),
+    "orc_obj_set_scalar": (_i32, [_vp, _cp, _d]),
+    "orc_vhcc_spmv_f64": (_i32, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp]),
+    "orc_vhcc_spmv_f32": (_i32, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp]),
     "orc_num_threads": (_i32, []),
     "orc_set_threads": (None, [_i32]),
 }
@@ -72,6 +81,14 @@ _REF_SIGS = {
     "ref_hcsr_spmv_f32": (_i32, [_vp, _vp, _f, _f, _vp]),
     "ref_hcsr_free_f64": (None, [_vp]),
     "ref_hcsr_free_f32": (None, [_vp]),
+    "ref_cvr_spmv_f64": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _d, _d, _vp]),
+    "ref_cvr_spmv_f32": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _i64, _vp, _vp, _vp, _f, _f, _vp]),
+    "ref_lav_spmv_f64": (_i32, [_i64, _i64, _i64, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                                _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _d,
+                                _d, _vp]),
+    "ref_lav_spmv_f32": (_i32, [_i64, _i64, _i64, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                                _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _f,
+                                _f, _vp]),
     "ref_sell_spmv_f64": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                  _vp, _vp, _d, _d, _vp]),
     "ref_sell_spmv_f32": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
@@ -108,16 +125,19 @@ def csr_spmv(rst, col, val, x, alpha=1.0, beta=0.0, y=None, use_ref=False):
 def csr_absdot(rst, col, val, x):
     m = rst.shape[0] - 1
     out = np.empty(m, dtype=np.float64)
-    orc.orc_csr_absdot(m, _p(rst), _p(col), _p(np.ascontiguousarray(val, np.float64)),
-                       _p(np.ascontiguousarray(x, np.float64)), _p(out))
+    # keep the converted copies alive for the duration of the call
+    v64 = np.ascontiguousarray(val, np.float64)
+    x64 = np.ascontiguousarray(x, np.float64)
+    orc.orc_csr_absdot(m, _p(rst), _p(col), _p(v64), _p(x64), _p(out))
     return out
 
 
 def csr_dot_dd(rst, col, val, x):
     m = rst.shape[0] - 1
     out = np.empty(m, dtype=np.float64)
-    orc.orc_csr_dot_dd(m, _p(rst), _p(col), _p(np.ascontiguousarray(val, np.float64)),
-                       _p(np.ascontiguousarray(x, np.float64)), _p(out))
+    v64 = np.ascontiguousarray(val, np.float64)
+    x64 = np.ascontiguousarray(x, np.float64)
+    orc.orc_csr_dot_dd(m, _p(rst), _p(col), _p(v64), _p(x64), _p(out))
     return out
 
 
@@ -230,4 +250,95 @@ def sell_spmv(arrays: dict, m, n, x, alpha=1.0, beta=0.0, y=None, use_ref=False,
         getattr(orc, "orc_sell_spmv_" + _sfx(dt))(
             m, nb, _p(a["vCl"]), _p(a["vCs"]), _p(a["vBs"]), _p(a["vSBs"]), _p(a["vCol"]),
             _p(a["vPerm"]), _p(a["vVal"]), _p(x), alpha, beta, _p(y))
+    return y
+
+
+# --------------------------------------------------------------- CSR5
+CSR5_ARRAYS = ("_csr_row_pointer", "_csr_column_index", "_csr_value", "_csr5_tile_pointer",
+               "_csr5_tile_descriptor", "_csr5_tile_empty_offset_pointer",
+               "_csr5_tile_empty_offset")
+CSR5_SCALARS = ("_m", "_n", "_nnz", "_p", "_omega", "_sigma", "_bit_y_offset", "_bit_seg_offset",
+                "_tile_desc_len", "_tail_tile_start", "_num_offsets")
+
+
+def csr5_build(m, n, rst, col, val, omega=64, sigma=16):
+    h = getattr(orc, "orc_csr5_build_" + _sfx(val.dtype))(m, n, col.shape[0], _p(rst), _p(col),
+                                                         _p(val), omega, sigma)
+    o = Obj(h, orc.orc_obj_array, orc.orc_obj_free, _orc_scalars(h, CSR5_SCALARS))
+    o.sfx = _sfx(val.dtype)
+    return o
+
+
+def obj_from(arrays: dict, scalars: dict, sfx: str) -> Obj:
+    """An oracle structure holding copies of the given arrays and scalars."""
+    h = orc.orc_obj_new()
+    for k, a in arrays.items():
+        a = np.ascontiguousarray(a)
+        orc.orc_obj_set_array(h, k.encode(), _p(a), a.shape[0], a.dtype.itemsize)
+    for k, v in scalars.items():
+        orc.orc_obj_set_scalar(h, k.encode(), float(v))
+    o = Obj(h, orc.orc_obj_array, orc.orc_obj_free, dict(scalars))
+    o.sfx = sfx
+    return o
+
+
+def cvr_spmv_ref(arrays: dict, scalars: dict, x, alpha=1.0, beta=0.0, y=None):
+    """The reference's own scalar CVR kernel on a CVR layout."""
+    vv = arrays["vPack_vec_vals"]
+    dt = vv.dtype
+    m, n = int(scalars["numRows"]), int(scalars["numCols"])
+    y = np.zeros(m, dtype=dt) if y is None else np.array(y, dtype=dt, copy=True)
+    x = np.ascontiguousarray(x, dtype=dt)
+    a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+    getattr(ref, "ref_cvr_spmv_" + _sfx(dt))(
+        m, n, int(scalars["nItems"]), int(scalars["SIMD_LEN"]), int(scalars["Nthreads"]),
+        _p(a["vPack_vec_vals"]), _p(a["vPack_vec_cols"]), a["vPack_vec_record"].shape[0],
+        _p(a["vPack_vec_record"]), _p(a["writeback_pos_record"]), _p(x), alpha, beta, _p(y))
+    return y
+
+
+def lav_spmv_ref(arrays: dict, scalars: dict, x, alpha=1.0, beta=0.0, y=None):
+    """The reference's own scalar LAV kernel on a LAV layout."""
+    vv = arrays["val"]
+    dt = vv.dtype
+    m, n = int(scalars["m"]), int(scalars["n"])
+    y = np.zeros(m, dtype=dt) if y is None else np.array(y, dtype=dt, copy=True)
+    x = np.ascontiguousarray(x, dtype=dt)
+    a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+    getattr(ref, "ref_lav_spmv_" + _sfx(dt))(
+        m, n, int(scalars["nz"]), int(scalars["SIMD_Lanes"]), int(scalars["segment_count"]),
+        int(scalars["chunk_count"]),
+        _p(a["chunk_offsets"]), a["chunk_offsets"].shape[0],
+        _p(a["chunk_offsets_full_mask"]), a["chunk_offsets_full_mask"].shape[0],
+        _p(a["segment_ptr"]), a["segment_ptr"].shape[0],
+        _p(a["out_order"]), a["out_order"].shape[0], _p(a["mask"]), a["mask"].shape[0],
+        _p(a["val"]), _p(a["col_id"]), a["val"].shape[0], a["sparse_part.Col"].shape[0],
+        _p(a["sparse_part.Rst"]), _p(a["sparse_part.Col"]), _p(a["sparse_part.Val"]),
+        _p(x), alpha, beta, _p(y))
+    return y
+
+
+def vhcc_spmv(arrays: dict, scalars: dict, x, alpha=1.0, beta=0.0, y=None):
+    """The reference vector VHCC multiply restated, on a VHCC layout given
+    as its arrays (_thr_info as 10 int32 per partition) and scalars."""
+    vv = arrays["_vals_arr"]
+    dt = vv.dtype
+    m = int(scalars["num_rows"])
+    y = np.zeros(m, dtype=dt) if y is None else np.array(y, dtype=dt, copy=True)
+    x = np.ascontiguousarray(x, dtype=dt)
+    a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+    getattr(orc, "orc_vhcc_spmv_" + _sfx(dt))(
+        m, int(scalars["vec_size"]), int(scalars["num_threads"]), int(scalars["pad_rows"]),
+        _p(a["_thr_info"]), _p(a["_veceor_ptr"]), _p(a["_scan_mask"]), _p(a["_row_arr"]),
+        _p(a["_col_arr"]), _p(a["_vals_arr"]), _p(x), alpha, beta, _p(y))
+    return y
+
+
+def csr5_spmv(obj, x, alpha=1.0, beta=0.0, y=None):
+    """The reference vector CSR5 multiply restated for one thread."""
+    dt = np.float64 if obj.sfx == "f64" else np.float32
+    m = int(obj.scalars["_m"])
+    y = np.zeros(m, dtype=dt) if y is None else np.array(y, dtype=dt, copy=True)
+    x = np.ascontiguousarray(x, dtype=dt)
+    getattr(orc, "orc_csr5_spmv_" + obj.sfx)(obj._h, _p(x), alpha, beta, _p(y))
     return y

@@ -175,7 +175,93 @@ static int sell_run(int64_t m, int64_t n, int64_t nb, int C, int sigma, const in
   return 0;
 }
 
+// reference src/scalar/spmv_mtx_cvr.cpp:21-118 on caller-provided CVR arrays.
+template <class T>
+static int cvr_run(int64_t m, int64_t n, int64_t nitems, int simd_len, int nthreads,
+                   const T* vals, const uint32_t* cols, int64_t nrec2, const int* rec,
+                   const int* wb, const T* b, T alpha, T beta, T* y) {
+  spMtxCVR<T> c;
+  c.nItems = (size_t)nitems;
+  c.numRows = (size_t)m;
+  c.numCols = (size_t)n;
+  c.SIMD_LEN = (size_t)simd_len;
+  c.Nthreads = c.Nthreads_avail = nthreads;
+  c.vPack_vec_vals.assign(vals, vals + nitems);
+  c.vPack_vec_cols.assign(cols, cols + nitems);
+  c.vPack_vec_record.assign(rec, rec + nrec2);
+  c.writeback_pos_record.assign(wb, wb + (size_t)nthreads * (simd_len + 2));
+  std::vector<T> vb(b, b + n), vy(y, y + m);
+  sparse_mv<T, spMtxCVR, false>(SPARSE_OPERATION_NON_TRANSPOSE, alpha, c, sparse_matrix_descr{}, vb,
+                                beta, vy);
+  std::memcpy(y, vy.data(), sizeof(T) * m);
+  return 0;
+}
+
+// reference src/scalar/spmv_mtx_lav.cpp:23-144 on caller-provided LAV arrays
+// (the sparse part as CSR arrays).
+template <class T>
+static int lav_run(int64_t m, int64_t n, int64_t nz, int lanes, int64_t segment_count,
+                   int64_t chunk_count, const int* chunk_offsets, int64_t n_co,
+                   const int* chunk_full, int64_t n_cf, const uint64_t* segment_ptr, int64_t n_sp,
+                   const uint32_t* out_order, int64_t n_oo, const uint16_t* mask, int64_t n_mask,
+                   const T* val, const uint32_t* col_id, int64_t n_val, int64_t sp_nz,
+                   const int* sp_rst, const int* sp_col, const T* sp_val, const T* b, T alpha,
+                   T beta, T* y) {
+  spMtxLAV<T> l;
+  l.SIMD_Lanes = (size_t)lanes;
+  l.m = (int)m;
+  l.n = (int)n;
+  l.nz = (int)nz;
+  l.segment_count = (size_t)segment_count;
+  l.chunk_count = (size_t)chunk_count;
+  l.chunk_offsets.assign(chunk_offsets, chunk_offsets + n_co);
+  l.chunk_offsets_full_mask.assign(chunk_full, chunk_full + n_cf);
+  l.segment_ptr.assign(segment_ptr, segment_ptr + n_sp);
+  l.out_order.assign(out_order, out_order + n_oo);
+  l.mask.assign(mask, mask + n_mask);
+  l.val.assign(val, val + n_val);
+  l.col_id.assign(col_id, col_id + n_val);
+  l.sparse_part = spMtxCRS<T>((size_t)m, (size_t)n, (size_t)sp_nz, sp_rst, sp_col, sp_val);
+  std::vector<T> vb(b, b + n), vy(y, y + m);
+  sparse_mv<T, spMtxLAV, false>(SPARSE_OPERATION_NON_TRANSPOSE, alpha, l, sparse_matrix_descr{}, vb,
+                                beta, vy);
+  std::memcpy(y, vy.data(), sizeof(T) * m);
+  return 0;
+}
+
 extern "C" {
+int ref_cvr_spmv_f64(int64_t m, int64_t n, int64_t nitems, int simd_len, int nthreads,
+                     const double* vals, const uint32_t* cols, int64_t nrec2, const int* rec,
+                     const int* wb, const double* b, double alpha, double beta, double* y) {
+  return cvr_run<double>(m, n, nitems, simd_len, nthreads, vals, cols, nrec2, rec, wb, b, alpha,
+                         beta, y);
+}
+int ref_cvr_spmv_f32(int64_t m, int64_t n, int64_t nitems, int simd_len, int nthreads,
+                     const float* vals, const uint32_t* cols, int64_t nrec2, const int* rec,
+                     const int* wb, const float* b, float alpha, float beta, float* y) {
+  return cvr_run<float>(m, n, nitems, simd_len, nthreads, vals, cols, nrec2, rec, wb, b, alpha,
+                        beta, y);
+}
+int ref_lav_spmv_f64(int64_t m, int64_t n, int64_t nz, int lanes, int64_t sc, int64_t cc,
+                     const int* co, int64_t n_co, const int* cf, int64_t n_cf, const uint64_t* sp,
+                     int64_t n_sp, const uint32_t* oo, int64_t n_oo, const uint16_t* mask,
+                     int64_t n_mask, const double* val, const uint32_t* col_id, int64_t n_val,
+                     int64_t sp_nz, const int* sp_rst, const int* sp_col, const double* sp_val,
+                     const double* b, double alpha, double beta, double* y) {
+  return lav_run<double>(m, n, nz, lanes, sc, cc, co, n_co, cf, n_cf, sp, n_sp, oo, n_oo, mask,
+                         n_mask, val, col_id, n_val, sp_nz, sp_rst, sp_col, sp_val, b, alpha, beta,
+                         y);
+}
+int ref_lav_spmv_f32(int64_t m, int64_t n, int64_t nz, int lanes, int64_t sc, int64_t cc,
+                     const int* co, int64_t n_co, const int* cf, int64_t n_cf, const uint64_t* sp,
+                     int64_t n_sp, const uint32_t* oo, int64_t n_oo, const uint16_t* mask,
+                     int64_t n_mask, const float* val, const uint32_t* col_id, int64_t n_val,
+                     int64_t sp_nz, const int* sp_rst, const int* sp_col, const float* sp_val,
+                     const float* b, float alpha, float beta, float* y) {
+  return lav_run<float>(m, n, nz, lanes, sc, cc, co, n_co, cf, n_cf, sp, n_sp, oo, n_oo, mask,
+                        n_mask, val, col_id, n_val, sp_nz, sp_rst, sp_col, sp_val, b, alpha, beta,
+                        y);
+}
 int ref_sell_spmv_f64(int64_t m, int64_t n, int64_t nb, int C, int sigma, const int* Cl,
                       const int* Cs, const int* Bs, const int* SBs, int64_t slots,
                       const uint32_t* Col, const uint32_t* Perm, const double* Val,

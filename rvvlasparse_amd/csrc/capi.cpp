@@ -340,11 +340,282 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
   return M.release();
 }
 
+static std::vector<int32_t> empty_row_list(int64_t m, const int* rst) {
+  std::vector<int32_t> e;
+  for (int64_t r = 0; r < m; ++r)
+    if (rst[r] == rst[r + 1]) e.push_back((int32_t)r);
+  return e;
+}
+
+template <class V>
+rvvla_mat_t upload_csr5(int dtype, const spMtxCSR5<V>& a) {
+  check_dtype_for(dtype, std::is_same<V, double>::value);
+  if (a.format() != CSR5_FORMAT_CSR5) throw Error(RVVLA_ERR_UNKNOWN_FORMAT, "CSR5: not converted");
+  if (a._omega != 64) throw Error(RVVLA_ERR_UNSUPPORTED_OMEGA, "CSR5: omega must be 64");
+  if (a._sigma < 1 || a._sigma > 64 || a._tile_desc_len < 1 || a._tile_desc_len > 3)
+    throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "CSR5: sigma must be in [1, 64]");
+  const int64_t m = a._m, n = a._n, nnz = a._nnz, W = 64LL * a._sigma, ntiles = a._p - 1;
+  check_csr(m, n, nnz, a._csr_row_pointer.data(), (const int*)a._csr_column_index.data());
+  if ((int64_t)a._csr5_tile_pointer.size() != a._p + 1 ||
+      (int64_t)a._csr5_tile_descriptor.size() != std::max<int64_t>(ntiles, 0) * 64 * a._tile_desc_len ||
+      ntiles * W > nnz)
+    throw Error(RVVLA_ERR_INVALID_ARG, "CSR5: array sizes do not match");
+  for (int64_t t = 0; t <= a._p; ++t)
+    if ((int64_t)(a._csr5_tile_pointer[(size_t)t] & 0x7FFFFFFFu) > m)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR5: tile_ptr out of range");
+  const int* rst = a._csr_row_pointer.data();
+  auto M = new_mat(RVVLA_FMT_CSR5, dtype, m, n, nnz);
+  DevCSR5& d = M->csr5;
+  d.ntiles = std::max<int64_t>(ntiles, 0);
+  d.sigma = a._sigma;
+  d.dl = a._tile_desc_len;
+  d.bit_y = a._bit_y_offset;
+  d.bit_seg = a._bit_seg_offset;
+  d.tail_base = d.ntiles * W;
+  d.tail_start = a._tail_tile_start;
+  d.first_tail_row = (a._tail_tile_start < m && rst[a._tail_tile_start] < d.tail_base)
+                         ? a._tail_tile_start + 1 : a._tail_tile_start;
+  const std::vector<int32_t> empty = empty_row_list(m, rst);
+  d.nempty = (int64_t)empty.size();
+  d.tile_ptr = M->put_vec(a._csr5_tile_pointer);
+  d.desc = M->put_vec(a._csr5_tile_descriptor);
+  d.eptr = M->put_vec(a._csr5_tile_empty_offset_pointer);
+  d.eoff = M->put_vec(a._csr5_tile_empty_offset);
+  d.rst = M->put_vec(a._csr_row_pointer);
+  d.col = M->put_vec(a._csr_column_index);
+  d.val = put_values(*M, dtype, a._csr_value.data(), a._csr_value.size());
+  d.empty_rows = M->put_vec(empty);
+  const size_t acc = dtype == RVVLA_F64 ? 8 : 4;
+  d.scratch = M->put(nullptr, (size_t)d.ntiles * (2 * acc + 4) + 16);
+  M->device_bytes -= (int64_t)((size_t)d.ntiles * (2 * acc + 4) + 16);
+  return M.release();
+}
+
+template <class V>
+rvvla_mat_t upload_cvr(int dtype, const spMtxCVR<V>& a, int64_t nnz) {
+  check_dtype_for(dtype, std::is_same<V, double>::value);
+  const int64_t L = (int64_t)a.SIMD_LEN, m = (int64_t)a.numRows, n = (int64_t)a.numCols;
+  const int64_t S = a.Nthreads;
+  if (L != 64) throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "CVR: SIMD_LEN must be 64 on gfx950");
+  if (S < 1 || (int64_t)a.writeback_pos_record.size() != S * (L + 2) ||
+      a.vPack_vec_vals.size() != a.nItems || a.vPack_vec_cols.size() != a.nItems ||
+      a.nItems % L != 0 || a.vPack_vec_record.size() % 2 != 0 || a.vPack_vec_record.size() < 2)
+    throw Error(RVVLA_ERR_INVALID_ARG, "CVR: array sizes do not match");
+  for (uint32_t c : a.vPack_vec_cols)
+    if ((int64_t)c >= n) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: column out of range");
+  const int64_t npacks = (int64_t)a.nItems / L, nrec = (int64_t)a.vPack_vec_record.size() / 2 - 1;
+  std::vector<int64_t> pstart((size_t)S + 1);
+  std::vector<int32_t> rstart((size_t)S), rec_rows, tails((size_t)(S * 64), -1);
+  std::vector<uint64_t> flush((size_t)npacks, 0);
+  std::vector<char> done((size_t)m, 0);
+  const int* wb = a.writeback_pos_record.data();
+  for (int64_t t = 0; t <= S; ++t) pstart[(size_t)t] = t < S ? wb[t * (L + 2)] : npacks;
+  for (int64_t t = 0; t < S; ++t) {
+    if (pstart[(size_t)t] < 0 || pstart[(size_t)t] > pstart[(size_t)t + 1])
+      throw Error(RVVLA_ERR_INVALID_ARG, "CVR: stream pack ranges");
+    const int64_t r0 = wb[t * (L + 2) + 1], r1 = t + 1 < S ? wb[(t + 1) * (L + 2) + 1] : nrec;
+    if (r0 < 0 || r0 > r1 || r1 > nrec) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: record ranges");
+    rstart[(size_t)t] = (int32_t)rec_rows.size();
+    for (int64_t q = r0; q < r1; ++q) {
+      const int64_t pos = a.vPack_vec_record[(size_t)(2 * q)];
+      const int64_t row = a.vPack_vec_record[(size_t)(2 * q + 1)];
+      const int64_t k = pos / L, lane = pos % L;
+      if (row < 0 || row >= m || done[(size_t)row] || k < pstart[(size_t)t] || k > pstart[(size_t)t + 1])
+        throw Error(RVVLA_ERR_INVALID_ARG, "CVR: record out of range or row flushed twice");
+      done[(size_t)row] = 1;
+      if (k == pstart[(size_t)t + 1]) {
+        tails[(size_t)(t * 64 + lane)] = (int32_t)row;  // flushed after the stream's last pack
+      } else {
+        if ((flush[(size_t)k] >> lane) & 1u) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: duplicate record");
+        flush[(size_t)k] |= uint64_t(1) << lane;
+        rec_rows.push_back((int32_t)row);
+      }
+    }
+  }
+  // records must come in (pack, lane) order inside a stream for the rank lookup
+  {
+    for (int64_t t = 0; t < S; ++t) {
+      const int64_t r0 = wb[t * (L + 2) + 1], r1 = t + 1 < S ? wb[(t + 1) * (L + 2) + 1] : nrec;
+      int64_t prev = -1;
+      for (int64_t q = r0; q < r1; ++q) {
+        const int64_t pos = a.vPack_vec_record[(size_t)(2 * q)];
+        if (pos <= prev) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: records not in position order");
+        prev = pos;
+      }
+    }
+  }
+  std::vector<int32_t> empty;
+  for (int64_t r = 0; r < m; ++r)
+    if (!done[(size_t)r]) empty.push_back((int32_t)r);
+  auto M = new_mat(RVVLA_FMT_CVR, dtype, m, n, nnz);
+  DevCVR& d = M->cvr;
+  d.nstreams = S;
+  d.nempty = (int64_t)empty.size();
+  d.pack_start = M->put_vec(pstart);
+  d.rec_start = M->put_vec(rstart);
+  d.flush = M->put_vec(flush);
+  d.rec_rows = M->put_vec(rec_rows);
+  d.tail_rows = M->put_vec(tails);
+  d.col = M->put_vec(a.vPack_vec_cols);
+  d.val = put_values(*M, dtype, a.vPack_vec_vals.data(), a.vPack_vec_vals.size());
+  d.empty_rows = M->put_vec(empty);
+  return M.release();
+}
+
+template <class V>
+rvvla_mat_t upload_vhcc(int dtype, const spMtxVHCC<V>& a) {
+  check_dtype_for(dtype, std::is_same<V, double>::value);
+  const vhcc_params P = a.get_params();
+  const int64_t m = (int64_t)a.get_num_rows(), n = (int64_t)a.get_num_cols();
+  const int64_t nvec = (int64_t)a.get_num_vectors(), T = (int64_t)a.thr_info().size();
+  if (P.vec_size != 64) throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "VHCC: vec_size must be 64 on gfx950");
+  const int nm = (int)P.n_masks, mb = 8;
+  if (nm != 6 || (int64_t)a.vals_arr().size() != nvec * 64 || a.col_arr().size() != a.vals_arr().size() ||
+      T < 1 || a.veceor().empty() || a.scan_mask().size() != (a.veceor().size() - 1) * (size_t)(nm + 1) * mb)
+    throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: array sizes do not match");
+  for (uint32_t c : a.col_arr())
+    if ((int64_t)c >= n) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: column out of range");
+  std::vector<uint64_t> eor((size_t)nvec, 0);
+  const size_t neor = a.veceor().size() - 1;
+  size_t nends = 0;
+  for (size_t k = 0; k < neor; ++k) {
+    const int64_t v = a.veceor()[k];
+    if (v < 0 || v >= nvec || (k > 0 && a.veceor()[k - 1] >= (uint32_t)v))
+      throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: row-end vector list");
+    uint64_t w = 0;
+    std::memcpy(&w, a.scan_mask().data() + (k * (nm + 1) + nm) * mb, 8);
+    if (!w) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: empty write mask");
+    eor[(size_t)v] = w;
+    nends += (size_t)__builtin_popcountll(w);
+  }
+  if (nends != a.row_arr().size()) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: row_arr length");
+  std::vector<int32_t> sv((size_t)T + 1), rb((size_t)T), fwd((size_t)T, -1);
+  std::vector<uint8_t> cont((size_t)T), pass((size_t)T, 0);
+  int64_t ends_before = 0, vprev = 0;
+  for (int64_t t = 0; t < T; ++t) {
+    const thr_info_t& I = a.thr_info()[(size_t)t];
+    if (I.start_vec != vprev || I.end_vec < I.start_vec || I.end_vec > nvec)
+      throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: partitions must tile the vectors in order");
+    sv[(size_t)t] = I.start_vec;
+    rb[(size_t)t] = (int32_t)ends_before;
+    for (int64_t v = I.start_vec; v < I.end_vec; ++v) ends_before += __builtin_popcountll(eor[(size_t)v]);
+    vprev = I.end_vec;
+    cont[(size_t)t] = I.last_row != I.overflow_row;
+    if (cont[(size_t)t] && (I.last_row < 0 || I.last_row >= m))
+      throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: last_row out of range");
+  }
+  if (vprev != nvec) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: partitions do not cover the vectors");
+  sv[(size_t)T] = (int32_t)nvec;
+  for (uint32_t r : a.row_arr())
+    if ((int64_t)r >= m + T) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: row_arr out of range");
+  for (int64_t t = 0; t < T; ++t) {
+    const bool nxt = t + 1 < T && cont[(size_t)t + 1];
+    const int nrow = nxt ? a.thr_info()[(size_t)t + 1].last_row : -1;
+    pass[(size_t)t] = cont[(size_t)t] && nxt && a.thr_info()[(size_t)t].last_row == nrow;
+    fwd[(size_t)t] = (nxt && !pass[(size_t)t]) ? nrow : -1;
+  }
+  // rows that never receive a value: empty rows
+  std::vector<char> hit((size_t)m, 0);
+  for (uint32_t r : a.row_arr())
+    if ((int64_t)r < m) hit[r] = 1;
+  std::vector<int32_t> empty;
+  for (int64_t r = 0; r < m; ++r)
+    if (!hit[(size_t)r]) empty.push_back((int32_t)r);
+  auto M = new_mat(RVVLA_FMT_VHCC, dtype, m, n, (int64_t)a.get_num_entries());
+  DevVHCC& d = M->vhcc;
+  d.nparts = T;
+  d.nempty = (int64_t)empty.size();
+  d.start_vec = M->put_vec(sv);
+  d.rbase = M->put_vec(rb);
+  d.eor = M->put_vec(eor);
+  d.row_arr = M->put_vec(a.row_arr());
+  d.fwd_row = M->put_vec(fwd);
+  d.pass = M->put_vec(pass);
+  d.col = M->put_vec(a.col_arr());
+  d.val = put_values(*M, dtype, a.vals_arr().data(), a.vals_arr().size());
+  d.empty_rows = M->put_vec(empty);
+  const size_t acc = dtype == RVVLA_F64 ? 8 : 4;
+  d.scratch = M->put(nullptr, (size_t)T * 2 * acc + 16);
+  M->device_bytes -= (int64_t)((size_t)T * 2 * acc + 16);
+  return M.release();
+}
+
+template <class V>
+rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
+  check_dtype_for(dtype, std::is_same<V, double>::value);
+  const int64_t L = (int64_t)a.SIMD_Lanes, m = a.m, n = a.n, S = (int64_t)a.segment_count;
+  const int64_t cc = (int64_t)a.chunk_count;
+  if (L != 64) throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "LAV: SIMD_Lanes must be 64 on gfx950");
+  const spMtxCRS<V>& sp = a.sparse_part;
+  if ((int64_t)sp.m != m || (int64_t)sp.n != n || cc != (m + L - 1) / L ||
+      (int64_t)a.chunk_offsets.size() != S * (cc + 1) ||
+      (int64_t)a.chunk_offsets_full_mask.size() != S * cc || (int64_t)a.segment_ptr.size() < S ||
+      (int64_t)a.out_order.size() != S * cc * L || a.val.size() != a.col_id.size() ||
+      a.val.size() != a.mask.size() * (size_t)L)
+    throw Error(RVVLA_ERR_INVALID_ARG, "LAV: array sizes do not match");
+  check_csr(m, n, (int64_t)sp.nz, sp.Rst, sp.Col);
+  const int64_t total = (int64_t)a.mask.size();
+  for (uint32_t c : a.col_id)
+    if ((int64_t)c >= n) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: column out of range");
+  {
+    std::vector<char> seen((size_t)m, 0);
+    for (int64_t i = 0; i < m; ++i) {
+      const uint32_t r = a.out_order[(size_t)i];
+      if ((int64_t)r >= m || seen[r]) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: out_order is not a row permutation");
+      seen[r] = 1;
+    }
+    for (int64_t i = m; i < cc * L; ++i)
+      if ((int64_t)a.out_order[(size_t)i] >= m) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: out_order out of range");
+    for (int64_t g = 1; g < S; ++g)
+      if (!std::equal(a.out_order.begin(), a.out_order.begin() + cc * L,
+                      a.out_order.begin() + g * cc * L))
+        throw Error(RVVLA_ERR_UNSUPPORTED_SPMV,
+                    "LAV: the gfx950 kernel needs the same row order in every segment");
+  }
+  std::vector<int64_t> sptr((size_t)S);
+  std::vector<V> vals(a.val);
+  for (int64_t g = 0; g < S; ++g) {
+    sptr[(size_t)g] = (int64_t)a.segment_ptr[(size_t)g];
+    for (int64_t c = 0; c < cc; ++c) {
+      const int64_t k0 = a.chunk_offsets[(size_t)(g * (cc + 1) + c)];
+      const int64_t k1 = a.chunk_offsets[(size_t)(g * (cc + 1) + c + 1)];
+      const int64_t kf = a.chunk_offsets_full_mask[(size_t)(g * cc + c)];
+      if (k0 < 0 || k1 < k0 || kf < k0 || kf > k1 || sptr[(size_t)g] + k1 > total)
+        throw Error(RVVLA_ERR_INVALID_ARG, "LAV: chunk offsets out of range");
+      for (int64_t k = kf; k < k1; ++k) {
+        const int64_t step = sptr[(size_t)g] + k;
+        for (int64_t i = a.mask[(size_t)step]; i < L; ++i) vals[(size_t)(step * L + i)] = V(0);
+      }
+    }
+  }
+  auto M = new_mat(RVVLA_FMT_LAV, dtype, m, n, a.nz);
+  const std::vector<int32_t> blocks = csr_blocks(m, sp.Rst);
+  M->csr.rst = (const int32_t*)M->put(sp.Rst, (sp.m + 1) * sizeof(int));
+  M->csr.col = (const int32_t*)M->put(sp.Col, sp.nz * sizeof(int));
+  M->csr.val = put_values(*M, dtype, sp.Val, sp.nz);
+  M->csr.blocks = M->put_vec(blocks);
+  M->csr.nblocks = (int64_t)blocks.size() - 1;
+  DevLAV& d = M->lav;
+  d.nseg = S;
+  d.nchunks = cc;
+  d.coff = M->put_vec(a.chunk_offsets);
+  d.sptr = M->put_vec(sptr);
+  d.out_order = (const uint32_t*)M->put(a.out_order.data(), (size_t)(cc * L) * 4);
+  d.col = M->put_vec(a.col_id);
+  d.val = put_values(*M, dtype, vals.data(), vals.size());
+  return M.release();
+}
+
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
 RVVLA_INST_UPLOAD(upload_csr, spMtxCRS)
 RVVLA_INST_UPLOAD(upload_hcsr, spMtxHCSR)
 RVVLA_INST_UPLOAD(upload_sell, spMtxSELL_C_Sigma)
+RVVLA_INST_UPLOAD(upload_csr5, spMtxCSR5)
+RVVLA_INST_UPLOAD(upload_vhcc, spMtxVHCC)
+RVVLA_INST_UPLOAD(upload_lav, spMtxLAV)
+template rvvla_mat_t upload_cvr<double>(int, const spMtxCVR<double>&, int64_t);
+template rvvla_mat_t upload_cvr<float>(int, const spMtxCVR<float>&, int64_t);
 
 }  // namespace rvvla
 
@@ -424,6 +695,94 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
                    {"cnt_b", (double)s->cnt_b}, {"C", (double)s->C}, {"Sigma", (double)s->Sigma}};
       auto* sp = s.get();
       L.upload = [sp, dtype]() { return upload_sell<V>(dtype, *sp); };
+      break;
+    }
+    case RVVLA_FMT_CSR5: {
+      auto c = std::make_shared<spMtxCSR5<V>>();
+      build_csr5<V>(*c, *crs, param_int(P, "omega", 64), param_int(P, "sigma", 0));
+      L.obj = c;
+      reg(L, "_csr_row_pointer", c->_csr_row_pointer);
+      reg(L, "_csr_column_index", c->_csr_column_index);
+      reg(L, "_csr_value", c->_csr_value);
+      reg(L, "_csr5_tile_pointer", c->_csr5_tile_pointer);
+      reg(L, "_csr5_tile_descriptor", c->_csr5_tile_descriptor);
+      reg(L, "_csr5_tile_empty_offset_pointer", c->_csr5_tile_empty_offset_pointer);
+      reg(L, "_csr5_tile_empty_offset", c->_csr5_tile_empty_offset);
+      L.scalars = {{"_m", (double)c->_m}, {"_n", (double)c->_n}, {"_nnz", (double)c->_nnz},
+                   {"_p", (double)c->_p}, {"_omega", (double)c->_omega},
+                   {"_sigma", (double)c->_sigma}, {"_bit_y_offset", (double)c->_bit_y_offset},
+                   {"_bit_seg_offset", (double)c->_bit_seg_offset},
+                   {"_tile_desc_len", (double)c->_tile_desc_len},
+                   {"_tail_tile_start", (double)c->_tail_tile_start},
+                   {"_num_offsets", (double)c->_num_offsets}};
+      auto* cp = c.get();
+      L.upload = [cp, dtype]() { return upload_csr5<V>(dtype, *cp); };
+      break;
+    }
+    case RVVLA_FMT_CVR: {
+      auto c = std::make_shared<spMtxCVR<V>>();
+      build_cvr<V>(*c, *crs, param_int(P, "SIMD_LEN", 64),
+                   param_int(P, "streams", default_parts((int64_t)crs->m, (int64_t)crs->nz, 2048)));
+      L.obj = c;
+      reg(L, "vPack_vec_vals", c->vPack_vec_vals);
+      reg(L, "vPack_vec_cols", c->vPack_vec_cols);
+      reg(L, "vPack_vec_record", c->vPack_vec_record);
+      reg(L, "writeback_pos_record", c->writeback_pos_record);
+      L.scalars = {{"nItems", (double)c->nItems}, {"numRows", (double)c->numRows},
+                   {"numCols", (double)c->numCols}, {"SIMD_LEN", (double)c->SIMD_LEN},
+                   {"Nthreads", (double)c->Nthreads}, {"Nthreads_avail", (double)c->Nthreads_avail}};
+      auto* cp = c.get();
+      const int64_t nz = (int64_t)crs->nz;
+      L.upload = [cp, dtype, nz]() { return upload_cvr<V>(dtype, *cp, nz); };
+      break;
+    }
+    case RVVLA_FMT_VHCC: {
+      auto c = std::make_shared<spMtxVHCC<V>>();
+      build_vhcc<V>(*c, *crs, param_int(P, "vec_size", 64),
+                    param_int(P, "threads", default_parts((int64_t)crs->m, (int64_t)crs->nz, 4096)));
+      L.obj = c;
+      reg(L, "_veceor_ptr", c->veceor());
+      reg(L, "_scan_mask", c->scan_mask());
+      reg(L, "_row_arr", c->row_arr());
+      reg(L, "_col_arr", c->col_arr());
+      reg(L, "_vals_arr", c->vals_arr());
+      {
+        const auto& ti = c->thr_info();
+        L.arrays["_thr_info"] = ArrayView{ti.data(), (int64_t)(ti.size() * 10), 4};
+      }
+      L.scalars = {{"num_rows", (double)c->get_num_rows()}, {"num_cols", (double)c->get_num_cols()},
+                   {"num_entries", (double)c->get_num_entries()},
+                   {"num_vectors", (double)c->get_num_vectors()},
+                   {"pad_rows", (double)c->get_pad_rows()},
+                   {"num_threads", (double)c->get_params().num_threads},
+                   {"vec_size", (double)c->get_params().vec_size},
+                   {"n_masks", (double)c->get_n_masks()}};
+      auto* cp = c.get();
+      L.upload = [cp, dtype]() { return upload_vhcc<V>(dtype, *cp); };
+      break;
+    }
+    case RVVLA_FMT_LAV: {
+      auto c = std::make_shared<spMtxLAV<V>>();
+      build_lav<V>(*c, *crs, param_int(P, "SIMD_Lanes", 64), param_double(P, "T", 0.2),
+                   param_int(P, "segment_len", 1 << 20));
+      L.obj = c;
+      reg(L, "chunk_offsets", c->chunk_offsets);
+      reg(L, "chunk_offsets_full_mask", c->chunk_offsets_full_mask);
+      reg(L, "segment_ptr", c->segment_ptr);
+      reg(L, "out_order", c->out_order);
+      reg(L, "mask", c->mask);
+      reg(L, "val", c->val);
+      reg(L, "col_id", c->col_id);
+      const spMtxCRS<V>& sp = c->sparse_part;
+      L.arrays["sparse_part.Rst"] = ArrayView{sp.Rst, (int64_t)sp.m + 1, 4};
+      L.arrays["sparse_part.Col"] = ArrayView{sp.Col, (int64_t)sp.nz, 4};
+      L.arrays["sparse_part.Val"] = ArrayView{sp.Val, (int64_t)sp.nz, (int)sizeof(V)};
+      L.scalars = {{"SIMD_Lanes", (double)c->SIMD_Lanes}, {"m", (double)c->m}, {"n", (double)c->n},
+                   {"nz", (double)c->nz}, {"segment_count", (double)c->segment_count},
+                   {"chunk_count", (double)c->chunk_count}, {"T", c->T},
+                   {"segment_len", (double)c->segment_len}, {"sparse_nz", (double)sp.nz}};
+      auto* cp = c.get();
+      L.upload = [cp, dtype]() { return upload_lav<V>(dtype, *cp); };
       break;
     }
     default:
@@ -652,6 +1011,10 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
     case RVVLA_FMT_CSR: return rvvla::launch_csr(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_HCSR: return rvvla::launch_hcsr(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_SELL: return rvvla::launch_sell(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_CSR5: return rvvla::launch_csr5(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_CVR: return rvvla::launch_cvr(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_VHCC: return rvvla::launch_vhcc(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_LAV: return rvvla::launch_lav(*A, alpha, x, beta, y, s);
   }
   return RVVLA_ERR_UNKNOWN_FORMAT;
 }

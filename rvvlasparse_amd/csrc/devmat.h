@@ -51,6 +51,64 @@ struct DevSELL {
   const void* val = nullptr;
 };
 
+// CSR5 (omega = 64): tiles as the host container holds them, plus the
+// empty-row list and per-tile scratch (leading / trailing partial sums and
+// the trailing segment's row).
+struct DevCSR5 {
+  int64_t ntiles = 0, nempty = 0, tail_base = 0;
+  int sigma = 0, dl = 0, bit_y = 0, bit_seg = 0, tail_start = 0, first_tail_row = 0;
+  const uint32_t* tile_ptr = nullptr;
+  const uint32_t* desc = nullptr;
+  const uint32_t* eptr = nullptr;
+  const uint32_t* eoff = nullptr;
+  const int32_t* rst = nullptr;
+  const uint32_t* col = nullptr;
+  const void* val = nullptr;
+  const int32_t* empty_rows = nullptr;
+  void* scratch = nullptr;
+};
+
+// CVR (SIMD_LEN = 64): one stream per wavefront; records re-laid as a
+// flush mask per pack plus the flushed rows in record order.
+struct DevCVR {
+  int64_t nstreams = 0, nempty = 0;
+  const int64_t* pack_start = nullptr;  // nstreams + 1
+  const int32_t* rec_start = nullptr;   // nstreams
+  const uint64_t* flush = nullptr;      // one mask per pack
+  const int32_t* rec_rows = nullptr;
+  const int32_t* tail_rows = nullptr;   // 64 per stream, -1 = none
+  const uint32_t* col = nullptr;
+  const void* val = nullptr;
+  const int32_t* empty_rows = nullptr;
+};
+
+// VHCC (vec_size = 64): one partition per wavefront; the row-end mask of
+// every vector, the target rows, and the partition hand-over description.
+struct DevVHCC {
+  int64_t nparts = 0, nempty = 0;
+  const int32_t* start_vec = nullptr;  // nparts + 1
+  const int32_t* rbase = nullptr;
+  const uint64_t* eor = nullptr;       // one row-end mask per vector
+  const uint32_t* row_arr = nullptr;   // >= m: the continued row's partial
+  const int32_t* fwd_row = nullptr;
+  const uint8_t* pass = nullptr;
+  const uint32_t* col = nullptr;
+  const void* val = nullptr;
+  const int32_t* empty_rows = nullptr;
+  void* scratch = nullptr;             // first_part[nparts], fwd_part[nparts]
+};
+
+// LAV (SIMD_Lanes = 64): the dense part's chunk tables (the sparse part
+// lives in rvvla_mat_s::csr).
+struct DevLAV {
+  int64_t nseg = 0, nchunks = 0;
+  const int32_t* coff = nullptr;       // nseg * (nchunks + 1) step offsets
+  const int64_t* sptr = nullptr;       // nseg segment bases (in steps)
+  const uint32_t* out_order = nullptr; // nchunks * 64 rows (shared by all segments)
+  const uint32_t* col = nullptr;
+  const void* val = nullptr;
+};
+
 }  // namespace rvvla
 
 struct rvvla_mat_s {
@@ -63,6 +121,10 @@ struct rvvla_mat_s {
   rvvla::DevCSR csr;
   rvvla::DevHCSR hcsr;
   rvvla::DevSELL sell;
+  rvvla::DevCSR5 csr5;
+  rvvla::DevCVR cvr;
+  rvvla::DevVHCC vhcc;
+  rvvla::DevLAV lav;
   // staging for rvvla_spmv_host
   void* xstage = nullptr;
   void* ystage = nullptr;
@@ -90,5 +152,13 @@ int launch_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double beta, 
                 hipStream_t s);
 int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s);
+int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s);
+int launch_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+               hipStream_t s);
+int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s);
+int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+               hipStream_t s);
 
 }  // namespace rvvla

@@ -55,6 +55,8 @@ void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCR
 int default_sell_sigma(int64_t m, int64_t nnz);
 int default_csr5_sigma(int64_t m, int64_t nnz);
 int default_parts(int64_t m, int64_t nnz, int64_t items_per_part);
+// Rows [b[t], b[t+1]) of part t: contiguous, about nnz/parts entries each.
+std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts);
 
 // ---- device matrices ----------------------------------------------------
 // Upload of each format's host container; value arrays are given as V
@@ -70,7 +72,7 @@ rvvla_mat_t upload_csr5(int dtype, const SparseMatrixLib::spMtxCSR5<V>& a);
 template <class V>
 rvvla_mat_t upload_lav(int dtype, const SparseMatrixLib::spMtxLAV<V>& a);
 template <class V>
-rvvla_mat_t upload_cvr(int dtype, const SparseMatrixLib::spMtxCVR<V>& a);
+rvvla_mat_t upload_cvr(int dtype, const SparseMatrixLib::spMtxCVR<V>& a, int64_t nnz);
 template <class V>
 rvvla_mat_t upload_vhcc(int dtype, const SparseMatrixLib::spMtxVHCC<V>& a);
 template <class V>

@@ -1,0 +1,210 @@
+// k_csr5.hip -- CSR5 SpMV for gfx950 (omega = 64: one tile lane per
+// wavefront lane).
+//
+// Semantics follow the reference vector kernel (reference
+// src/rvv/spmv_mtx_csr5.cpp:85-296 compute, :318-335 calibrate, :337-365
+// tail): per tile, lane partial sums split at row-start flags, a segmented
+// sum across lanes, and a CSR pass over the tail rows.  The reference fixes
+// rows that cross tile boundaries with per-thread calibrators; here every
+// row is finished by exactly one writer, so no atomics are needed and the
+// result is deterministic:
+//  * k_csr5_tiles (one wavefront per full tile): rows that start and end
+//    inside the tile are written directly; the tile's leading segment
+//    (before its first counted flag) goes to first_part[t], and the segment
+//    opened by its last flag to last_part[t] / last_row[t].
+//  * k_csr5_fixup (one thread per full tile, then one per tail row, then
+//    one per empty row): the owner of each open segment walks the following
+//    tiles' leading segments until the row ends (adding the tail entries if
+//    it runs into the tail), tail rows are summed as CSR, empty rows get
+//    alpha*0 + beta*y.
+#include "kcommon.h"
+
+namespace rvvla {
+
+struct Csr5Args {
+  const uint32_t* tile_ptr;
+  const uint32_t* desc;
+  const uint32_t* eptr;
+  const uint32_t* eoff;
+  const int32_t* rst;
+  const uint32_t* col;
+  const int32_t* empty_rows;
+  int64_t ntiles, nempty, m;
+  int sigma, dl, bit_y, bit_seg, tail_start, first_tail_row;
+  int64_t tail_base;  // (p-1)*omega*sigma
+};
+
+// flag bit i (0-based) of one lane, from its descriptor words
+__device__ __forceinline__ uint64_t csr5_flags(const uint32_t* w, int sigma, int f0) {
+  uint64_t fl = 0;
+  for (int i = 0; i < sigma; ++i) {
+    uint32_t bit;
+    if (i < f0) bit = (w[0] >> (31 - (32 - f0) - i)) & 1u;
+    else bit = (w[1 + (i - f0) / 32] >> (31 - (i - f0) % 32)) & 1u;
+    fl |= (uint64_t)bit << i;
+  }
+  return fl;
+}
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_csr5_tiles(
+    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ first_part,
+    typename Val<VT>::acc* __restrict__ last_part, int32_t* __restrict__ last_row,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (t >= a.ntiles) return;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
+  const int sigma = a.sigma, f0 = 32 - a.bit_y - a.bit_seg;
+  const uint32_t tp = a.tile_ptr[t];
+  const bool has_empty = (tp >> 31) != 0;
+  const int64_t row0 = tp & 0x7FFFFFFFu;
+  uint32_t w[3] = {0, 0, 0};
+  const uint32_t* dp = a.desc + t * 64 * a.dl + lane;
+  for (int k = 0; k < a.dl; ++k) w[k] = dp[k * 64];
+  const uint32_t yoff = w[0] >> (32 - a.bit_y);
+  uint64_t fl = csr5_flags(w, sigma, f0);
+  const bool starts_here = lane == 0 && (fl & 1u);  // tile_ptr[t] begins at this tile
+  if (lane == 0) fl &= ~uint64_t(1);
+  const uint32_t ebase = has_empty ? a.eptr[t] : 0u;
+  auto row_of = [&](uint32_t ord) -> int64_t {  // ord: 1-based counted-flag ordinal
+    return has_empty ? row0 + 1 + (int64_t)a.eoff[ebase + ord - 1] : row0 + (int64_t)ord;
+  };
+  A head = 0, cur = 0;
+  uint32_t nfl = 0;
+  const int64_t base = t * 64 * (int64_t)sigma + lane;
+  for (int i = 0; i < sigma; ++i) {
+    const int64_t e = base + (int64_t)i * 64;
+    const A p = Val<VT>::get(val, e) * (A)x[a.col[e]];
+    if ((fl >> i) & 1u) {
+      if (nfl == 0) {
+        head = cur;
+      } else {
+        const int64_t r = row_of(yoff + nfl);
+        y[r] = alpha * cur + scaled_y(beta, y, r);
+      }
+      cur = 0;
+      ++nfl;
+    }
+    cur += p;
+  }
+  const bool flagged = nfl > 0;
+  if (!flagged) head = cur;
+  // U[l] = head[l] + head[l+1] + ... up to and including the next flagged
+  // lane (segmented suffix scan; a flagged lane closes its own chain)
+  A U = head;
+  int closed = flagged ? 1 : 0;
+  for (int d = 1; d < 64; d <<= 1) {
+    const A ov = __shfl_down(U, d, 64);
+    const int oc = __shfl_down(closed, d, 64);
+    if (!closed && lane + d < 64) {
+      U += ov;
+      closed |= oc;
+    }
+  }
+  A cont = __shfl_down(U, 1, 64);
+  if (lane == 63) cont = 0;
+  const uint64_t mask = __ballot(flagged);
+  const int last_lane = mask ? 63 - __builtin_clzll(mask) : -1;
+  if (flagged) {
+    const A s = (cur) + cont;
+    const int64_t r = row_of(yoff + nfl);
+    if (lane != last_lane) {
+      y[r] = alpha * s + scaled_y(beta, y, r);
+    } else {
+      last_part[t] = s;
+      last_row[t] = (int32_t)r;
+    }
+  }
+  if (lane == 0) {
+    if (starts_here && mask) {
+      y[row0] = alpha * U + scaled_y(beta, y, row0);
+    } else if (starts_here) {
+      last_part[t] = U;
+      last_row[t] = (int32_t)row0;
+    } else {
+      first_part[t] = U;
+      if (!mask) last_row[t] = -1;
+    }
+  }
+}
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_csr5_fixup(
+    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
+    const typename Val<VT>::acc* __restrict__ last_part, const int32_t* __restrict__ last_row,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  const int f0 = 32 - a.bit_y - a.bit_seg;
+  const int64_t ntail = a.m - a.first_tail_row;
+  if (i < a.ntiles) {
+    const int32_t r = last_row[i];
+    if (r < 0) return;
+    A s = last_part[i];
+    int64_t q = i + 1;
+    for (; q < a.ntiles; ++q) {
+      const uint32_t w0 = a.desc[q * 64 * a.dl];  // lane 0, word 0
+      if ((w0 >> (31 - (32 - f0))) & 1u) break;    // tile q starts a new row
+      s += first_part[q];
+      if (last_row[q] >= 0) break;                 // the row ends inside tile q
+    }
+    if (q == a.ntiles) {  // ran past the full tiles: the row may go on into the tail
+      const int64_t e1 = a.rst[r + 1];
+      for (int64_t e = a.tail_base; e < e1; ++e) s += Val<VT>::get(val, e) * (A)x[a.col[e]];
+    }
+    y[r] = alpha * s + scaled_y(beta, y, r);
+  } else if (i < a.ntiles + ntail) {
+    const int64_t r = a.first_tail_row + (i - a.ntiles);
+    const int64_t e0 = a.rst[r], e1 = a.rst[r + 1];
+    if (e0 == e1) return;  // empty rows are in the empty list
+    A s = 0;
+    for (int64_t e = e0; e < e1; ++e) s += Val<VT>::get(val, e) * (A)x[a.col[e]];
+    y[r] = alpha * s + scaled_y(beta, y, r);
+  } else if (i < a.ntiles + ntail + a.nempty) {
+    const int64_t r = a.empty_rows[i - a.ntiles - ntail];
+    y[r] = scaled_y(beta, y, r);
+  }
+}
+
+template <class VT>
+static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                    hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevCSR5& d = A.csr5;
+  Csr5Args a{d.tile_ptr, d.desc, d.eptr, d.eoff, d.rst, d.col, d.empty_rows, d.ntiles, d.nempty,
+             A.m, d.sigma, d.dl, d.bit_y, d.bit_seg, d.tail_start, d.first_tail_row, d.tail_base};
+  AC* fp = (AC*)d.scratch;
+  AC* lp = fp + d.ntiles;
+  int32_t* lr = (int32_t*)(lp + d.ntiles);
+  if (d.ntiles > 0) {
+    const unsigned g1 = (unsigned)((d.ntiles + kWG / 64 - 1) / (kWG / 64));
+    hipLaunchKernelGGL((k_csr5_tiles<VT>), dim3(g1), dim3(kWG), 0, s, a, (const VT*)d.val,
+                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    if (int rc = hip_check(hipGetLastError(), "k_csr5_tiles launch")) return rc;
+  }
+  const int64_t nfix = d.ntiles + (A.m - d.first_tail_row) + d.nempty;
+  if (nfix > 0) {
+    const unsigned g2 = (unsigned)((nfix + kWG - 1) / kWG);
+    hipLaunchKernelGGL((k_csr5_fixup<VT>), dim3(g2), dim3(kWG), 0, s, a, (const VT*)d.val,
+                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    return hip_check(hipGetLastError(), "k_csr5_fixup launch");
+  }
+  return RVVLA_OK;
+}
+
+int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_csr5<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_csr5<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_csr5<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_csr5<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

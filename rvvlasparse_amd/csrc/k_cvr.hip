@@ -1,0 +1,94 @@
+// k_cvr.hip -- CVR SpMV for gfx950 (SIMD_LEN = 64: one stream per
+// wavefront, one CVR lane per wavefront lane).
+//
+// Semantics follow the reference scalar kernel (reference
+// src/scalar/spmv_mtx_cvr.cpp:32-66): every lane accumulates the entries of
+// its current row in pack order; when a record says the lane switches rows
+// before pack k, the lane's sum is flushed into that row; at the stream's
+// end the remaining lanes flush into their open rows.  Each non-empty row
+// is flushed exactly once, so the flush writes y = alpha*acc + beta*y
+// directly (the reference scales y by beta first); empty rows come from a
+// separate list.  The records are re-laid at upload as one 64-bit flush
+// mask per pack (a uniform scalar load) plus the flushed rows in record
+// order; a lane finds its row by counting the flushing lanes below it.
+#include "kcommon.h"
+
+namespace rvvla {
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_cvr(
+    const int64_t* __restrict__ pack_start, const int32_t* __restrict__ rec_start,
+    const uint64_t* __restrict__ flush, const int32_t* __restrict__ rec_rows,
+    const int32_t* __restrict__ tail_rows, const uint32_t* __restrict__ col,
+    const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, int64_t nstreams, typename Val<VT>::acc alpha,
+    typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (t >= nstreams) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  const int64_t k0 = pack_start[t], k1 = pack_start[t + 1];
+  int32_t rb = rec_start[t];
+  A acc = 0;
+  for (int64_t k = k0; k < k1; ++k) {
+    const uint64_t mk = flush[k];
+    if (mk) {
+      if ((mk >> lane) & 1u) {
+        const int32_t r = rec_rows[rb + __popcll(mk & below)];
+        y[r] = alpha * acc + scaled_y(beta, y, r);
+        acc = 0;
+      }
+      rb += __popcll(mk);
+    }
+    const int64_t e = k * 64 + lane;
+    acc += Val<VT>::get(val, e) * (A)x[col[e]];
+  }
+  const int32_t r = tail_rows[t * 64 + lane];
+  if (r >= 0) y[r] = alpha * acc + scaled_y(beta, y, r);
+}
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_scale_rows(const int32_t* __restrict__ rows, int64_t n,
+                                                     typename Val<VT>::io* __restrict__ y,
+                                                     typename Val<VT>::acc beta) {
+  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (i < n) {
+    const int32_t r = rows[i];
+    y[r] = scaled_y(beta, y, r);
+  }
+}
+
+template <class VT>
+static int run_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                   hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevCVR& d = A.cvr;
+  if (d.nstreams > 0) {
+    const unsigned g = (unsigned)((d.nstreams + kWG / 64 - 1) / (kWG / 64));
+    hipLaunchKernelGGL((k_cvr<VT>), dim3(g), dim3(kWG), 0, s, d.pack_start, d.rec_start, d.flush,
+                       d.rec_rows, d.tail_rows, d.col, (const VT*)d.val, (const IO*)x, (IO*)y,
+                       d.nstreams, (AC)alpha, (AC)beta);
+    if (int rc = hip_check(hipGetLastError(), "k_cvr launch")) return rc;
+  }
+  if (d.nempty > 0) {
+    hipLaunchKernelGGL((k_scale_rows<VT>), dim3((unsigned)((d.nempty + kWG - 1) / kWG)), dim3(kWG),
+                       0, s, d.empty_rows, d.nempty, (IO*)y, (AC)beta);
+    return hip_check(hipGetLastError(), "k_scale_rows launch");
+  }
+  return RVVLA_OK;
+}
+
+int launch_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+               hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_cvr<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_cvr<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_cvr<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_cvr<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

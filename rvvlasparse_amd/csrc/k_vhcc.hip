@@ -1,0 +1,155 @@
+// k_vhcc.hip -- VHCC SpMV for gfx950 (vec_size = 64: one vector per
+// wavefront step, one partition per wavefront).
+//
+// Semantics follow the reference vector kernel (reference
+// src/rvv/spmv_vhcc.cpp:73-162 and its epilogue :281-313): vectors of the
+// CSR-ordered entry stream are multiplied lane-wise; at a vector holding row
+// ends, a segmented inclusive scan gives every ending row its sum, which is
+// written to row_arr's row; the open segment carries into lane 0 of the
+// next vector; rows crossing partitions are completed by a fix-up.
+// gfx950 form:
+//  * vectors without a row end add into lane-private sums (no cross-lane
+//    work); the next row-end vector folds them into its carry;
+//  * the segmented scan's masks follow from the row-end mask by bit
+//    arithmetic (lane j adds lane j-2^s iff j-2^s >= its segment start), so
+//    only the 64-bit row-end mask per vector is read;
+//  * instead of the reference's overflow rows and atomic fix-up, a partition
+//    stores the partial of the row it continues (first_part) and of the row
+//    it hands on (fwd_part); one owner thread per handed-on row adds the
+//    pieces in partition order (deterministic), y = alpha*sum + beta*y.
+#include "kcommon.h"
+
+namespace rvvla {
+
+struct VhccArgs {
+  const int32_t* start_vec;  // nparts + 1
+  const int32_t* rbase;
+  const uint64_t* eor;
+  const uint32_t* row_arr;
+  const int32_t* fwd_row;    // row handed on to the next partition, or -1
+  const uint8_t* pass;       // partition's only row end is the handed-on one
+  const uint32_t* col;
+  const int32_t* empty_rows;
+  int64_t nparts, nempty, m;
+};
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__ val,
+                                               const typename Val<VT>::io* __restrict__ x,
+                                               typename Val<VT>::io* __restrict__ y,
+                                               typename Val<VT>::acc* __restrict__ first_part,
+                                               typename Val<VT>::acc* __restrict__ fwd_part,
+                                               typename Val<VT>::acc alpha,
+                                               typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (t >= a.nparts) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  const uint64_t upto = below | (uint64_t(1) << lane);
+  const int64_t v0 = a.start_vec[t], v1 = a.start_vec[t + 1];
+  int64_t rank = a.rbase[t];
+  const bool has_fwd = a.fwd_row[t] >= 0;
+  const int64_t last_entry = v1 * 64 - 1;
+  A hist = 0, carry = 0;
+  bool pending = false;  // hist holds sums of vectors without row ends
+  for (int64_t v = v0; v < v1; ++v) {
+    const int64_t e = v * 64 + lane;
+    const A p = Val<VT>::get(val, e) * (A)x[a.col[e]];
+    const uint64_t mk = a.eor[v];
+    if (!mk) {
+      hist += p;
+      pending = true;
+      continue;
+    }
+    if (pending) {
+      A h = hist;
+      for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
+      carry += h;
+      hist = 0;
+      pending = false;
+    }
+    A s = p + (lane == 0 ? carry : A(0));
+    const uint64_t heads = 1u | (mk << 1);
+    const int hp = 63 - __builtin_clzll(heads & upto);  // this lane's segment start
+    for (int d = 1; d < 64; d <<= 1) {
+      const A o = __shfl_up(s, d, 64);
+      if (lane - d >= hp) s += o;
+    }
+    if ((mk >> lane) & 1u) {
+      const uint32_t r = a.row_arr[rank + __popcll(mk & below)];
+      if (r >= (uint32_t)a.m) {
+        first_part[t] = s;  // the row this partition continues
+      } else if (has_fwd && e == last_entry) {
+        fwd_part[t] = s;    // the row handed on to the next partition
+      } else {
+        y[r] = alpha * s + scaled_y(beta, y, r);
+      }
+    }
+    rank += __popcll(mk);
+    const A last = __shfl(s, 63, 64);
+    carry = (mk >> 63) ? A(0) : last;
+  }
+}
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_vhcc_fixup(VhccArgs a,
+                                                     typename Val<VT>::io* __restrict__ y,
+                                                     const typename Val<VT>::acc* __restrict__ first_part,
+                                                     const typename Val<VT>::acc* __restrict__ fwd_part,
+                                                     typename Val<VT>::acc alpha,
+                                                     typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (i < a.nparts) {
+    const int32_t r = a.fwd_row[i];
+    if (r < 0) return;
+    A s = fwd_part[i];
+    for (int64_t q = i + 1; q < a.nparts; ++q) {
+      s += first_part[q];
+      if (!a.pass[q]) break;
+    }
+    y[r] = alpha * s + scaled_y(beta, y, r);
+  } else if (i < a.nparts + a.nempty) {
+    const int32_t r = a.empty_rows[i - a.nparts];
+    y[r] = scaled_y(beta, y, r);
+  }
+}
+
+template <class VT>
+static int run_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                    hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevVHCC& d = A.vhcc;
+  VhccArgs a{d.start_vec, d.rbase, d.eor, d.row_arr, d.fwd_row, d.pass, d.col, d.empty_rows,
+             d.nparts, d.nempty, A.m};
+  AC* fp = (AC*)d.scratch;
+  AC* wp = fp + d.nparts;
+  if (d.nparts > 0) {
+    const unsigned g = (unsigned)((d.nparts + kWG / 64 - 1) / (kWG / 64));
+    hipLaunchKernelGGL((k_vhcc<VT>), dim3(g), dim3(kWG), 0, s, a, (const VT*)d.val, (const IO*)x,
+                       (IO*)y, fp, wp, (AC)alpha, (AC)beta);
+    if (int rc = hip_check(hipGetLastError(), "k_vhcc launch")) return rc;
+  }
+  const int64_t nfix = d.nparts + d.nempty;
+  if (nfix > 0) {
+    hipLaunchKernelGGL((k_vhcc_fixup<VT>), dim3((unsigned)((nfix + kWG - 1) / kWG)), dim3(kWG), 0,
+                       s, a, (IO*)y, fp, wp, (AC)alpha, (AC)beta);
+    return hip_check(hipGetLastError(), "k_vhcc_fixup launch");
+  }
+  return RVVLA_OK;
+}
+
+int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_vhcc<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_vhcc<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_vhcc<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_vhcc<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-FORMATS_READY = ["csr", "hcsr", "sell"]
+FORMATS_READY = ["csr", "hcsr", "sell", "csr5", "lav", "vhcc", "cvr"]
 
 
 def _device():
