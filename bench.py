@@ -164,7 +164,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-READY = ("csr", "hcsr", "sell", "csr5", "lav", "vhcc", "cvr")
+READY = ("csr", "hcsr", "sell", "csr5", "lav", "vnec", "vhcc", "cvr")
 
 if __name__ == "__main__":
     main()

@@ -523,9 +523,75 @@ void vhcc_spmv_refsem(int64_t m, int vec, int nparts, int64_t pad_rows, const in
   for (int64_t i = 0; i < m; ++i) y[i] = y[i] * beta + alpha * tmp[(size_t)i];
 }
 
+// reference src/rvv/spmv_mtx_vnec.cpp:42-71 (MergePathDivide).
+void merge_path_divide(int diagonal, const int* a, int a_len, int b_len, int* x, int* y) {
+  int x_min = std::max(diagonal - b_len, 0);
+  int x_max = std::min(diagonal, a_len);
+  while (x_min < x_max) {
+    const int pivot = (x_min + x_max) >> 1;
+    if (a[pivot] <= diagonal - pivot - 1) x_min = pivot + 1;  // b[k] = k
+    else x_max = pivot;
+  }
+  *x = std::min(x_min, a_len);
+  *y = diagonal - x_min;
+}
+
+// reference include/VNEC.h:78-190 (IRD_VNEC), one thread walking the
+// partitions in order (the reference's shared group_index is then
+// race-free): merge-path coordinates, per-partition column compaction
+// through the IDX_OFFSET prefix of not-null flags, lane groups.
+float vnec_ird(int m, int n, int nnz, const int* rst, const int* col, int nthreads, int nlanes,
+               int* coords) {
+  const int items = nnz + m;
+  const int per = (items + nthreads - 1) / nthreads;
+  std::vector<int> sx(nthreads), sy(nthreads), ex(nthreads), ey(nthreads);
+  for (int t = 0; t < nthreads; ++t) {
+    const int d0 = std::min(per * t, items), d1 = std::min(d0 + per, items);
+    merge_path_divide(d0, rst + 1, m, nnz, &sx[t], &sy[t]);
+    merge_path_divide(d1, rst + 1, m, nnz, &ex[t], &ey[t]);
+    if (coords) {
+      coords[4 * t] = sx[t];
+      coords[4 * t + 1] = sy[t];
+      coords[4 * t + 2] = ex[t];
+      coords[4 * t + 3] = ey[t];
+    }
+  }
+  std::vector<int> ecr((size_t)nnz + 10, 0);
+  std::vector<int> flag(n), off(n);
+  for (int t = 0; t < nthreads; ++t) {
+    for (int c = 0; c < n; ++c) flag[c] = 1;
+    for (int j = sy[t]; j < ey[t]; ++j) flag[col[j]] = 0;
+    off[0] = flag[0];
+    for (int c = 1; c < n; ++c) off[c] = off[c - 1] + flag[c];
+    for (int j = sy[t]; j < ey[t]; ++j) ecr[j] = col[j] - off[col[j]];
+  }
+  long long group = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    for (int i = sx[t]; i < ex[t]; ++i) {
+      const int start = rst[i] > sy[t] ? rst[i] : sy[t];
+      const int len = rst[i + 1] - start;
+      int first = ecr[start];
+      for (int j = 1; j < len; j++)
+        if (ecr[start + j] - first >= nlanes) {
+          ++group;
+          first = ecr[start + j];
+        }
+      if (len != 0) ++group;
+    }
+  }
+  return (float)nnz / (float)(group * nlanes);
+}
+
 }  // namespace
 
 extern "C" {
+
+// IRD of reference include/VNEC.h:78-190 and the merge-path coordinates
+// (start.x, start.y, end.x, end.y per partition) into coords (or NULL).
+float orc_vnec_ird(int m, int n, int nnz, const int* rst, const int* col, int nthreads, int nlanes,
+                   int* coords) {
+  return vnec_ird(m, n, nnz, rst, col, nthreads, nlanes, coords);
+}
 
 int orc_vhcc_spmv_f64(int64_t m, int vec, int nparts, int64_t pad_rows, const int* thr,
                       const uint32_t* veceor, const uint8_t* masks, const uint32_t* row_arr,

@@ -64,6 +64,7 @@ _ORC_SIGS = {
     "orc_obj_set_scalar": (_i32, [_vp, _cp, _d]),
     "orc_vhcc_spmv_f64": (_i32, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp]),
     "orc_vhcc_spmv_f32": (_i32, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _f, _vp]),
+    "orc_vnec_ird": (C.c_float, [_i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp]),
     "orc_num_threads": (_i32, []),
     "orc_set_threads": (None, [_i32]),
 }
@@ -280,6 +281,14 @@ def obj_from(arrays: dict, scalars: dict, sfx: str) -> Obj:
     o = Obj(h, orc.orc_obj_array, orc.orc_obj_free, dict(scalars))
     o.sfx = sfx
     return o
+
+
+def vnec_ird(m, n, rst, col, nthreads, nlanes):
+    """(IRD, coords[nthreads, 4]) as the reference's IRD_VNEC computes them
+    on one thread; coords rows are (start.x, start.y, end.x, end.y)."""
+    coords = np.zeros((nthreads, 4), dtype=np.int32)
+    ird = orc.orc_vnec_ird(m, n, col.shape[0], _p(rst), _p(col), nthreads, nlanes, _p(coords))
+    return ird, coords
 
 
 def cvr_spmv_ref(arrays: dict, scalars: dict, x, alpha=1.0, beta=0.0, y=None):

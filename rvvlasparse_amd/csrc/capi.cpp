@@ -606,6 +606,65 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   return M.release();
 }
 
+template <class V>
+rvvla_mat_t upload_vnec(int dtype, const spMtxVNEC<V>& a, const spMtxCRS<V>& src) {
+  check_dtype_for(dtype, std::is_same<V, double>::value);
+  const int64_t m = (int64_t)src.m, n = (int64_t)src.n, nnz = (int64_t)src.nz;
+  const int64_t P = a.NUM_THREADS_VNEC;
+  check_csr(m, n, nnz, src.Rst, src.Col);
+  if (P < 1 || (int64_t)a.thread_coord_start.size() != P || (int64_t)a.thread_coord_end.size() != P ||
+      (int64_t)a.nec_ptr.size() != P + 1 || (int64_t)a.ecr_indices.size() != nnz ||
+      (int64_t)a.use_x_indices.size() != a.nec_ptr[(size_t)P] || a.m != m || a.nnz != nnz)
+    throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: array sizes do not match");
+  std::vector<int32_t> cx((size_t)P + 1), cy((size_t)P + 1);
+  for (int64_t t = 0; t < P; ++t) {
+    cx[(size_t)t] = a.thread_coord_start[(size_t)t].x;
+    cy[(size_t)t] = a.thread_coord_start[(size_t)t].y;
+    if (t > 0 && (a.thread_coord_start[(size_t)t].x != a.thread_coord_end[(size_t)t - 1].x ||
+                  a.thread_coord_start[(size_t)t].y != a.thread_coord_end[(size_t)t - 1].y))
+      throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: partitions must be contiguous");
+  }
+  cx[(size_t)P] = a.thread_coord_end[(size_t)P - 1].x;
+  cy[(size_t)P] = a.thread_coord_end[(size_t)P - 1].y;
+  if (cx[0] != 0 || cy[0] != 0 || cx[(size_t)P] != m || cy[(size_t)P] != nnz)
+    throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: partitions must cover rows and entries");
+  std::vector<uint16_t> ecr((size_t)nnz);
+  for (int64_t t = 0; t < P; ++t) {
+    const int64_t x0 = cx[(size_t)t], x1 = cx[(size_t)t + 1], y0 = cy[(size_t)t], y1 = cy[(size_t)t + 1];
+    const int64_t nec = a.nec_ptr[(size_t)t + 1] - a.nec_ptr[(size_t)t];
+    if (x1 < x0 || y1 < y0 || y1 - y0 > kVnecItems || nec < 0 || nec > kVnecItems ||
+        x1 - x0 + 1 > (int64_t)1 << 30)
+      throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "VNEC: a partition exceeds the LDS staging size");
+    // the merge path: rows before x0 end at or before y0, row x0 has not ended
+    if ((x0 > 0 && src.Rst[x0] > y0) || (x0 < m && src.Rst[x0 + 1] < y0 && x1 > x0))
+      throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: coordinates are not on the merge path");
+    for (int64_t e = y0; e < y1; ++e) {
+      const int k = a.ecr_indices[(size_t)e];
+      if (k < 0 || k >= nec || a.use_x_indices[(size_t)(a.nec_ptr[(size_t)t] + k)] != src.Col[e])
+        throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: compacted column does not match");
+      ecr[(size_t)e] = (uint16_t)k;
+    }
+  }
+  for (int c : a.use_x_indices)
+    if (c < 0 || c >= n) throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: column out of range");
+  auto M = new_mat(RVVLA_FMT_VNEC, dtype, m, n, nnz);
+  DevVNEC& d = M->vnec;
+  d.nparts = P;
+  d.cx = M->put_vec(cx);
+  d.cy = M->put_vec(cy);
+  d.nec_ptr = M->put_vec(a.nec_ptr);
+  d.use_x = M->put_vec(a.use_x_indices);
+  d.ecr = M->put_vec(ecr);
+  d.rst = (const int32_t*)M->put(src.Rst, (size_t)(m + 1) * 4);
+  d.val = put_values(*M, dtype, src.Val, (size_t)nnz);
+  const size_t acc = dtype == RVVLA_F64 ? 8 : 4;
+  d.scratch = M->put(nullptr, (size_t)P * 2 * acc + 16);
+  M->device_bytes -= (int64_t)((size_t)P * 2 * acc + 16);
+  return M.release();
+}
+template rvvla_mat_t upload_vnec<double>(int, const spMtxVNEC<double>&, const spMtxCRS<double>&);
+template rvvla_mat_t upload_vnec<float>(int, const spMtxVNEC<float>&, const spMtxCRS<float>&);
+
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
 RVVLA_INST_UPLOAD(upload_csr, spMtxCRS)
@@ -783,6 +842,28 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
                    {"segment_len", (double)c->segment_len}, {"sparse_nz", (double)sp.nz}};
       auto* cp = c.get();
       L.upload = [cp, dtype]() { return upload_lav<V>(dtype, *cp); };
+      break;
+    }
+    case RVVLA_FMT_VNEC: {
+      auto c = std::make_shared<spMtxVNEC<V>>();
+      const int64_t items = (int64_t)crs->m + (int64_t)crs->nz;
+      const int dflt = (int)std::max<int64_t>(1, (items + kVnecItems - 1) / kVnecItems);
+      build_vnec<V>(*c, *crs, param_int(P, "threads", dflt));
+      L.obj = c;
+      reg(L, "ecr_indices", c->ecr_indices);
+      reg(L, "NEC_NUM", c->NEC_NUM);
+      reg(L, "nec_ptr", c->nec_ptr);
+      reg(L, "use_x_indices", c->use_x_indices);
+      L.arrays["thread_coord_start"] =
+          ArrayView{c->thread_coord_start.data(), (int64_t)c->thread_coord_start.size() * 2, 4};
+      L.arrays["thread_coord_end"] =
+          ArrayView{c->thread_coord_end.data(), (int64_t)c->thread_coord_end.size() * 2, 4};
+      L.scalars = {{"NUM_THREADS_VNEC", (double)c->NUM_THREADS_VNEC}, {"m", (double)c->m},
+                   {"n", (double)c->n}, {"nnz", (double)c->nnz}, {"IRD_mat", (double)c->IRD_mat},
+                   {"vnec_t", (double)(int)c->vnec_t}};
+      auto* cp = c.get();
+      auto* src = crs.get();
+      L.upload = [cp, src, dtype]() { return upload_vnec<V>(dtype, *cp, *src); };
       break;
     }
     default:
@@ -1015,6 +1096,7 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
     case RVVLA_FMT_CVR: return rvvla::launch_cvr(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_VHCC: return rvvla::launch_vhcc(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_LAV: return rvvla::launch_lav(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_VNEC: return rvvla::launch_vnec(*A, alpha, x, beta, y, s);
   }
   return RVVLA_ERR_UNKNOWN_FORMAT;
 }

@@ -109,6 +109,21 @@ struct DevLAV {
   const void* val = nullptr;
 };
 
+// VNEC: merge-path partitions of at most kVnecItems rows + entries, each
+// with its compact list of touched columns (staged in LDS).
+constexpr int kVnecItems = 2048;
+struct DevVNEC {
+  int64_t nparts = 0;
+  const int32_t* cx = nullptr;       // nparts + 1 merge-path row coordinates
+  const int32_t* cy = nullptr;       // nparts + 1 merge-path entry coordinates
+  const int32_t* nec_ptr = nullptr;  // nparts + 1
+  const int32_t* use_x = nullptr;
+  const uint16_t* ecr = nullptr;     // compacted column rank per entry
+  const int32_t* rst = nullptr;
+  const void* val = nullptr;
+  void* scratch = nullptr;           // first_part[nparts], fwd_part[nparts]
+};
+
 }  // namespace rvvla
 
 struct rvvla_mat_s {
@@ -125,6 +140,7 @@ struct rvvla_mat_s {
   rvvla::DevCVR cvr;
   rvvla::DevVHCC vhcc;
   rvvla::DevLAV lav;
+  rvvla::DevVNEC vnec;
   // staging for rvvla_spmv_host
   void* xstage = nullptr;
   void* ystage = nullptr;
@@ -160,5 +176,7 @@ int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, 
                 hipStream_t s);
 int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                hipStream_t s);
+int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s);
 
 }  // namespace rvvla

@@ -1,0 +1,125 @@
+// k_vnec.hip -- VNEC SpMV for gfx950: merge-path partitions with the
+// partition's non-empty columns of x staged in LDS.
+//
+// Semantics follow the reference VNEC kernels (reference
+// src/rvv/spmv_mtx_vnec.cpp:192-586): each merge-path partition gathers the
+// x values of the columns it touches into a compact buffer (there one per
+// thread, here in LDS), multiplies its entries through the compacted column
+// ranks (16-bit here), finishes the rows that end inside it, and hands the
+// partial sum of the row still open at its end on as a carry (reference
+// :357-361, :441-445, serially added).  The D/S/L variants of the reference
+// are lane shapes of a CPU SIMD unit; on gfx950 all three run this kernel.
+// Rows crossing partitions are finished by one owner thread that adds the
+// pieces in partition order (deterministic, no atomics):
+// y = alpha*sum + beta*y.
+#include "kcommon.h"
+
+namespace rvvla {
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_vnec(
+    const int32_t* __restrict__ cx, const int32_t* __restrict__ cy,
+    const int32_t* __restrict__ nec_ptr, const int32_t* __restrict__ use_x,
+    const uint16_t* __restrict__ ecr, const int32_t* __restrict__ rst,
+    const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ first_part,
+    typename Val<VT>::acc* __restrict__ fwd_part, int64_t m, typename Val<VT>::acc alpha,
+    typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  __shared__ A xs[kVnecItems];
+  __shared__ A prod[kVnecItems];
+  const int t = blockIdx.x, tid = threadIdx.x;
+  const int x0 = cx[t], x1 = cx[t + 1], y0 = cy[t], y1 = cy[t + 1];
+  const int k0 = nec_ptr[t], nec = nec_ptr[t + 1] - k0;
+  for (int k = tid; k < nec; k += kWG) xs[k] = (A)x[use_x[k0 + k]];
+  __syncthreads();
+  for (int e = y0 + tid; e < y1; e += kWG) prod[e - y0] = Val<VT>::get(val, e) * xs[ecr[e]];
+  __syncthreads();
+  const int nrows = x1 - x0 + (x1 < m ? 1 : 0);
+  auto finish = [&](int r, A s) {
+    if (r == x1) {
+      fwd_part[t] = s;  // the row still open at the partition's end
+    } else if (r == x0 && rst[r] < y0) {
+      first_part[t] = s;  // the row continued from earlier partitions
+    } else {
+      y[r] = alpha * s + scaled_y(beta, y, r);
+    }
+  };
+  if (nrows > kWG / 2) {
+    for (int i = tid; i < nrows; i += kWG) {
+      const int r = x0 + i;
+      const int lo = max(rst[r], y0) - y0, hi = min(rst[r + 1], y1) - y0;
+      A s = 0;
+      for (int k = lo; k < hi; ++k) s += prod[k];
+      finish(r, s);
+    }
+    return;
+  }
+  if (nrows == 0) return;
+  int width = 64;
+  while (width * nrows > kWG) width >>= 1;
+  const int g = tid / width, l = tid & (width - 1);
+  Acc<A, false> a;
+  if (g < nrows) {
+    const int r = x0 + g;
+    const int lo = max(rst[r], y0) - y0, hi = min(rst[r + 1], y1) - y0;
+    for (int k = lo + l; k < hi; k += width) a.add(prod[k]);
+  }
+  a = group_reduce(a, width);
+  if (g < nrows && l == 0) finish(x0 + g, a.value());
+}
+
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_vnec_fixup(
+    const int32_t* __restrict__ cx, const int32_t* __restrict__ cy,
+    const int32_t* __restrict__ rst, typename Val<VT>::io* __restrict__ y,
+    const typename Val<VT>::acc* __restrict__ first_part,
+    const typename Val<VT>::acc* __restrict__ fwd_part, int64_t nparts, int64_t m,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t t = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (t >= nparts) return;
+  const int r = cx[t + 1];
+  if (r >= m || rst[r] < cy[t] || rst[r] >= cy[t + 1]) return;  // not the row's first partition
+  A s = fwd_part[t];
+  for (int64_t q = t + 1; q < nparts; ++q) {
+    if (cx[q] == cx[q + 1]) {
+      s += fwd_part[q];  // the whole partition lies inside the row
+    } else {
+      s += first_part[q];
+      break;
+    }
+  }
+  y[r] = alpha * s + scaled_y(beta, y, r);
+}
+
+template <class VT>
+static int run_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                    hipStream_t s) {
+  using IO = typename Val<VT>::io;
+  using AC = typename Val<VT>::acc;
+  const DevVNEC& d = A.vnec;
+  if (d.nparts == 0) return RVVLA_OK;
+  AC* fp = (AC*)d.scratch;
+  AC* wp = fp + d.nparts;
+  hipLaunchKernelGGL((k_vnec<VT>), dim3((unsigned)d.nparts), dim3(kWG), 0, s, d.cx, d.cy,
+                     d.nec_ptr, d.use_x, d.ecr, d.rst, (const VT*)d.val, (const IO*)x, (IO*)y, fp,
+                     wp, A.m, (AC)alpha, (AC)beta);
+  if (int rc = hip_check(hipGetLastError(), "k_vnec launch")) return rc;
+  hipLaunchKernelGGL((k_vnec_fixup<VT>), dim3((unsigned)((d.nparts + kWG - 1) / kWG)), dim3(kWG),
+                     0, s, d.cx, d.cy, d.rst, (IO*)y, fp, wp, d.nparts, A.m, (AC)alpha, (AC)beta);
+  return hip_check(hipGetLastError(), "k_vnec_fixup launch");
+}
+
+int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                hipStream_t s) {
+  switch (A.dtype) {
+    case RVVLA_F64: return run_vnec<double>(A, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_vnec<float>(A, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_vnec<bf16_t>(A, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_vnec<_Float16>(A, alpha, x, beta, y, s);
+  }
+  return RVVLA_ERR_VALUE_TYPE;
+}
+
+}  // namespace rvvla

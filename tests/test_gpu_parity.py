@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-FORMATS_READY = ["csr", "hcsr", "sell", "csr5", "lav", "vhcc", "cvr"]
+FORMATS_READY = ["csr", "hcsr", "sell", "csr5", "lav", "vnec", "vhcc", "cvr"]
 
 
 def _device():
