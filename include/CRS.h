@@ -7,6 +7,8 @@
 #include <cstring>
 #include <ostream>
 
+#include "device_image.h"
+
 namespace SparseMatrixLib {
 
 typedef char MM_typecode[4];
@@ -22,6 +24,8 @@ class spMtxCRS {
   int* Rst = nullptr;
   int* Col = nullptr;
   ValT* Val = nullptr;
+  /* device image, uploaded on the first multiply (not copied with the matrix) */
+  mutable DeviceImage device;
 
   spMtxCRS() = default;
   spMtxCRS(size_t rows, size_t cols) : m(rows), n(cols) { Rst = new int[m + 1](); }

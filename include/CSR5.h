@@ -68,7 +68,7 @@ class spMtxCSR5 {
   std::vector<uint32_t> _csr5_tile_empty_offset_pointer;
   std::vector<uint32_t> _csr5_tile_empty_offset;
   std::vector<ValT> _temp_calibrator;
-  DeviceImage device;
+  mutable DeviceImage device;
 
   int format() const { return _format; }
   void freeMem() {

@@ -33,7 +33,7 @@ class spMtxCVR {
   std::vector<uint32_t> vPack_vec_cols;
   std::vector<int> vPack_vec_record;
   std::vector<int> writeback_pos_record;
-  DeviceImage device;
+  mutable DeviceImage device;
 };
 
 }  // namespace SparseMatrixLib

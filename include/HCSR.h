@@ -29,7 +29,7 @@ class spMtxHCSR {
   std::vector<uint32_t> row_ptr;
   std::vector<uint16_t> col_idx;
   std::vector<ValT> values;
-  DeviceImage device;
+  mutable DeviceImage device;
 
   spMtxHCSR() = default;
   spMtxHCSR(size_t rows, size_t cols, size_t nnz, size_t r, size_t c)

@@ -42,7 +42,7 @@ struct spMtxLAV {
   std::vector<type> val;
   std::vector<uint32_t> col_id;
   spMtxCRS<type> sparse_part;
-  DeviceImage device;
+  mutable DeviceImage device;
 };
 
 }  // namespace SparseMatrixLib

@@ -31,7 +31,7 @@ class spMtxSELL_C_Sigma {
   std::vector<uint32_t> vPerm;
   std::vector<ValT> vVal;
   std::vector<int> vPartition;
-  DeviceImage device;
+  mutable DeviceImage device;
 
   void freeMem() {
     vCl.clear(); vCs.clear(); vBs.clear(); vSBs.clear();

@@ -65,7 +65,7 @@ class spMtxVHCC {
   vhcc_params params;
 
  public:
-  DeviceImage device;
+  mutable DeviceImage device;
 
   size_t get_n_masks() const { return params.n_masks; }
   int get_pad_rows() const { return (int)_pad_rows; }

@@ -44,7 +44,7 @@ class spMtxVNEC {
   std::vector<int> use_x_indices;
   float IRD_mat = 0.f;
   SpB_VNEC_type vnec_t = SpB_VNEC_L;
-  DeviceImage device;
+  mutable DeviceImage device;
 };
 
 }  // namespace SparseMatrixLib

@@ -11,7 +11,12 @@ struct DeviceImage {
   std::shared_ptr<rvvla_mat_s> handle;
   rvvla_mat_t get() const { return handle.get(); }
   explicit operator bool() const { return (bool)handle; }
-  void reset(rvvla_mat_t h) { handle.reset(h, [](rvvla_mat_t p) { if (p) rvvla_mat_free(p); }); }
+  void reset(rvvla_mat_t h) {
+    if (h) handle.reset(h, [](rvvla_mat_t p) { rvvla_mat_free(p); });
+    else handle.reset();
+  }
+  /* drop the device copy (after changing the host fields) */
+  void clear() { handle.reset(); }
 };
 
 }  // namespace SparseMatrixLib

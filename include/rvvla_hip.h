@@ -146,6 +146,15 @@ int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx,
  * (reference include/spmv_mtx.h:21-29). */
 int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host,
                     double beta, void* y_host);
+/* The three stages of that call, separately (reference
+ * include/types.h:33-38): PREPARATION copies b and y into the matrix's
+ * device staging vectors (y_host may be NULL: y then starts at zero),
+ * MULT multiplies them in place, GET_RESULTS copies y back.  All three are
+ * synchronous; the staging vectors belong to the matrix, so concurrent
+ * host-vector calls on one matrix must be serialised by the caller. */
+int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host);
+int rvvla_mult_staged(rvvla_mat_t A, double alpha, double beta);
+int rvvla_stage_out(rvvla_mat_t A, void* y_host);
 
 /* ---- synthetic matrices (host, deterministic, OpenMP) ----------------
  * Generators for the benchmark matrices.  Every array is allocated by the

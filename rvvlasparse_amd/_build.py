@@ -22,8 +22,8 @@ BUILD = ROOT / "_build"
 LIB = PKG / "librvvla_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / "capi.cpp", CSRC / "cxx_api.cpp"]
-HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name not in ("capi.cpp", "cxx_api.cpp"))
+HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / "capi.cpp"]
+HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name != "capi.cpp")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -65,6 +65,11 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     _run(["g++", "-shared", "-fopenmp", "-o", str(tmp), *map(str, objs),
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
+    # the C++ drop-in example/test program, linked against the library
+    demo = ROOT / "tests" / "cxx" / "api_drop_in.cpp"
+    if demo.exists():
+        _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(demo), f"-L{PKG}", "-lrvvla_hip",
+              "-Wl,-rpath,$ORIGIN/../rvvlasparse_amd", "-o", str(BUILD / "api_drop_in")])
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", flush=True)
     return LIB

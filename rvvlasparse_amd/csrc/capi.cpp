@@ -1112,8 +1112,8 @@ int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx, d
   return rvvla::launch_csr_spmm(*A, k, alpha, X, ldx, beta, Y, ldy, (hipStream_t)stream);
 }
 
-int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host, double beta, void* y_host) {
-  if (!A || (A->n > 0 && !b_host) || (A->m > 0 && !y_host)) return RVVLA_ERR_INVALID_ARG;
+int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host) {
+  if (!A || (A->n > 0 && !b_host)) return RVVLA_ERR_INVALID_ARG;
   const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;
   int rc;
   if (!A->xstage && (rc = rvvla::hip_check(hipMalloc(&A->xstage, (size_t)A->n * es + 8), "hipMalloc x")))
@@ -1122,10 +1122,36 @@ int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host, double beta
     return rc;
   if ((rc = rvvla::hip_check(hipMemcpy(A->xstage, b_host, (size_t)A->n * es, hipMemcpyHostToDevice), "copy b")))
     return rc;
-  if ((rc = rvvla::hip_check(hipMemcpy(A->ystage, y_host, (size_t)A->m * es, hipMemcpyHostToDevice), "copy y")))
-    return rc;
-  if ((rc = rvvla_spmv(A, alpha, A->xstage, beta, A->ystage, nullptr))) return rc;
+  if (y_host)
+    return rvvla::hip_check(hipMemcpy(A->ystage, y_host, (size_t)A->m * es, hipMemcpyHostToDevice), "copy y");
+  return rvvla::hip_check(hipMemset(A->ystage, 0, (size_t)A->m * es), "clear y");
+}
+
+int rvvla_mult_staged(rvvla_mat_t A, double alpha, double beta) {
+  if (!A) return RVVLA_ERR_INVALID_ARG;
+  if (!A->xstage || !A->ystage) {
+    rvvla::set_last_error("MULT before PREPARATION");
+    return RVVLA_ERR_LOGIC;
+  }
+  if (int rc = rvvla_spmv(A, alpha, A->xstage, beta, A->ystage, nullptr)) return rc;
+  return rvvla::hip_check(hipDeviceSynchronize(), "multiply");
+}
+
+int rvvla_stage_out(rvvla_mat_t A, void* y_host) {
+  if (!A || (A->m > 0 && !y_host)) return RVVLA_ERR_INVALID_ARG;
+  if (!A->ystage) {
+    rvvla::set_last_error("GET_RESULTS before PREPARATION");
+    return RVVLA_ERR_LOGIC;
+  }
+  const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;
   return rvvla::hip_check(hipMemcpy(y_host, A->ystage, (size_t)A->m * es, hipMemcpyDeviceToHost), "copy y back");
+}
+
+int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host, double beta, void* y_host) {
+  if (!A || (A->n > 0 && !b_host) || (A->m > 0 && !y_host)) return RVVLA_ERR_INVALID_ARG;
+  if (int rc = rvvla_stage_in(A, b_host, y_host)) return rc;
+  if (int rc = rvvla_mult_staged(A, alpha, beta)) return rc;
+  return rvvla_stage_out(A, y_host);
 }
 
 }  // extern "C"

@@ -1,0 +1,221 @@
+// cxx_api.cpp -- the C++ drop-in: convert<T, Fmt, true> and
+// sparse_mv<T, Fmt, true, stage> for the eight formats, on top of the C-ABI.
+//
+// Mirrors the reference API (reference include/spmv_mtx.h:21-29,
+// include/convert.h:30-31): the same template names, arguments and
+// std::vector signature.  convert builds the format's host fields and
+// uploads its device image once (dist.device); sparse_mv uploads lazily when
+// a container was filled some other way.  Stages (reference
+// include/types.h:33-38): PREPARATION copies b and y to the device,
+// MULT runs the kernel, GET_RESULTS copies y back, ALL_STAGES does all
+// three.  Status codes are the RVVLA_* codes (0 = success); the reference
+// ignores type_op, this library answers SPARSE_OPERATION_TRANSPOSE with
+// RVVLA_ERR_UNSUPPORTED_SPMV instead of computing the untransposed product.
+#include <algorithm>
+#include <type_traits>
+
+#include "internal.h"
+
+namespace SparseMatrixLib {
+
+namespace {
+
+using rvvla::Error;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return RVVLA_OK;
+  } catch (const Error& e) {
+    rvvla::set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    return RVVLA_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception& e) {
+    rvvla::set_last_error(e.what());
+    return RVVLA_ERR_LOGIC;
+  }
+}
+
+template <class T>
+constexpr int dtype_of() {
+  return std::is_same<T, double>::value ? RVVLA_F64 : RVVLA_F32;
+}
+
+// Device image of each container (uploaded on first use).
+template <class T>
+rvvla_mat_t image(const spMtxCRS<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_csr<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxHCSR<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_hcsr<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxSELL_C_Sigma<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_sell<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxCSR5<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_csr5<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxLAV<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_lav<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxCVR<T>& a) {
+  if (!a.device) {
+    int64_t nz = 0;
+    for (const T& v : a.vPack_vec_vals) nz += v != T(0);
+    a.device.reset(rvvla::upload_cvr<T>(dtype_of<T>(), a, nz));
+  }
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxVHCC<T>& a) {
+  if (!a.device) a.device.reset(rvvla::upload_vhcc<T>(dtype_of<T>(), a));
+  return a.device.get();
+}
+template <class T>
+rvvla_mat_t image(const spMtxVNEC<T>& a) {
+  if (!a.device) {
+    if (!a.G) throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: no source matrix");
+    a.device.reset(rvvla::upload_vnec<T>(dtype_of<T>(), a, *a.G));
+  }
+  return a.device.get();
+}
+
+template <class T, class M>
+sparse_matrix_status run_stage(sparse_matrix_mv_stage st, sparse_operation_t op, T alpha, const M& mat,
+                               const std::vector<T>& b, T beta, std::vector<T>& y) {
+  sparse_matrix_status status;
+  if (op != SPARSE_OPERATION_NON_TRANSPOSE) {
+    rvvla::set_last_error("only SPARSE_OPERATION_NON_TRANSPOSE is implemented");
+    status.code = RVVLA_ERR_UNSUPPORTED_SPMV;
+    return status;
+  }
+  status.code = guard([&]() {
+    rvvla_mat_t A = image(mat);
+    int64_t m = 0, n = 0;
+    rvvla_mat_info(A, &m, &n, nullptr, nullptr, nullptr, nullptr);
+    if ((int64_t)b.size() < n || (int64_t)y.size() < m)
+      throw Error(RVVLA_ERR_INVALID_ARG, "b or y shorter than the matrix");
+    int rc = RVVLA_OK;
+    switch (st) {
+      case SPARSE_MATRIX_MV_PREPARATION: rc = rvvla_stage_in(A, b.data(), y.data()); break;
+      case SPARSE_MATRIX_MV_MULT: rc = rvvla_mult_staged(A, alpha, beta); break;
+      case SPARSE_MATRIX_MV_GET_RESULTS: rc = rvvla_stage_out(A, y.data()); break;
+      case SPARSE_MATRIX_MV_ALL_STAGES: rc = rvvla_spmv_host(A, alpha, b.data(), beta, y.data()); break;
+    }
+    if (rc) throw Error(rc, rvvla_strerror(rc));
+  });
+  return status;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- sparse_mv
+#define RVVLA_SPMV(T, FMT, STAGE)                                                              \
+  template <>                                                                                  \
+  sparse_matrix_status sparse_mv<T, FMT, true, STAGE>(                                          \
+      sparse_operation_t op, T alpha, const FMT<T>& mat, sparse_matrix_descr,                   \
+      const std::vector<T>& b, T beta, std::vector<T>& y) {                                     \
+    return run_stage<T, FMT<T>>(STAGE, op, alpha, mat, b, beta, y);                             \
+  }
+#define RVVLA_SPMV_ALL(T, FMT)                       \
+  RVVLA_SPMV(T, FMT, SPARSE_MATRIX_MV_PREPARATION)   \
+  RVVLA_SPMV(T, FMT, SPARSE_MATRIX_MV_MULT)          \
+  RVVLA_SPMV(T, FMT, SPARSE_MATRIX_MV_GET_RESULTS)   \
+  RVVLA_SPMV(T, FMT, SPARSE_MATRIX_MV_ALL_STAGES)
+#define RVVLA_SPMV_FMT(FMT) RVVLA_SPMV_ALL(double, FMT) RVVLA_SPMV_ALL(float, FMT)
+
+RVVLA_SPMV_FMT(spMtxCRS)
+RVVLA_SPMV_FMT(spMtxHCSR)
+RVVLA_SPMV_FMT(spMtxSELL_C_Sigma)
+RVVLA_SPMV_FMT(spMtxCSR5)
+RVVLA_SPMV_FMT(spMtxLAV)
+RVVLA_SPMV_FMT(spMtxVNEC)
+RVVLA_SPMV_FMT(spMtxVHCC)
+RVVLA_SPMV_FMT(spMtxCVR)
+
+// ---------------------------------------------------------------- convert
+namespace {
+
+int iparam(const convertParams& p, const char* k, int dflt) {
+  auto it = p.param.find(k);
+  return it == p.param.end() ? dflt : it->second.i;
+}
+double dparam(const convertParams& p, const char* k, double dflt) {
+  auto it = p.param.find(k);
+  return it == p.param.end() ? dflt : it->second.d;
+}
+
+}  // namespace
+
+// convert returns void as in the reference; failures are thrown as
+// rvvla::Error (a std::runtime_error carrying the RVVLA_* code), where the
+// reference converter throws std::invalid_argument / length_error /
+// logic_error (reference src/scalar/crs_to_hcsr.cpp:38,55,87-91).
+// HCSR keys "R", "C" (int) and "p" (double), defaults as the reference's
+// (reference src/scalar/crs_to_hcsr.cpp:141-149, :158-166).
+template <>
+void convert<double, spMtxHCSR, true>(spMtxHCSR<double>& d, const spMtxCRS<double>& s,
+                                      const convertParams& p) {
+  rvvla::build_hcsr<double>(d, s, (size_t)iparam(p, "R", 2000), (size_t)iparam(p, "C", 2000),
+                            dparam(p, "p", 2.0));
+  d.device.reset(rvvla::upload_hcsr<double>(RVVLA_F64, d));
+}
+template <>
+void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>& s,
+                                     const convertParams& p) {
+  rvvla::build_hcsr<float>(d, s, (size_t)iparam(p, "R", 4000), (size_t)iparam(p, "C", 4000),
+                           dparam(p, "p", 2.0));
+  d.device.reset(rvvla::upload_hcsr<float>(RVVLA_F32, d));
+}
+
+#define RVVLA_CONVERT(T, FMT, BUILD, UPLOAD)                                                       \
+  template <>                                                                                      \
+  void convert<T, FMT, true>(FMT<T> & d, const spMtxCRS<T>& s, const convertParams& p) {           \
+    BUILD;                                                                                         \
+    d.device.reset(UPLOAD);                                                                        \
+  }
+
+#define RVVLA_CONVERT_BOTH(FMT, BUILD, UPLOAD)                                                     \
+  RVVLA_CONVERT(double, FMT, BUILD, UPLOAD) RVVLA_CONVERT(float, FMT, BUILD, UPLOAD)
+
+// SELL-C-sigma keys "C" (64), "sigma"
+RVVLA_CONVERT_BOTH(spMtxSELL_C_Sigma,
+                   rvvla::build_sell(d, s, iparam(p, "C", 64),
+                                     iparam(p, "sigma", rvvla::default_sell_sigma((int64_t)s.m, (int64_t)s.nz))),
+                   rvvla::upload_sell(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+// CSR5 keys "omega" (64), "sigma" (0 = chosen from the row lengths)
+RVVLA_CONVERT_BOTH(spMtxCSR5, rvvla::build_csr5(d, s, iparam(p, "omega", 64), iparam(p, "sigma", 0)),
+                   rvvla::upload_csr5(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+// LAV keys "SIMD_Lanes" (64), "T" (double, 0.2), "segment_len"
+RVVLA_CONVERT_BOTH(spMtxLAV,
+                   rvvla::build_lav(d, s, iparam(p, "SIMD_Lanes", 64), dparam(p, "T", 0.2),
+                                    iparam(p, "segment_len", 1 << 20)),
+                   rvvla::upload_lav(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+// CVR keys "SIMD_LEN" (64), "streams"
+RVVLA_CONVERT_BOTH(spMtxCVR,
+                   rvvla::build_cvr(d, s, iparam(p, "SIMD_LEN", 64),
+                                    iparam(p, "streams", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, 2048))),
+                   rvvla::upload_cvr(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, (int64_t)s.nz))
+// VHCC keys "vec_size" (64), "threads"
+RVVLA_CONVERT_BOTH(spMtxVHCC,
+                   rvvla::build_vhcc(d, s, iparam(p, "vec_size", 64),
+                                     iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, 4096))),
+                   rvvla::upload_vhcc(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+// VNEC key "threads" (merge-path partitions; default: 2048 rows+entries each)
+RVVLA_CONVERT_BOTH(spMtxVNEC,
+                   rvvla::build_vnec(d, s, iparam(p, "threads", (int)std::max<int64_t>(1, ((int64_t)s.m + (int64_t)s.nz + 2047) / 2048))),
+                   rvvla::upload_vnec(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, s))
+
+}  // namespace SparseMatrixLib
