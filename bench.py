@@ -88,6 +88,61 @@ def cpu_baseline(a, x64, byts):
                       f"({med * 1e3:.3f} ms each)"}
 
 
+def powerlaw_leg(R, torch, steps, warmup, fmts):
+    """Power-law matrix (n = 4,194,304, lengths ~ k^-1.8 on [1, 50000] rescaled
+    to 64M entries, half within +-1024 of the diagonal, values N(0,1)): the
+    FP64 formats, then FP32 and the 16-bit value forms (FP32 x/y and
+    accumulation) in the best FP64 format.  Errors of the reduced forms are
+    stated against the FP64 result of the same format on the GPU (whose own
+    parity with the FP64 oracle is checked in the GPU tests): normwise
+    ||y - y64||_2 / ||y64||_2 and max-row max_i |y_i - y64_i| / ||y64||_inf."""
+    dev = torch.device("cuda:0")
+    log("bench: generating power-law matrix")
+    a = R.gen_powerlaw(4194304, 1.8, 50000, 64_000_000, 1024, seed=0)
+    x64 = R.gen_uniform(a.n, -1.0, 1.0, seed=0, purpose=3)
+    out = {"workload": f"power-law n={a.m}, nnz={a.nnz}, alpha=1.8, max row {int(a.row_lengths().max())}"}
+    per = {}
+    xs = {R.F64: torch.from_numpy(x64).to(dev), R.F32: torch.from_numpy(x64).float().to(dev)}
+    ys = {}
+    for fmt in fmts:
+        log(f"bench: powerlaw {fmt} f64")
+        try:
+            M = R.convert(fmt, a, R.F64).upload()
+        except R.RvvlaError as e:
+            per[fmt] = {"f64": {"error": str(e)}}
+            continue
+        y = torch.zeros(a.m, dtype=torch.float64, device=dev)
+        ev_ms, wall_ms = time_format(M, xs[R.F64], y, steps, warmup, torch)
+        byts = algo_bytes(a.m, a.n, a.nnz, 8, 8, 8)
+        per[fmt] = {"f64": {"ms": round(ev_ms, 5), "GBs": round(byts / ev_ms / 1e6, 1),
+                            "frac": round(byts / ev_ms / 1e6 / PEAK_GBS, 4)}}
+        del M
+    best = max((f for f in per if "ms" in per[f]["f64"]), key=lambda f: per[f]["f64"]["GBs"])
+    out["best_format"] = best
+    for dtype, tag, sv in ((R.F64, "f64", 8), (R.F32, "f32", 4), (R.BF16, "bf16", 2), (R.F16, "f16", 2)):
+        log(f"bench: powerlaw {best} {tag}")
+        M = R.convert(best, a, dtype).upload()
+        io = torch.float64 if dtype == R.F64 else torch.float32
+        y = torch.zeros(a.m, dtype=io, device=dev)
+        ev_ms, _ = time_format(M, xs[R.F64 if dtype == R.F64 else R.F32], y, steps, warmup, torch)
+        sx = 8 if dtype == R.F64 else 4
+        byts = algo_bytes(a.m, a.n, a.nnz, sv, sx, sx)
+        per[best][tag] = {"ms": round(ev_ms, 5), "GBs": round(byts / ev_ms / 1e6, 1),
+                          "frac": round(byts / ev_ms / 1e6 / PEAK_GBS, 4)}
+        ys[tag] = y.double()
+        del M
+    y64 = ys["f64"]
+    for tag in ("f32", "bf16", "f16"):
+        d = ys[tag] - y64
+        per[best][tag]["normwise_rel_err"] = float(torch.linalg.norm(d) / torch.linalg.norm(y64))
+        per[best][tag]["maxrow_rel_err"] = float(d.abs().max() / y64.abs().max())
+        per[best][tag]["speedup_vs_f64"] = round(per[best]["f64"]["ms"] / per[best][tag]["ms"], 3)
+    out["formats"] = per
+    out["bf16_vs_f64_speedup"] = per[best]["bf16"]["speedup_vs_f64"]
+    out["bf16_normwise_rel_err"] = per[best]["bf16"]["normwise_rel_err"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +151,7 @@ def main():
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--formats", default="all")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-powerlaw", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -158,6 +214,8 @@ def main():
         "roofline": roofline,
         "formats": results,
     }
+    if not args.no_powerlaw:
+        line["powerlaw"] = powerlaw_leg(R, torch, args.steps, args.warmup, fmts)
     if not args.no_cpu:
         log("bench: CPU baseline")
         line["cpu_baseline"] = cpu_baseline(a, x64, byts64)

@@ -47,6 +47,23 @@ void build_cvr(spMtxCVR<V>& d, const spMtxCRS<V>& s, int simd_len, int nstreams)
   if (nstreams < 1) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: at least one stream");
   const int64_t m = (int64_t)s.m, L = simd_len;
   const std::vector<int64_t> rb = balanced_row_parts(m, s.Rst, nstreams);
+  // A stream lasts at least as many packs as its longest row: with rows far
+  // longer than a stream's share, the lanes idle on padding.  Refuse layouts
+  // whose lower bound on packed slots exceeds 16x the entries (plus 2^20).
+  {
+    int64_t bound = 0;
+    for (int t = 0; t < nstreams; ++t) {
+      int64_t longest = 0;
+      for (int64_t r = rb[(size_t)t]; r < rb[(size_t)t + 1]; ++r)
+        longest = std::max<int64_t>(longest, s.Rst[r + 1] - s.Rst[r]);
+      const int64_t nz_t = s.Rst[rb[(size_t)t + 1]] - s.Rst[rb[(size_t)t]];
+      bound += std::max<int64_t>((nz_t + L - 1) / L, longest) * L;
+    }
+    if (bound > 16 * (int64_t)s.nz + (int64_t(1) << 20) || bound > 2147483647LL)
+      throw Error(RVVLA_ERR_UNSUPPORTED_SPMV,
+                  "CVR: rows are too long for the stream layout (at least " + std::to_string(bound) +
+                      " packed slots for " + std::to_string(s.nz) + " entries)");
+  }
   // pass 1 (parallel): each stream packed on its own
   struct Stream {
     std::vector<V> vals;

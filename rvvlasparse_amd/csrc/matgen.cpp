@@ -50,6 +50,27 @@ void top_up(std::vector<int>& v, size_t need, int64_t lo, int64_t hi, Rng& r) {
   }
 }
 
+// `v` (empty on entry) receives k distinct sorted values of [lo, hi]: drawn
+// directly when k is at most half the range, otherwise the range minus
+// (range - k) drawn values.
+void pick_distinct(std::vector<int>& v, size_t k, int64_t lo, int64_t hi, Rng& r) {
+  const size_t w = (size_t)(hi - lo + 1);
+  if (2 * k <= w) {
+    top_up(v, k, lo, hi, r);
+    return;
+  }
+  std::vector<int> skip;
+  top_up(skip, w - k, lo, hi, r);
+  size_t s = 0;
+  for (int64_t c = lo; c <= hi; ++c) {
+    if (s < skip.size() && skip[s] == c) {
+      ++s;
+      continue;
+    }
+    v.push_back((int)c);
+  }
+}
+
 int alloc_out(rvvla_gen_csr* out, int64_t m, int64_t n, int64_t nnz) {
   out->m = m;
   out->n = n;
@@ -197,7 +218,7 @@ int rvvla_gen_powerlaw(int64_t n, double alpha, int64_t maxlen, int64_t target_n
       const int64_t lo = std::max<int64_t>(0, r - band), hi = std::min<int64_t>(n - 1, r + band);
       const int64_t near = std::min<int64_t>((L + 1) / 2, hi - lo + 1);
       cols.clear();
-      top_up(cols, (size_t)near, lo, hi, rs);
+      pick_distinct(cols, (size_t)near, lo, hi, rs);
       top_up(cols, (size_t)L, 0, n - 1, rs);
       const int64_t base = out->rst[r];
       for (int64_t j = 0; j < L; ++j) {
