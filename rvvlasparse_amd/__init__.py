@@ -35,6 +35,15 @@ if not LIB_PATH.exists():
         f"{LIB_PATH} is not built; run `python -m rvvlasparse_amd._build` "
         "(the HIP library is required: there is no CPU fallback)")
 
+# One HIP runtime per process: torch ships its own libamdhip64.so.7.  Loading
+# torch first makes the library's libamdhip64.so.7 dependency resolve to that
+# copy; loading the library first would bring in a second HIP/HSA runtime
+# (/opt/rocm) that then finds no device.
+try:
+    import torch as _torch  # noqa: F401
+except ImportError:
+    _torch = None
+
 lib = C.CDLL(str(LIB_PATH))
 
 
