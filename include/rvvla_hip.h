@@ -141,6 +141,27 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta,
  * (CSR, F32/F64).  No reference counterpart: the reference takes one vector. */
 int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx,
                double beta, void* Y, int64_t ldy, void* stream);
+/* ---- differentiable SpMM (CSR, F32 / F64) ------------------------------
+ * No reference counterpart (the reference multiplies one vector with fixed
+ * values); these back the PyTorch autograd extension.
+ * rvvla_spmm_values: Y = alpha * A(vals) X + beta * Y with A's CSR
+ *   structure and the caller's values `vals` (device, CSR order; NULL = the
+ *   uploaded values).
+ * rvvla_spmm_transpose: Y = alpha * A(vals)^T X + beta * Y (Y is n x k)
+ *   through the cached CSC permutation -- gather-based, no atomics.
+ * rvvla_sddmm: out[e] = alpha * <DY[row_e, :], X[col_e, :]> for every entry
+ *   e (the gradient of the values of A in Y = A X).
+ * rvvla_prepare_transpose builds and uploads the CSC permutation (the first
+ * rvvla_spmm_transpose call does it otherwise; it allocates and copies
+ * synchronously, so do it before capturing a stream into a graph). */
+int rvvla_prepare_transpose(rvvla_mat_t A);
+int rvvla_spmm_values(rvvla_mat_t A, const void* vals, int k, double alpha, const void* X,
+                      int64_t ldx, double beta, void* Y, int64_t ldy, void* stream);
+int rvvla_spmm_transpose(rvvla_mat_t A, const void* vals, int k, double alpha, const void* X,
+                         int64_t ldx, double beta, void* Y, int64_t ldy, void* stream);
+int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy, const void* X,
+                int64_t ldx, void* out_vals, void* stream);
+
 /* ALL_STAGES on host vectors: upload b (and y), multiply, download y,
  * synchronously -- the std::vector form of sparse_mv
  * (reference include/spmv_mtx.h:21-29). */

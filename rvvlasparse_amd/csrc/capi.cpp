@@ -1112,6 +1112,74 @@ int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx, d
   return rvvla::launch_csr_spmm(*A, k, alpha, X, ldx, beta, Y, ldy, (hipStream_t)stream);
 }
 
+static int need_csr_real(rvvla_mat_t A) {
+  if (!A) return RVVLA_ERR_INVALID_ARG;
+  if (A->format != RVVLA_FMT_CSR) {
+    rvvla::set_last_error("differentiable SpMM needs a CSR matrix");
+    return RVVLA_ERR_UNSUPPORTED_SPMV;
+  }
+  if (A->dtype != RVVLA_F64 && A->dtype != RVVLA_F32) {
+    rvvla::set_last_error("differentiable SpMM needs FP32 or FP64 values");
+    return RVVLA_ERR_VALUE_TYPE;
+  }
+  return RVVLA_OK;
+}
+
+int rvvla_prepare_transpose(rvvla_mat_t A) {
+  if (int rc = need_csr_real(A)) return rc;
+  if (A->csc.ptr) return RVVLA_OK;
+  return guarded([&]() {
+    const int64_t m = A->m, n = A->n, nnz = A->nnz;
+    std::vector<int32_t> rst((size_t)m + 1), col((size_t)nnz);
+    int rc = rvvla::hip_check(hipMemcpy(rst.data(), A->csr.rst, rst.size() * 4, hipMemcpyDeviceToHost), "copy Rst");
+    if (!rc && nnz)
+      rc = rvvla::hip_check(hipMemcpy(col.data(), A->csr.col, col.size() * 4, hipMemcpyDeviceToHost), "copy Col");
+    if (rc) throw Error(rc, rvvla::last_error());
+    // counting sort by column; entries of a column stay in row order
+    std::vector<int32_t> ptr((size_t)n + 1, 0), row((size_t)nnz), perm((size_t)nnz);
+    for (int64_t e = 0; e < nnz; ++e) ptr[(size_t)col[(size_t)e] + 1]++;
+    for (int64_t c = 0; c < n; ++c) ptr[(size_t)c + 1] += ptr[(size_t)c];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t r = 0; r < m; ++r)
+      for (int32_t e = rst[(size_t)r]; e < rst[(size_t)r + 1]; ++e) {
+        const int32_t p = fill[(size_t)col[(size_t)e]]++;
+        row[(size_t)p] = (int32_t)r;
+        perm[(size_t)p] = e;
+      }
+    A->csc.ptr = A->put_vec(ptr);
+    A->csc.row = A->put_vec(row);
+    A->csc.perm = A->put_vec(perm);
+    return RVVLA_OK;
+  });
+}
+
+int rvvla_spmm_values(rvvla_mat_t A, const void* vals, int k, double alpha, const void* X,
+                      int64_t ldx, double beta, void* Y, int64_t ldy, void* stream) {
+  if (int rc = need_csr_real(A)) return rc;
+  if (k < 0 || ldx < k || ldy < k || (k > 0 && ((A->n > 0 && !X) || (A->m > 0 && !Y))))
+    return RVVLA_ERR_INVALID_ARG;
+  return rvvla::launch_spmm_values(*A, false, vals ? vals : A->csr.val, k, alpha, X, ldx, beta, Y,
+                                   ldy, (hipStream_t)stream);
+}
+
+int rvvla_spmm_transpose(rvvla_mat_t A, const void* vals, int k, double alpha, const void* X,
+                         int64_t ldx, double beta, void* Y, int64_t ldy, void* stream) {
+  if (int rc = need_csr_real(A)) return rc;
+  if (k < 0 || ldx < k || ldy < k || (k > 0 && ((A->m > 0 && !X) || (A->n > 0 && !Y))))
+    return RVVLA_ERR_INVALID_ARG;
+  if (int rc = rvvla_prepare_transpose(A)) return rc;
+  return rvvla::launch_spmm_values(*A, true, vals ? vals : A->csr.val, k, alpha, X, ldx, beta, Y,
+                                   ldy, (hipStream_t)stream);
+}
+
+int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy, const void* X,
+                int64_t ldx, void* out_vals, void* stream) {
+  if (int rc = need_csr_real(A)) return rc;
+  if (k < 0 || lddy < k || ldx < k || (A->nnz > 0 && (!DY || !X || !out_vals)))
+    return RVVLA_ERR_INVALID_ARG;
+  return rvvla::launch_sddmm(*A, k, DY, lddy, X, ldx, alpha, out_vals, (hipStream_t)stream);
+}
+
 int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host) {
   if (!A || (A->n > 0 && !b_host)) return RVVLA_ERR_INVALID_ARG;
   const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;

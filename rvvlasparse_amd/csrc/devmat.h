@@ -124,6 +124,14 @@ struct DevVNEC {
   void* scratch = nullptr;           // first_part[nparts], fwd_part[nparts]
 };
 
+// CSC view of a CSR matrix for transposed products: column pointers, row
+// of each entry in column order, and the CSR position of that entry.
+struct DevCSC {
+  const int32_t* ptr = nullptr;   // n + 1
+  const int32_t* row = nullptr;   // nnz
+  const int32_t* perm = nullptr;  // nnz
+};
+
 }  // namespace rvvla
 
 struct rvvla_mat_s {
@@ -141,6 +149,7 @@ struct rvvla_mat_s {
   rvvla::DevVHCC vhcc;
   rvvla::DevLAV lav;
   rvvla::DevVNEC vnec;
+  rvvla::DevCSC csc;
   // staging for rvvla_spmv_host
   void* xstage = nullptr;
   void* ystage = nullptr;
@@ -178,5 +187,10 @@ int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, v
                hipStream_t s);
 int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s);
+int launch_spmm_values(const rvvla_mat_s& A, bool transpose, const void* vals, int k, double alpha,
+                       const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
+                       hipStream_t s);
+int launch_sddmm(const rvvla_mat_s& A, int k, const void* DY, int64_t lddy, const void* X,
+                 int64_t ldx, double alpha, void* out, hipStream_t s);
 
 }  // namespace rvvla
