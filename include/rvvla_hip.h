@@ -162,6 +162,33 @@ int rvvla_spmm_transpose(rvvla_mat_t A, const void* vals, int k, double alpha, c
 int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy, const void* X,
                 int64_t ldx, void* out_vals, void* stream);
 
+/* ---- batched uneven SpMM in one persistent launch ----------------------
+ * No reference counterpart.  `count` CSR matrices (host arrays; values
+ * double for F64, float for F32), matrix i with k[i] in 1..16 right-hand
+ * sides, become one device work queue.  The batch owns row-major X_i
+ * (n_i x k_i) and Y_i (m_i x k_i) blocks in HBM (rvvla_batch_block gives
+ * their device addresses).  rvvla_batch_run computes every
+ * Y_i = alpha A_i X_i + beta Y_i with one persistent kernel (plus a one-word
+ * queue reset); rvvla_batch_capture records that pair into a hipGraph once
+ * and rvvla_batch_replay launches it.  params: "tune=0|1" (default 1: the
+ * autotuner times the candidate item kernels on the first matrix of each
+ * row-length-histogram key), "format=0|1,param=P" to force CSR rows (P rows
+ * per item) or SELL-64 (sigma P), "cache=<path>" to persist the tuner's
+ * choices. */
+typedef struct rvvla_batch_s* rvvla_batch_t;
+int rvvla_batch_create(int dtype, int count, const int64_t* m, const int64_t* n,
+                       const int32_t* const* rst, const int32_t* const* col,
+                       const void* const* val, const int* k, const char* params,
+                       rvvla_batch_t* out);
+/* Device addresses of X_i and Y_i, and the item kernel chosen for matrix i
+ * (RVVLA_FMT_CSR or RVVLA_FMT_SELL). */
+int rvvla_batch_block(rvvla_batch_t B, int i, void** x, void** y, int* format);
+int rvvla_batch_tune_key(rvvla_batch_t B, int i, char* buf, int len);
+int rvvla_batch_run(rvvla_batch_t B, double alpha, double beta, void* stream);
+int rvvla_batch_capture(rvvla_batch_t B, double alpha, double beta);
+int rvvla_batch_replay(rvvla_batch_t B, void* stream);
+int rvvla_batch_free(rvvla_batch_t B);
+
 /* ALL_STAGES on host vectors: upload b (and y), multiply, download y,
  * synchronously -- the std::vector form of sparse_mv
  * (reference include/spmv_mtx.h:21-29). */
@@ -196,6 +223,10 @@ int rvvla_gen_laplacian5(int64_t g, rvvla_gen_csr* out);
  * columns; values uniform[-1,1] (normal=0) or N(0, stdev^2) (normal=1). */
 int rvvla_gen_random_rows(int64_t n, int k, int include_diag, int normal,
                           double stdev, uint64_t seed, rvvla_gen_csr* out);
+/* n x n, row lengths uniform in [kmin, kmax], distinct uniform columns,
+ * sorted; values N(0,1). */
+int rvvla_gen_random_rows_var(int64_t n, int kmin, int kmax, uint64_t seed,
+                              rvvla_gen_csr* out);
 /* n x n power-law rows: length ~ discrete power law exponent `alpha` on
  * [1, maxlen], rescaled (stochastic rounding, clipped to [1, maxlen]) so the
  * expected total is `target_nnz`; half of each row within +-`band` of the

@@ -77,6 +77,7 @@ _sigs = {
     "rvvla_gen_laplacian5": (_i32, [_i64, C.POINTER(GenCSR)]),
     "rvvla_gen_random_rows": (_i32, [_i64, _i32, _i32, _i32, _d, C.c_uint64, C.POINTER(GenCSR)]),
     "rvvla_gen_powerlaw": (_i32, [_i64, _d, _i64, _i64, _i64, C.c_uint64, C.POINTER(GenCSR)]),
+    "rvvla_gen_random_rows_var": (_i32, [_i64, _i32, _i32, C.c_uint64, C.POINTER(GenCSR)]),
     "rvvla_gen_uniform": (_i32, [_i64, _d, _d, C.c_uint64, C.c_uint64, _vp]),
     "rvvla_gen_free": (_i32, [C.POINTER(GenCSR)]),
 }
@@ -262,6 +263,73 @@ def csr_upload(a: CSR, dtype: int = F64) -> DeviceMatrix:
 
 def convert(fmt: str | int, a: CSR, dtype: int = F64, params: str = "") -> Layout:
     return Layout(fmt, a, dtype, params)
+
+
+for _name, (_res, _args) in {
+    "rvvla_batch_create": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _cp, C.POINTER(_vp)]),
+    "rvvla_batch_block": (_i32, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i32)]),
+    "rvvla_batch_tune_key": (_i32, [_vp, _i32, _cp, _i32]),
+    "rvvla_batch_run": (_i32, [_vp, _d, _d, _vp]),
+    "rvvla_batch_capture": (_i32, [_vp, _d, _d]),
+    "rvvla_batch_replay": (_i32, [_vp, _vp]),
+    "rvvla_batch_free": (_i32, [_vp]),
+}.items():
+    _f = getattr(lib, _name)
+    _f.restype, _f.argtypes = _res, _args
+
+
+class Batch:
+    """Many CSR matrices (each with its own 1..16 right-hand sides) multiplied
+    in one persistent launch (rvvla_batch_*).  X_i and Y_i live in the
+    batch's device buffers; ``x(i)`` / ``y(i)`` return device addresses."""
+
+    def __init__(self, mats: list, ks: list, dtype: int = F32, params: str = ""):
+        cnt = len(mats)
+        self.mats, self.ks, self.dtype = mats, list(ks), dtype
+        vt = np.float64 if dtype == F64 else np.float32
+        self._vals = [np.ascontiguousarray(a.val, vt) for a in mats]
+        ms = np.array([a.m for a in mats], np.int64)
+        ns = np.array([a.n for a in mats], np.int64)
+        rp = (C.c_void_p * cnt)(*[a.rst.ctypes.data for a in mats])
+        cp = (C.c_void_p * cnt)(*[a.col.ctypes.data for a in mats])
+        vp = (C.c_void_p * cnt)(*[v.ctypes.data for v in self._vals])
+        kk = np.array(ks, np.int32)
+        h = C.c_void_p()
+        check(lib.rvvla_batch_create(dtype, cnt, ms.ctypes.data, ns.ctypes.data, rp, cp, vp,
+                                     kk.ctypes.data, params.encode(), C.byref(h)))
+        self._h = h
+
+    def block(self, i: int):
+        x, y, f = C.c_void_p(), C.c_void_p(), C.c_int()
+        check(lib.rvvla_batch_block(self._h, i, C.byref(x), C.byref(y), C.byref(f)))
+        return x.value, y.value, f.value
+
+    def tune_key(self, i: int) -> str:
+        buf = C.create_string_buffer(512)
+        check(lib.rvvla_batch_tune_key(self._h, i, buf, 512))
+        return buf.value.decode()
+
+    def run(self, alpha=1.0, beta=0.0, stream=None):
+        check(lib.rvvla_batch_run(self._h, alpha, beta, stream))
+
+    def capture(self, alpha=1.0, beta=0.0):
+        check(lib.rvvla_batch_capture(self._h, alpha, beta))
+
+    def replay(self, stream=None):
+        check(lib.rvvla_batch_replay(self._h, stream))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.rvvla_batch_free(self._h)
+            self._h = None
+
+
+def gen_random_rows_var(n: int, kmin: int, kmax: int, seed: int = 0) -> CSR:
+    """n x n, row lengths uniform in [kmin, kmax], distinct uniform columns,
+    values N(0,1) (rvvla_gen_random_rows with per-row lengths)."""
+    g = GenCSR()
+    check(lib.rvvla_gen_random_rows_var(n, kmin, kmax, seed, C.byref(g)))
+    return _take(g)
 
 
 def io_dtype(dtype: int):

@@ -22,8 +22,9 @@ BUILD = ROOT / "_build"
 LIB = PKG / "librvvla_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / "capi.cpp"]
-HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name != "capi.cpp")
+HIP_HOST = ("capi.cpp", "batch_host.cpp")  # host code that calls the HIP runtime
+HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / f for f in HIP_HOST]
+HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name not in HIP_HOST)
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -176,6 +176,10 @@ static void check_csr(int64_t m, int64_t n, int64_t nnz, const int* rst, const i
     if (col[e] < 0 || col[e] >= n) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: column out of range");
 }
 
+void check_csr_public(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col) {
+  check_csr(m, n, nnz, rst, col);
+}
+
 // Adaptive row blocks (see k_csr.hip).
 static std::vector<int32_t> csr_blocks(int64_t m, const int* rst) {
   std::vector<int32_t> b;

@@ -132,6 +132,34 @@ struct DevCSC {
   const int32_t* perm = nullptr;  // nnz
 };
 
+// Batched persistent launch: per-matrix metadata and the work items.
+struct BatchMat {
+  int64_t rst_off = 0;   // into the concatenated row pointers (CSR item kernel)
+  int64_t ent_off = 0;   // into the concatenated CSR entries
+  int64_t sl_off = 0;    // into the concatenated slice tables (SELL item kernel)
+  int64_t slot_off = 0;  // into the concatenated SELL slots
+  int64_t row_off = 0;   // into the concatenated SELL permutations
+  int64_t x_off = 0, y_off = 0;
+  int32_t m = 0, n = 0, k = 0, kp = 1, fmt = 0, pad = 0;
+};
+struct BatchItem {
+  int32_t mat, first, count, pad;
+};
+struct BatchArgs {
+  const BatchMat* mats;
+  const BatchItem* items;
+  int64_t nitems;
+  int* counter;
+  const int32_t* rst;
+  const int32_t* col;
+  const int32_t* cl;
+  const int32_t* cs;
+  const int32_t* scol;
+  const int32_t* perm;
+};
+int launch_batch(int dtype, const BatchArgs& a, const void* val, const void* sval, const void* X,
+                 void* Y, double alpha, double beta, int grid, hipStream_t s);
+
 }  // namespace rvvla
 
 struct rvvla_mat_s {

@@ -23,6 +23,10 @@ struct Error : std::runtime_error {
 void set_last_error(const std::string& s);
 const std::string& last_error();
 
+// Validates CSR arrays (shape, monotone row pointers, columns in range);
+// throws Error(RVVLA_ERR_INVALID_ARG).
+void check_csr_public(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col);
+
 // convertParams from "k=v,k=v" text.
 SparseMatrixLib::convertParams parse_params(const char* text);
 int param_int(const SparseMatrixLib::convertParams& p, const char* key, int dflt);

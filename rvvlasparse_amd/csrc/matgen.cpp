@@ -157,6 +157,39 @@ int rvvla_gen_random_rows(int64_t n, int k, int include_diag, int normal, double
   return RVVLA_OK;
 }
 
+int rvvla_gen_random_rows_var(int64_t n, int kmin, int kmax, uint64_t seed, rvvla_gen_csr* out) {
+  if (!out || n < 1 || kmin < 0 || kmax < kmin || kmax > n) return RVVLA_ERR_INVALID_ARG;
+  std::vector<int64_t> len((size_t)n);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    Rng rs(seed, 1, (uint64_t)r);
+    len[(size_t)r] = kmin + rs.below(kmax - kmin + 1);
+  }
+  int64_t nnz = 0;
+  for (int64_t r = 0; r < n; ++r) nnz += len[(size_t)r];
+  if (nnz > 2147483647LL) return RVVLA_ERR_LENGTH;
+  if (int rc = alloc_out(out, n, n, nnz)) return rc;
+  out->rst[0] = 0;
+  for (int64_t r = 0; r < n; ++r) out->rst[r + 1] = out->rst[r] + (int32_t)len[(size_t)r];
+#pragma omp parallel
+  {
+    std::vector<int> cols;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t r = 0; r < n; ++r) {
+      Rng rs(seed, 1, (uint64_t)r), rv(seed, 2, (uint64_t)r);
+      rs.below(1);  // the length draw
+      cols.clear();
+      pick_distinct(cols, (size_t)len[(size_t)r], 0, n - 1, rs);
+      const int64_t base = out->rst[r];
+      for (int64_t j = 0; j < len[(size_t)r]; ++j) {
+        out->col[base + j] = cols[(size_t)j];
+        out->val[base + j] = rv.normal();
+      }
+    }
+  }
+  return RVVLA_OK;
+}
+
 int rvvla_gen_powerlaw(int64_t n, double alpha, int64_t maxlen, int64_t target_nnz, int64_t band,
                        uint64_t seed, rvvla_gen_csr* out) {
   if (!out || n < 1 || maxlen < 1 || maxlen > n || alpha <= 1.0 || band < 0 || target_nnz < n)
