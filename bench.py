@@ -59,6 +59,26 @@ def time_format(M, x, y, steps, warmup, torch):
     return ev0.elapsed_time(ev1) / steps, wall
 
 
+def copy_probe(torch, nbytes=1 << 30, steps=20):
+    """Achievable device-to-device copy rate (read + write bytes per second)
+    with torch's copy kernel on `nbytes` buffers; also the known-size kernel
+    used to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    del src, dst
+    return {"bytes_moved": 2 * nbytes, "ms": round(ms, 4), "GBs": round(2 * nbytes / ms / 1e6, 1)}
+
+
 def cpu_baseline(a, x64, byts):
     """Scalar OpenMP CSR restatement of the reference (reference
     src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores."""
@@ -143,6 +163,89 @@ def powerlaw_leg(R, torch, steps, warmup, fmts):
     return out
 
 
+def autograd_leg(R, torch, steps, warmup):
+    """Differentiable SpMM Y = A(theta) X: A n=2^20 with 32 distinct uniform
+    columns per row, theta N(0,1/32), X 2^20 x 64, FP32; loss = sum(W*Y).
+    Times the forward and the backward (dX = A^T dY and dtheta) with HIP
+    events on the current stream."""
+    from rvvlasparse_amd.autograd import SparseStructure, spmm
+    n, k = 1 << 20, 64
+    log("bench: autograd SpMM setup")
+    a = R.gen_random_rows(n, 32, seed=5, include_diag=False, normal=True, stdev=32 ** -0.5)
+    A = SparseStructure(a.m, a.n, a.rst, a.col, torch.float32)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    theta = torch.tensor(a.val, dtype=torch.float32, device="cuda", requires_grad=True)
+    X = torch.randn(n, k, device="cuda", generator=g, requires_grad=True)
+    W = torch.randn(n, k, device="cuda", generator=g)
+
+    def fwd():
+        return spmm(A, theta, X)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(warmup):
+        Y = fwd()
+        Y.backward(W)
+    torch.cuda.synchronize()
+    f_ms = b_ms = 0.0
+    for _ in range(steps):
+        ev[0].record()
+        Y = fwd()
+        ev[1].record()
+        Y.backward(W)
+        ev[2].record()
+        torch.cuda.synchronize()
+        f_ms += ev[0].elapsed_time(ev[1])
+        b_ms += ev[1].elapsed_time(ev[2])
+    f_ms /= steps
+    b_ms /= steps
+    fb = a.nnz * 8 + (n + 1) * 4 + 2 * n * k * 4  # vals+cols, row ptr, X and Y
+    return {"workload": f"SpMM n={n}, nnz={a.nnz}, k={k}, FP32", "forward_ms": round(f_ms, 4),
+            "backward_ms": round(b_ms, 4), "backward_over_forward": round(b_ms / f_ms, 3),
+            "forward_GFLOPs": round(2 * a.nnz * k / f_ms / 1e6, 1),
+            "forward_compulsory_GBs": round(fb / f_ms / 1e6, 1)}
+
+
+def batch_leg(R, torch, steps, warmup, count=512):
+    """512 square matrices, n log-uniform in [1e3, 2e5], 8..64 distinct
+    uniform columns per row, values N(0,1), k in 1..16 right-hand sides,
+    FP32, one persistent launch per step (autotuned item kernels)."""
+    rng = np.random.default_rng(11)
+    ns = np.exp(rng.uniform(np.log(1e3), np.log(2e5), count)).astype(int)
+    ks = [int(k) for k in rng.integers(1, 17, count)]
+    log("bench: batch generation")
+    mats = [R.gen_random_rows_var(int(n), 8, 64, seed=100 + i) for i, n in enumerate(ns)]
+    nnz = sum(a.nnz for a in mats)
+    log(f"bench: batch create ({count} matrices, {nnz} entries)")
+    t0 = time.perf_counter()
+    B = R.Batch(mats, ks, R.F32)
+    create_s = time.perf_counter() - t0
+    s = torch.cuda.current_stream().cuda_stream
+    for _ in range(warmup):
+        B.run(1.0, 0.0, s)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        B.run(1.0, 0.0, s)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    B.capture(1.0, 0.0)
+    e0.record()
+    for _ in range(steps):
+        B.replay(s)
+    e1.record()
+    torch.cuda.synchronize()
+    gms = e0.elapsed_time(e1) / steps
+    byts = sum(a.nnz * 8 + (a.m + 1) * 4 + a.n * k * 4 + a.m * k * 4 for a, k in zip(mats, ks))
+    flops = sum(2 * a.nnz * k for a, k in zip(mats, ks))
+    fmts = [B.block(i)[2] for i in range(count)]
+    return {"workload": f"{count} matrices, {sum(a.m for a in mats)} rows, {nnz} entries, k 1..16, FP32",
+            "ms": round(ms, 4), "graph_ms": round(gms, 4), "GBs": round(byts / ms / 1e6, 1),
+            "GFLOPs": round(flops / ms / 1e6, 1), "create_s": round(create_s, 2),
+            "item_kernels": {"csr": fmts.count(0), "sell": fmts.count(2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +255,10 @@ def main():
     ap.add_argument("--formats", default="all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-powerlaw", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the autograd-SpMM and batched legs")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only the 1 GiB copy probe (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -164,6 +271,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
+    if args.copy_only:
+        print(json.dumps({"copy_probe": copy_probe(torch, steps=args.steps)}), flush=True)
+        return
     g = args.grid
     log(f"bench: generating Laplacian {g}x{g}")
     a = R.gen_laplacian5(g)
@@ -214,8 +324,12 @@ def main():
         "roofline": roofline,
         "formats": results,
     }
+    line["copy_probe"] = copy_probe(torch)
     if not args.no_powerlaw:
         line["powerlaw"] = powerlaw_leg(R, torch, args.steps, args.warmup, fmts)
+    if not args.no_extra:
+        line["autograd_spmm"] = autograd_leg(R, torch, min(args.steps, 20), 3)
+        line["batched"] = batch_leg(R, torch, min(args.steps, 20), 3)
     if not args.no_cpu:
         log("bench: CPU baseline")
         line["cpu_baseline"] = cpu_baseline(a, x64, byts64)

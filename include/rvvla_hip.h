@@ -120,6 +120,12 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz,
                       const int32_t* vSBs, int64_t slots, const uint32_t* vCol,
                       const uint32_t* vPerm, const void* vVal, rvvla_mat_t* out);
 
+/* Host-validation mode (per process, off by default): uploads run every
+ * check and re-lay of their format but keep the arrays in host memory and
+ * need no device; the resulting matrices cannot multiply
+ * (RVVLA_ERR_NO_DEVICE).  For validating layouts where no GPU is present. */
+int rvvla_set_host_validation(int on);
+
 int rvvla_mat_free(rvvla_mat_t A);
 /* Shape and storage of a device matrix.  device_bytes = bytes of the
  * format's arrays resident in HBM (what one multiply streams, x/y apart). */

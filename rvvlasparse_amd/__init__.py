@@ -68,6 +68,7 @@ _sigs = {
                                  _vp, _vp, _vp, C.POINTER(_vp)]),
     "rvvla_sell_upload": (_i32, [_i32, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp,
                                  _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "rvvla_set_host_validation": (_i32, [_i32]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),

@@ -8,7 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cctype>
+#include <cstdlib>
 #include <cstdio>
 #include <functional>
 #include <mutex>
@@ -107,8 +109,15 @@ uint16_t f32_to_f16(float f) {
 using rvvla::Error;
 
 // ---------------------------------------------------------------- device matrix
+namespace rvvla {
+static std::atomic<int> g_host_validation{0};
+}
+
 rvvla_mat_s::~rvvla_mat_s() {
-  for (void* p : allocs) (void)hipFree(p);
+  for (void* p : allocs) {
+    if (host_only) std::free(p);
+    else (void)hipFree(p);
+  }
   if (xstage) (void)hipFree(xstage);
   if (ystage) (void)hipFree(ystage);
 }
@@ -116,6 +125,14 @@ rvvla_mat_s::~rvvla_mat_s() {
 void* rvvla_mat_s::put(const void* host, size_t bytes) {
   void* d = nullptr;
   const size_t nb = bytes ? bytes : 4;
+  if (host_only) {  // host-validation mode: same copies, host memory
+    d = std::malloc(nb);
+    if (!d) throw Error(RVVLA_ERR_OUT_OF_MEMORY, "host allocation");
+    allocs.push_back(d);
+    device_bytes += (int64_t)bytes;
+    if (host && bytes) std::memcpy(d, host, bytes);
+    return d;
+  }
   int rc = rvvla::hip_check(hipMalloc(&d, nb), "hipMalloc");
   if (rc) throw Error(rc, rvvla::last_error());
   allocs.push_back(d);
@@ -150,10 +167,14 @@ static const void* put_values(rvvla_mat_s& M, int dtype, const V* v, size_t coun
 
 static std::unique_ptr<rvvla_mat_s> new_mat(int format, int dtype, int64_t m, int64_t n,
                                             int64_t nnz) {
-  int dev = 0;
-  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
-  if (rc) throw Error(RVVLA_ERR_NO_DEVICE, last_error());
+  int dev = -1;
+  const bool host_only = g_host_validation.load() != 0;
+  if (!host_only) {
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) throw Error(RVVLA_ERR_NO_DEVICE, last_error());
+  }
   auto M = std::make_unique<rvvla_mat_s>();
+  M->host_only = host_only;
   M->format = format;
   M->dtype = dtype;
   M->m = m;
@@ -561,7 +582,7 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   const int64_t total = (int64_t)a.mask.size();
   for (uint32_t c : a.col_id)
     if ((int64_t)c >= n) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: column out of range");
-  {
+  if (S > 0) {  // no dense part: out_order is empty and never read
     std::vector<char> seen((size_t)m, 0);
     for (int64_t i = 0; i < m; ++i) {
       const uint32_t r = a.out_order[(size_t)i];
@@ -604,7 +625,8 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   d.nchunks = cc;
   d.coff = M->put_vec(a.chunk_offsets);
   d.sptr = M->put_vec(sptr);
-  d.out_order = (const uint32_t*)M->put(a.out_order.data(), (size_t)(cc * L) * 4);
+  d.out_order = (const uint32_t*)M->put(S > 0 ? a.out_order.data() : nullptr,
+                                        S > 0 ? (size_t)(cc * L) * 4 : 0);
   d.col = M->put_vec(a.col_id);
   d.val = put_values(*M, dtype, vals.data(), vals.size());
   return M.release();
@@ -1072,6 +1094,11 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz, int64_t cnt_b
   });
 }
 
+int rvvla_set_host_validation(int on) {
+  rvvla::g_host_validation.store(on ? 1 : 0);
+  return RVVLA_OK;
+}
+
 int rvvla_mat_free(rvvla_mat_t A) {
   delete A;
   return RVVLA_OK;
@@ -1089,8 +1116,18 @@ int rvvla_mat_info(rvvla_mat_t A, int64_t* m, int64_t* n, int64_t* nnz, int* for
   return RVVLA_OK;
 }
 
+static int need_device(rvvla_mat_t A) {
+  if (!A) return RVVLA_ERR_INVALID_ARG;
+  if (A->host_only) {
+    rvvla::set_last_error("matrix was uploaded in host-validation mode");
+    return RVVLA_ERR_NO_DEVICE;
+  }
+  return RVVLA_OK;
+}
+
 int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y, void* stream) {
   if (!A || (A->n > 0 && !x) || (A->m > 0 && !y)) return RVVLA_ERR_INVALID_ARG;
+  if (int rc = need_device(A)) return rc;
   hipStream_t s = (hipStream_t)stream;
   switch (A->format) {
     case RVVLA_FMT_CSR: return rvvla::launch_csr(*A, alpha, x, beta, y, s);
@@ -1107,6 +1144,7 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
 
 int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx, double beta,
                void* Y, int64_t ldy, void* stream) {
+  if (int rc = need_device(A)) return rc;
   if (!A || k < 0 || ldx < k || ldy < k || (k > 0 && ((A->n > 0 && !X) || (A->m > 0 && !Y))))
     return RVVLA_ERR_INVALID_ARG;
   if (A->format != RVVLA_FMT_CSR) {
@@ -1117,7 +1155,7 @@ int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx, d
 }
 
 static int need_csr_real(rvvla_mat_t A) {
-  if (!A) return RVVLA_ERR_INVALID_ARG;
+  if (int rc = need_device(A)) return rc;
   if (A->format != RVVLA_FMT_CSR) {
     rvvla::set_last_error("differentiable SpMM needs a CSR matrix");
     return RVVLA_ERR_UNSUPPORTED_SPMV;
@@ -1185,6 +1223,7 @@ int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy
 }
 
 int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host) {
+  if (int rc = need_device(A)) return rc;
   if (!A || (A->n > 0 && !b_host)) return RVVLA_ERR_INVALID_ARG;
   const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;
   int rc;

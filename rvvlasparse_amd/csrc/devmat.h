@@ -167,6 +167,7 @@ struct rvvla_mat_s {
   int dtype = -1;
   int64_t m = 0, n = 0, nnz = 0;
   int device = 0;
+  bool host_only = false;  // uploaded in host-validation mode (no device copy)
   int64_t device_bytes = 0;
   std::vector<void*> allocs;
   rvvla::DevCSR csr;

@@ -20,25 +20,35 @@ __global__ __launch_bounds__(kWG) void k_sell64(
     typename Val<VT>::io* __restrict__ y, int64_t nslices, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t slice = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  // the slice index is uniform across the wavefront: keep it (and the slice
+  // descriptor loads) scalar
+  const int64_t slice =
+      __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
-  const int height = bs[slice];
+  const int height = bs[slice], width = cl[slice];
+  const int64_t base = cs[slice];
+  const int srow = sbs[slice];
   if (lane >= height) return;
-  const int width = cl[slice];
-  int64_t idx = (int64_t)cs[slice] + lane;
-  A a0 = 0, a1 = 0;
+  int64_t idx = base + lane;
+  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   int j = 0;
-  for (; j + 1 < width; j += 2) {
+  for (; j + 3 < width; j += 4) {
+    // all column and value loads first, then the four gathers
     const uint32_t c0 = col[idx], c1 = col[idx + height];
+    const uint32_t c2 = col[idx + 2 * (int64_t)height], c3 = col[idx + 3 * (int64_t)height];
     const A v0 = Val<VT>::get(val, idx), v1 = Val<VT>::get(val, idx + height);
+    const A v2 = Val<VT>::get(val, idx + 2 * (int64_t)height);
+    const A v3 = Val<VT>::get(val, idx + 3 * (int64_t)height);
     a0 += v0 * (A)x[c0];
     a1 += v1 * (A)x[c1];
-    idx += 2 * (int64_t)height;
+    a2 += v2 * (A)x[c2];
+    a3 += v3 * (A)x[c3];
+    idx += 4 * (int64_t)height;
   }
-  if (j < width) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
-  const uint32_t r = perm[sbs[slice] + lane];
-  y[r] = alpha * (a0 + a1) + scaled_y(beta, y, r);
+  for (; j < width; ++j, idx += height) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
+  const uint32_t r = perm[srow + lane];
+  y[r] = alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r);
 }
 
 template <class VT>
