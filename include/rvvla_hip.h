@@ -126,6 +126,15 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz,
  * (RVVLA_ERR_NO_DEVICE).  For validating layouts where no GPU is present. */
 int rvvla_set_host_validation(int on);
 
+/* Read-only view of one array of a matrix uploaded in host-validation mode
+ * (no reference counterpart; for layout tests).  CSR fields: "rst", "col",
+ * "blocks" and, when the column-sliced image was built, "xs.rst", "xs.col",
+ * "xs.val", "xs.blk", "xs.mask", "xs.base", "xs.pref" ("xs.val" in the
+ * storage type of the dtype).  *count is 0 for an absent array.  Matrices
+ * on a device give RVVLA_ERR_LOGIC. */
+int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t* count,
+                   int* elem_bytes);
+
 int rvvla_mat_free(rvvla_mat_t A);
 /* Shape and storage of a device matrix.  device_bytes = bytes of the
  * format's arrays resident in HBM (what one multiply streams, x/y apart). */

@@ -69,6 +69,7 @@ _sigs = {
     "rvvla_sell_upload": (_i32, [_i32, _i64, _i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp,
                                  _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "rvvla_set_host_validation": (_i32, [_i32]),
+    "rvvla_mat_view": (_i32, [_vp, _cp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -233,6 +234,16 @@ class DeviceMatrix:
         k = int(X.shape[1])
         check(lib.rvvla_spmm(self._h, k, alpha, _ptr(X), int(X.stride(0)), beta, _ptr(Y),
                              int(Y.stride(0)), _stream(stream, Y)))
+
+    def view(self, field: str) -> np.ndarray:
+        """Copy of one array of a matrix uploaded in host-validation mode."""
+        d, cnt, eb = C.c_void_p(), C.c_int64(), C.c_int()
+        check(lib.rvvla_mat_view(self._h, field.encode(), C.byref(d), C.byref(cnt), C.byref(eb)))
+        raw = _ELEM[eb.value]
+        if cnt.value == 0:
+            return np.empty(0, dtype=raw)
+        buf = (C.c_char * (cnt.value * eb.value)).from_address(d.value)
+        return np.frombuffer(buf, dtype=raw).copy()
 
     def spmv_host(self, b: np.ndarray, y: np.ndarray, alpha: float = 1.0, beta: float = 0.0):
         check(lib.rvvla_spmv_host(self._h, alpha, _ptr(b), beta, _ptr(y)))

@@ -225,8 +225,28 @@ static std::vector<int32_t> csr_blocks(int64_t m, const int* rst) {
   return b;
 }
 
+// Number of column slices for the sliced CSR image: `request` > 0 is taken
+// as given (1 = no slicing), otherwise one slice per XCD when x is larger
+// than one XCD's L2 (4 MiB) and more than a fifth of the entries lie over
+// 65536 columns away from the row's diagonal position.
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a) {
+static int choose_xslices(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request) {
+  if (request > 8) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xslices must be at most 8");
+  if (request == 0 || request == 1) return 0;
+  if (request > 1) return request;
+  int xcc = 8;
+  if (!M.host_only) {
+    if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, M.device) != hipSuccess)
+      xcc = 1;
+  }
+  xcc = std::min(xcc, 8);
+  const int64_t sx = M.dtype == RVVLA_F64 ? 8 : 4;
+  if (xcc < 2 || (int64_t)a.n * sx <= (int64_t(4) << 20)) return 0;
+  return far_fraction((int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, 65536) > 0.2 ? xcc : 0;
+}
+
+template <class V>
+rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   check_csr((int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, a.Rst, a.Col);
   auto M = new_mat(RVVLA_FMT_CSR, dtype, (int64_t)a.m, (int64_t)a.n, (int64_t)a.nz);
@@ -236,6 +256,24 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a) {
   M->csr.val = put_values(*M, dtype, a.Val, a.nz);
   M->csr.blocks = M->put_vec(blocks);
   M->csr.nblocks = (int64_t)blocks.size() - 1;
+  const int P = choose_xslices(*M, a, xslices);
+  if (P > 0) {
+    XSliceHost<V> X;
+    build_xslice<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, P);
+    DevXSlice& d = M->csr.xs;
+    d.P = P;
+    d.nq = X.nq;
+    d.pairs = X.base[(size_t)P];
+    d.nblk = (int64_t)X.blk.size() / 2;
+    d.rst = M->put_vec(X.rst);
+    d.col = M->put_vec(X.col);
+    d.val = put_values(*M, dtype, X.val.data(), X.val.size());
+    d.blk = M->put_vec(X.blk);
+    d.mask = M->put_vec(X.mask);
+    d.base = M->put_vec(X.base);
+    d.pref = M->put_vec(X.pref);
+    d.part = M->put(nullptr, (size_t)d.pairs * (dtype == RVVLA_F64 ? 8 : 4));
+  }
   return M.release();
 }
 
@@ -693,7 +731,8 @@ template rvvla_mat_t upload_vnec<float>(int, const spMtxVNEC<float>&, const spMt
 
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
-RVVLA_INST_UPLOAD(upload_csr, spMtxCRS)
+template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int);
+template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int);
 RVVLA_INST_UPLOAD(upload_hcsr, spMtxHCSR)
 RVVLA_INST_UPLOAD(upload_sell, spMtxSELL_C_Sigma)
 RVVLA_INST_UPLOAD(upload_csr5, spMtxCSR5)
@@ -741,7 +780,8 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
       L.arrays["Col"] = ArrayView{a->Col, (int64_t)a->nz, 4};
       L.arrays["Val"] = ArrayView{a->Val, (int64_t)a->nz, (int)sizeof(V)};
       L.scalars = {{"m", (double)a->m}, {"n", (double)a->n}, {"nz", (double)a->nz}};
-      L.upload = [a, dtype]() { return upload_csr<V>(dtype, *a); };
+      const int xs = param_int(P, "xslices", -1);
+      L.upload = [a, dtype, xs]() { return upload_csr<V>(dtype, *a, xs); };
       break;
     }
     case RVVLA_FMT_HCSR: {
@@ -1096,6 +1136,45 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz, int64_t cnt_b
 
 int rvvla_set_host_validation(int on) {
   rvvla::g_host_validation.store(on ? 1 : 0);
+  return RVVLA_OK;
+}
+
+int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t* count,
+                   int* elem_bytes) {
+  if (!A || !field || !data || !count || !elem_bytes) return RVVLA_ERR_INVALID_ARG;
+  if (!A->host_only) {
+    rvvla::set_last_error("rvvla_mat_view needs a matrix uploaded in host-validation mode");
+    return RVVLA_ERR_LOGIC;
+  }
+  if (A->format != RVVLA_FMT_CSR) {
+    rvvla::set_last_error("rvvla_mat_view: CSR matrices only");
+    return RVVLA_ERR_UNKNOWN_FORMAT;
+  }
+  const rvvla::DevCSR& c = A->csr;
+  const rvvla::DevXSlice& x = c.xs;
+  const int vb = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
+  const int64_t nblk = c.nblocks ? c.nblocks + 1 : 0;
+  const int64_t xnnz = x.P ? A->nnz : 0, xpairs = x.P ? x.pairs + 1 : 0;
+  const std::string f = field;
+  struct V { const void* p; int64_t n; int e; };
+  const V v = f == "rst"     ? V{c.rst, A->m + 1, 4}
+            : f == "col"     ? V{c.col, A->nnz, 4}
+            : f == "blocks"  ? V{c.blocks, nblk, 4}
+            : f == "xs.rst"  ? V{x.rst, xpairs, 4}
+            : f == "xs.col"  ? V{x.col, xnnz, 4}
+            : f == "xs.val"  ? V{x.val, xnnz, vb}
+            : f == "xs.blk"  ? V{x.blk, 2 * x.nblk, 4}
+            : f == "xs.mask" ? V{x.mask, x.P ? A->m : 0, 1}
+            : f == "xs.base" ? V{x.base, x.P ? x.P + 1 : 0, 4}
+            : f == "xs.pref" ? V{x.pref, x.P * x.nq, 4}
+                             : V{nullptr, -1, 0};
+  if (v.n < 0) {
+    rvvla::set_last_error("rvvla_mat_view: unknown field " + f);
+    return RVVLA_ERR_NOT_FOUND;
+  }
+  *data = v.p;
+  *count = v.n;
+  *elem_bytes = v.e;
   return RVVLA_OK;
 }
 

@@ -1,0 +1,169 @@
+// csr_xslice.cpp -- column-sliced device image of a CSR matrix.
+//
+// On gfx950 every XCD has its own L2 and workgroups are handed to the XCDs
+// round-robin by workgroup index.  A matrix whose columns are spread over
+// an x larger than one L2 (e.g. a power-law graph with uniform columns)
+// turns every x gather into a miss to the memory-side cache.  The sliced
+// image cuts the columns into P contiguous slices (P = number of XCDs) and
+// re-lays the entries slice-major: slice p holds, for each row with entries
+// in columns of slice p, those entries in CSR order.  The multiply then
+// runs slice p only on the workgroups of XCD p (workgroup b takes a block of
+// slice b mod P), so each XCD's gathers stay inside its own 1/P of x, which
+// fits its L2.  Each (row, slice) pair leaves one partial sum; a second
+// kernel adds a row's partials in slice order (deterministic) and applies
+// alpha and beta.
+//
+// Arrays (pairs = number of (row, slice) pairs with at least one entry):
+//   rst[pairs + 1]   entry offsets of the compressed rows, slice-major,
+//                    rows ascending within a slice
+//   col[nnz], val[nnz]
+//   blk[2 * nb * P]  (first, end) compressed-row range of each workgroup;
+//                    entry b belongs to slice b mod P (empty ranges pad)
+//   mask[m]          bit p set when row r has entries in slice p
+//   base[P + 1]      first pair of each slice
+//   pref[P * nq]     present rows of slice p in rows [0, 64 q), nq = m/64
+//                    rounded up; a wavefront of 64 rows finds the pair of
+//                    row r in slice p as base[p] + pref[p*nq + q] + the
+//                    number of lower lanes with bit p set.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "csr_blocks.h"
+#include "internal.h"
+
+namespace rvvla {
+
+// Adaptive row blocks over rows [r0, r1) of a CSR (same rules as the plain
+// CSR image: one row of more than kLongRow entries, or up to kStreamRows
+// rows holding at most kStreamCap entries).
+static void row_blocks(const std::vector<int32_t>& rst, int64_t r0, int64_t r1,
+                       std::vector<int32_t>& out) {
+  int64_t r = r0;
+  while (r < r1) {
+    out.push_back((int32_t)r);
+    if (rst[(size_t)r + 1] - rst[(size_t)r] > kLongRow) {
+      out.push_back((int32_t)++r);
+      continue;
+    }
+    const int64_t start = r;
+    int64_t cnt = 0;
+    while (r < r1 && r - start < kStreamRows) {
+      const int64_t len = rst[(size_t)r + 1] - rst[(size_t)r];
+      if (len > kLongRow || cnt + len > kStreamCap) break;
+      cnt += len;
+      ++r;
+    }
+    out.push_back((int32_t)r);
+  }
+}
+
+double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window) {
+  if (m == 0 || rst[m] == 0) return 0.0;
+  const double scale = (double)n / (double)m;
+  int64_t far = 0;
+#pragma omp parallel for schedule(static) reduction(+ : far)
+  for (int64_t r = 0; r < m; ++r) {
+    const double c0 = (double)r * scale;
+    for (int e = rst[r]; e < rst[r + 1]; ++e)
+      far += std::abs((double)col[e] - c0) > (double)window;
+  }
+  return (double)far / (double)rst[m];
+}
+
+template <class V>
+void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
+                  const V* val, int P) {
+  if (P < 2 || P > 8) throw Error(RVVLA_ERR_INVALID_ARG, "column slices: P in [2, 8]");
+  const int64_t nnz = m ? rst[m] : 0;
+  const int64_t w = (n + P - 1) / P;
+  X.P = P;
+  X.width = w;
+  X.mask.assign((size_t)m, 0);
+  // per row: entries in each slice
+  std::vector<int32_t> cnt((size_t)(m * P), 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t r = 0; r < m; ++r) {
+    int32_t* c = &cnt[(size_t)(r * P)];
+    for (int e = rst[r]; e < rst[r + 1]; ++e) c[col[e] / w]++;
+    uint8_t mk = 0;
+    for (int p = 0; p < P; ++p) mk |= (uint8_t)((c[p] > 0) << p);
+    X.mask[(size_t)r] = mk;
+  }
+  // pairs per slice and per 64-row group
+  const int64_t nq = (m + 63) / 64;
+  X.nq = nq;
+  X.pref.assign((size_t)(P * nq), 0);
+  X.base.assign((size_t)P + 1, 0);
+  for (int p = 0; p < P; ++p) {
+    int64_t acc = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+      X.pref[(size_t)(p * nq + q)] = (int32_t)acc;
+      const int64_t hi = std::min(m, (q + 1) * 64);
+      for (int64_t r = q * 64; r < hi; ++r) acc += (X.mask[(size_t)r] >> p) & 1;
+    }
+    X.base[(size_t)p + 1] = X.base[(size_t)p] + (int32_t)acc;
+  }
+  const int64_t pairs = X.base[(size_t)P];
+  // compressed row pointers, slice-major
+  X.rst.assign((size_t)pairs + 1, 0);
+  {
+    int64_t k = 0;
+    for (int p = 0; p < P; ++p)
+      for (int64_t r = 0; r < m; ++r) {
+        const int32_t c = cnt[(size_t)(r * P + p)];
+        if (c) {
+          X.rst[(size_t)k + 1] = X.rst[(size_t)k] + c;
+          ++k;
+        }
+      }
+  }
+  // entries: the pair of row r in slice p is base[p] + rank of r among the
+  // rows present in slice p
+  X.col.assign((size_t)nnz, 0);
+  X.val.assign((size_t)nnz, V(0));
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < nq; ++q) {
+    int32_t rank[8];
+    for (int p = 0; p < P; ++p) rank[p] = X.base[(size_t)p] + X.pref[(size_t)(p * nq + q)];
+    const int64_t hi = std::min(m, (q + 1) * 64);
+    for (int64_t r = q * 64; r < hi; ++r) {
+      int32_t fill[8];
+      for (int p = 0; p < P; ++p) fill[p] = X.rst[(size_t)rank[p]];
+      for (int e = rst[r]; e < rst[r + 1]; ++e) {
+        const int p = (int)(col[e] / w);
+        X.col[(size_t)fill[p]] = col[e];
+        X.val[(size_t)fill[p]++] = val[e];
+      }
+      for (int p = 0; p < P; ++p) rank[p] += (X.mask[(size_t)r] >> p) & 1;
+    }
+  }
+  // workgroup blocks: per slice, then interleaved so that block b is in
+  // slice b mod P
+  std::vector<std::vector<int32_t>> per((size_t)P);
+  size_t nb = 0;
+  for (int p = 0; p < P; ++p) {
+    row_blocks(X.rst, X.base[(size_t)p], X.base[(size_t)p + 1], per[(size_t)p]);
+    nb = std::max(nb, per[(size_t)p].size() / 2);
+  }
+  X.blk.assign(2 * nb * (size_t)P, 0);
+  for (size_t i = 0; i < nb; ++i)
+    for (int p = 0; p < P; ++p) {
+      const std::vector<int32_t>& v = per[(size_t)p];
+      const size_t o = 2 * (i * (size_t)P + (size_t)p);
+      if (2 * i + 1 < v.size()) {
+        X.blk[o] = v[2 * i];
+        X.blk[o + 1] = v[2 * i + 1];
+      } else {
+        X.blk[o] = X.blk[o + 1] = X.base[(size_t)p + 1];  // empty
+      }
+    }
+}
+
+template void build_xslice<double>(XSliceHost<double>&, int64_t, int64_t, const int*, const int*,
+                                   const double*, int);
+template void build_xslice<float>(XSliceHost<float>&, int64_t, int64_t, const int*, const int*,
+                                  const float*, int);
+
+}  // namespace rvvla

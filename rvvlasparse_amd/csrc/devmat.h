@@ -6,24 +6,39 @@
 #include <string>
 #include <vector>
 
+#include "csr_blocks.h"
 #include "rvvla_hip.h"
 
 namespace rvvla {
 
-// CSR: adaptive row blocks.  blocks[b] .. blocks[b+1] is the row range of
-// workgroup b: either one row longer than kLongRow entries, or up to
-// kStreamRows rows holding at most kStreamCap entries in total.
-constexpr int kWG = 256;
-constexpr int kStreamCap = 2048;
-constexpr int kStreamRows = 256;
-constexpr int kLongRow = 1024;
+// Column-sliced image of a CSR matrix (see csr_xslice.cpp): P slices, one
+// per XCD; compressed (row, slice) rows, workgroup blocks as (first, end)
+// pairs with block b in slice b mod P, presence masks, per-slice bases and
+// per-64-row prefixes, and the partial-sum scratch (one per pair).
+struct DevXSlice {
+  int P = 0;
+  int64_t nblk = 0, nq = 0, pairs = 0;
+  const int32_t* rst = nullptr;
+  const int32_t* col = nullptr;
+  const void* val = nullptr;
+  const int32_t* blk = nullptr;
+  const uint8_t* mask = nullptr;
+  const int32_t* base = nullptr;
+  const int32_t* pref = nullptr;
+  void* part = nullptr;
+};
 
+// CSR: adaptive row blocks.  blocks[b] .. blocks[b+1] is the row range of
+// workgroup b (see csr_blocks.h).  When xs.P > 0 the single-vector product
+// runs on the column-sliced image; the plain arrays serve SpMM and the
+// autograd products.
 struct DevCSR {
   const int32_t* rst = nullptr;
   const int32_t* col = nullptr;
   const void* val = nullptr;
   const int32_t* blocks = nullptr;
   int64_t nblocks = 0;
+  DevXSlice xs;
 };
 
 // HCSR: non-empty blocks only, grouped by block-row.

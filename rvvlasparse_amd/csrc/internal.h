@@ -65,8 +65,10 @@ std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts
 // ---- device matrices ----------------------------------------------------
 // Upload of each format's host container; value arrays are given as V
 // (double for F64, float otherwise) and narrowed for BF16/F16.
+// xslices: column slices of the sliced image (-1 = chosen from the matrix
+// and the device, 0 or 1 = plain CSR only, 2..8 = that many).
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a);
+rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a, int xslices = -1);
 template <class V>
 rvvla_mat_t upload_hcsr(int dtype, const SparseMatrixLib::spMtxHCSR<V>& a);
 template <class V>
@@ -82,6 +84,22 @@ rvvla_mat_t upload_vhcc(int dtype, const SparseMatrixLib::spMtxVHCC<V>& a);
 template <class V>
 rvvla_mat_t upload_vnec(int dtype, const SparseMatrixLib::spMtxVNEC<V>& a,
                         const SparseMatrixLib::spMtxCRS<V>& src);
+
+// Column-sliced CSR image (csr_xslice.cpp).
+template <class V>
+struct XSliceHost {
+  int P = 0;
+  int64_t width = 0, nq = 0;
+  std::vector<int32_t> rst, col, blk, base, pref;
+  std::vector<V> val;
+  std::vector<uint8_t> mask;
+};
+template <class V>
+void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
+                  const V* val, int P);
+// Fraction of entries whose column is more than `window` away from the
+// row's diagonal position r * n / m.
+double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window);
 
 // 16-bit value encodings (round to nearest even).
 uint16_t f32_to_bf16(float f);

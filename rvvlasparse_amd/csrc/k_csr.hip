@@ -14,11 +14,17 @@
 //  * long block: one row with more than 1024 entries; all 256 threads
 //    stride over it, then wave and workgroup reductions.  In the 16-bit
 //    value forms the partial sums are compensated (TwoSum) there.
+//
+// Column-sliced image (csr_xslice.cpp): the same kernel runs over the
+// compressed (row, slice) rows with blocks given as (first, end) pairs,
+// block b of slice b mod P so that each XCD gathers from its own slice of x,
+// and writes one partial per pair (alpha = 1, beta = 0); k_xs_combine then
+// adds each row's partials in slice order and applies alpha and beta.
 #include "kcommon.h"
 
 namespace rvvla {
 
-template <class VT, bool COMP>
+template <class VT, bool COMP, bool PAIRS>
 __global__ __launch_bounds__(kWG) void k_csr_adaptive(
     const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
     const VT* __restrict__ val, const int32_t* __restrict__ blocks,
@@ -28,7 +34,9 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
   __shared__ A prod[kStreamCap];
   __shared__ A wsum[2 * (kWG / 64)];
   const int tid = threadIdx.x;
-  const int r0 = blocks[blockIdx.x], r1 = blocks[blockIdx.x + 1];
+  const int r0 = PAIRS ? blocks[2 * blockIdx.x] : blocks[blockIdx.x];
+  const int r1 = PAIRS ? blocks[2 * blockIdx.x + 1] : blocks[blockIdx.x + 1];
+  if (r0 == r1) return;  // padding block of a sliced image (uniform)
   const int e0 = rst[r0], e1 = rst[r1];
   const int nnz = e1 - e0, nrows = r1 - r0;
 
@@ -108,6 +116,28 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
   }
 }
 
+// y[r] = alpha * (sum over slices p in order of part[pair(r, p)]) + beta*y[r].
+// One wavefront per 64 rows; the pair of row r in slice p is base[p] +
+// pref[p*nq + q] + (lower lanes of the wavefront with bit p set).
+template <class IO, class A>
+__global__ __launch_bounds__(kWG) void k_xs_combine(
+    const uint8_t* __restrict__ mask, const int32_t* __restrict__ base,
+    const int32_t* __restrict__ pref, const IO* __restrict__ part, IO* __restrict__ y,
+    int64_t m, int64_t nq, int P, A alpha, A beta) {
+  const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  const int64_t q = r >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const unsigned mk = r < m ? mask[r] : 0u;
+  A s = 0;
+  for (int p = 0; p < P; ++p) {
+    const bool has = (mk >> p) & 1u;
+    const uint64_t bal = __ballot(has);
+    if (has) s += (A)part[base[p] + pref[p * nq + q] + __popcll(bal & below)];
+  }
+  if (r < m) y[r] = alpha * s + scaled_y(beta, y, r);
+}
+
 // SpMM: Y = alpha*A*X + beta*Y, X n x k, Y m x k (row-major).  One
 // wavefront per row; lanes cover the k right-hand sides (k <= 64 per pass),
 // so X rows are read as contiguous k-vectors.
@@ -138,16 +168,35 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevCSR& d = A.csr;
-  if (d.nblocks == 0) return RVVLA_OK;
   const bool comp = sizeof(VT) == 2;
+  if (d.xs.P > 0) {
+    const DevXSlice& xs = d.xs;
+    if (A.m == 0) return RVVLA_OK;
+    if (xs.nblk > 0) {
+      if (comp)
+        hipLaunchKernelGGL((k_csr_adaptive<VT, true, true>), dim3((unsigned)xs.nblk), dim3(kWG), 0,
+                           s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
+                           (IO*)xs.part, (AC)1, (AC)0);
+      else
+        hipLaunchKernelGGL((k_csr_adaptive<VT, false, true>), dim3((unsigned)xs.nblk), dim3(kWG),
+                           0, s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
+                           (IO*)xs.part, (AC)1, (AC)0);
+      if (int rc = hip_check(hipGetLastError(), "k_csr_adaptive (sliced) launch")) return rc;
+    }
+    const unsigned g = (unsigned)((A.m + kWG - 1) / kWG);
+    hipLaunchKernelGGL((k_xs_combine<IO, AC>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base, xs.pref,
+                       (const IO*)xs.part, (IO*)y, A.m, xs.nq, xs.P, (AC)alpha, (AC)beta);
+    return hip_check(hipGetLastError(), "k_xs_combine launch");
+  }
+  if (d.nblocks == 0) return RVVLA_OK;
   if (comp)
-    hipLaunchKernelGGL((k_csr_adaptive<VT, true>), dim3((unsigned)d.nblocks), dim3(kWG), 0, s,
-                       d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y, (AC)alpha,
-                       (AC)beta);
+    hipLaunchKernelGGL((k_csr_adaptive<VT, true, false>), dim3((unsigned)d.nblocks), dim3(kWG), 0,
+                       s, d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y,
+                       (AC)alpha, (AC)beta);
   else
-    hipLaunchKernelGGL((k_csr_adaptive<VT, false>), dim3((unsigned)d.nblocks), dim3(kWG), 0, s,
-                       d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y, (AC)alpha,
-                       (AC)beta);
+    hipLaunchKernelGGL((k_csr_adaptive<VT, false, false>), dim3((unsigned)d.nblocks), dim3(kWG),
+                       0, s, d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y,
+                       (AC)alpha, (AC)beta);
   return hip_check(hipGetLastError(), "k_csr_adaptive launch");
 }
 

@@ -4,8 +4,9 @@ seeded inputs, through the C-ABI (librvvla_hip.so).
 FP64 / FP32: |y - y_oracle| <= 4*(len+2)*u*(|alpha|*sum|a||x| + |beta|*|y0|)
 per row (u = 2^-53 / 2^-24).  16-bit values (FP32 x, y and accumulation):
 per row |y - y_fp64| <= (u_v + (4*len+10)*2^-24) * (|alpha|*sum|a||x| +
-|beta|*|y0|) against the FP64 oracle on the unrounded matrix, u_v = 2^-9
-(bf16) or 2^-12 (fp16) -- the value rounding plus FP32 arithmetic.
+|beta|*|y0|) against the FP64 oracle on the unrounded matrix, u_v = 2^-8
+(bf16, 8-bit significand) or 2^-11 (fp16, 11-bit significand), the unit
+roundoff of round-to-nearest value storage, plus FP32 arithmetic.
 """
 from __future__ import annotations
 
@@ -124,7 +125,7 @@ def test_spmv_16bit_values(fmt, dtype, name):
     x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
     y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), alpha, beta,
                      y032.astype(np.float64))
-    uv = 2.0 ** -9 if dtype == R.BF16 else 2.0 ** -12
+    uv = 2.0 ** -8 if dtype == R.BF16 else 2.0 ** -11
     ln = np.diff(a.rst).astype(np.float64)
     scale = abs(alpha) * O.csr_absdot(a.rst, a.col, a.val, x32.astype(np.float64)) \
         + abs(beta) * np.abs(y032.astype(np.float64))
@@ -133,6 +134,37 @@ def test_spmv_16bit_values(fmt, dtype, name):
     assert np.all(err <= bound), f"{fmt}/{name}: max err/bound {np.max(err / np.maximum(bound, 1e-300)):.3f}"
     if np.linalg.norm(y64) > 0:
         rel = np.linalg.norm(y - y64) / np.linalg.norm(y64)
+        assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
+
+
+@pytest.mark.parametrize("P", [3, 8])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("name", ["rand_long", "powerlaw", "one_long_row", "empty", "lap64"])
+def test_csr_column_slices(P, dtype, name):
+    """The column-sliced CSR image (one slice per XCD, partials combined in
+    slice order) against the oracle, with and without beta."""
+    a = case(name)
+    x64, y064 = inputs(a, 6)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        y = run_gpu("csr", dtype, a, x64, y064, alpha, beta, f"xslices={P}")
+        v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        y_ref = O.csr_spmv(a.rst, a.col, v, x, alpha, beta, y0)
+        assert_within_bound(y, y_ref, a.rst, a.col, v, x, alpha, beta, y0, dt,
+                            f"csr xslices={P}/{name}/a={alpha},b={beta}")
+
+
+@pytest.mark.parametrize("dtype", [R.BF16, R.F16])
+def test_csr_column_slices_16bit(dtype):
+    a = case("powerlaw")
+    x64, y064 = inputs(a, 7)
+    y = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, "xslices=8")
+    y_plain = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, "xslices=0")
+    x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+    y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 1.0, 0.5,
+                     y032.astype(np.float64))
+    for out in (y, y_plain):
+        rel = np.linalg.norm(out - y64) / np.linalg.norm(y64)
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 

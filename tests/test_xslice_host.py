@@ -1,0 +1,138 @@
+"""Column-sliced CSR image, checked on the host (host-validation uploads):
+the slice masks, bases and 64-row prefixes, the slice-major re-lay of the
+entries (each (row, slice) pair holds exactly that row's entries in that
+slice, in CSR order, bit for bit), the workgroup blocks (block b in slice
+b mod P, covering the slice's pairs in order under the adaptive-block
+limits), and the sliced product emulated from those arrays against the
+oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import rvvlasparse_amd as R
+from helpers import assert_within_bound, rand_csr
+from oracle import oracle as O
+
+LONG, CAP, ROWS = 1024, 2048, 256
+
+
+@pytest.fixture(autouse=True)
+def host_validation():
+    R.check(R.lib.rvvla_set_host_validation(1))
+    yield
+    R.check(R.lib.rvvla_set_host_validation(0))
+
+
+def _matrix():
+    rst, col, val = rand_csr(4000, 30000, 0.0015, 7, empty_every=11,
+                             long_rows=(9, 3000, 3001))
+    return R.CSR(4000, 30000, rst, col, val)
+
+
+def _check_image(a: R.CSR, M: R.DeviceMatrix, P: int):
+    w = -(-a.n // P)
+    mask = M.view("xs.mask")
+    base = M.view("xs.base").astype(np.int64)
+    pref = M.view("xs.pref").astype(np.int64)
+    xrst = M.view("xs.rst").astype(np.int64)
+    xcol = M.view("xs.col").astype(np.int64)
+    xval = M.view("xs.val").view(np.float64 if M.dtype == R.F64 else np.float32)
+    blk = M.view("xs.blk").astype(np.int64).reshape(-1, 2)
+    m, nq = a.m, -(-a.m // 64)
+    assert mask.shape == (m,) and base.shape == (P + 1,) and pref.shape == (P * nq,)
+    lens = np.diff(a.rst)
+    rows = np.repeat(np.arange(m), lens)
+    sl = a.col.astype(np.int64) // w
+    want = np.zeros(m, np.int64)
+    np.bitwise_or.at(want, rows, 1 << sl)
+    assert np.array_equal(mask.astype(np.int64), want)
+    for p in range(P):
+        has = (want >> p) & 1
+        assert base[p + 1] - base[p] == has.sum()
+        cum = np.concatenate([[0], np.cumsum(has)])
+        assert np.array_equal(pref[p * nq:(p + 1) * nq], cum[np.arange(nq) * 64])
+    assert xrst[0] == 0 and xrst[-1] == a.nnz and len(xrst) == base[P] + 1
+    vals = a.values_for(M.dtype)
+    for r in range(m):
+        e0, e1 = a.rst[r], a.rst[r + 1]
+        for p in range(P):
+            if not (want[r] >> p) & 1:
+                continue
+            k = base[p] + pref[p * nq + r // 64] + int(
+                sum((want[r0] >> p) & 1 for r0 in range(r // 64 * 64, r)))
+            sel = np.nonzero(sl[e0:e1] == p)[0] + e0
+            assert np.array_equal(xcol[xrst[k]:xrst[k + 1]], a.col[sel]), (r, p)
+            assert np.array_equal(xval[xrst[k]:xrst[k + 1]], vals[sel]), (r, p)
+    # blocks: entry b belongs to slice b mod P; a slice's non-empty blocks
+    # tile its pairs in order and keep the adaptive limits
+    plen = np.diff(xrst)
+    for p in range(P):
+        b = blk[p::P]
+        ne = b[b[:, 1] > b[:, 0]]
+        assert np.all(b[:, 0] >= base[p]) and np.all(b[:, 1] <= base[p + 1])
+        if base[p + 1] > base[p]:
+            assert ne[0, 0] == base[p] and ne[-1, 1] == base[p + 1]
+            assert np.array_equal(ne[1:, 0], ne[:-1, 1])
+        for lo, hi in ne:
+            if hi - lo == 1 and plen[lo] > LONG:
+                continue
+            assert hi - lo <= ROWS and plen[lo:hi].sum() <= CAP and plen[lo:hi].max() <= LONG
+    return base, pref, xrst, xcol, xval
+
+
+@pytest.mark.parametrize("P", [2, 3, 8])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_sliced_image(P, dtype):
+    a = _matrix()
+    M = R.convert("csr", a, dtype, f"xslices={P}").upload()
+    base, pref, xrst, xcol, xval = _check_image(a, M, P)
+    # the sliced product: partial per pair, then slice-order sums per row
+    dt = np.float64 if dtype == R.F64 else np.float32
+    x = R.gen_uniform(a.n, -1, 1, seed=5).astype(dt)
+    part = O.csr_spmv(xrst.astype(np.int32), xcol.astype(np.int32), xval, x, 1.0, 0.0,
+                      np.zeros(len(xrst) - 1, dt))
+    mask = M.view("xs.mask").astype(np.int64)
+    nq = -(-a.m // 64)
+    y = np.zeros(a.m, dt)
+    for r in range(a.m):
+        s = dt(0)
+        for p in range(P):
+            if (mask[r] >> p) & 1:
+                q0 = r // 64 * 64
+                k = base[p] + pref[p * nq + r // 64] + int(((mask[q0:r] >> p) & 1).sum())
+                s = dt(s + part[k])
+        y[r] = s
+    y_ref = O.csr_spmv(a.rst, a.col, a.values_for(dtype), x, 1.0, 0.0, np.zeros(a.m, dt))
+    assert_within_bound(y, y_ref, a.rst, a.col, a.values_for(dtype), x, 1.0, 0.0,
+                        np.zeros(a.m, dt), dt, f"sliced P={P}")
+
+
+def test_sliced_16bit_values():
+    a = _matrix()
+    M = R.convert("csr", a, R.BF16, "xslices=8").upload()
+    xv = M.view("xs.val")
+    assert xv.dtype == np.uint16 and xv.shape == (a.nnz,)
+
+
+def test_auto_choice():
+    """Auto: sliced when x exceeds one L2 and columns are far from the
+    diagonal; plain for a banded matrix or a small x; xslices=1 disables."""
+    n = 600_000
+    g = R.gen_random_rows(n, 2, seed=1)  # diagonal + one uniform column
+    assert len(R.convert("csr", g, R.F64).upload().view("xs.base")) == 9
+    assert len(R.convert("csr", g, R.F64, "xslices=1").upload().view("xs.base")) == 0
+    lap = R.gen_laplacian5(800)  # n = 640000, banded
+    assert len(R.convert("csr", lap, R.F64).upload().view("xs.base")) == 0
+    small = _matrix()
+    assert len(R.convert("csr", small, R.F64).upload().view("xs.base")) == 0
+
+
+def test_empty_and_bad_request():
+    e = R.CSR(5, 7, np.zeros(6, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    M = R.convert("csr", e, R.F64, "xslices=4").upload()
+    assert np.array_equal(M.view("xs.base"), np.zeros(5, np.uint32))
+    assert M.view("xs.blk").size % 8 == 0
+    with pytest.raises(R.RvvlaError) as ex:
+        R.convert("csr", _matrix(), R.F64, "xslices=9").upload()
+    assert ex.value.code == -10
