@@ -7,8 +7,14 @@
 // candidate item kernels on the first matrix of each key, on the device,
 // with HIP events; the winner is cached for the process (and, with the
 // parameter cache=<path>, in a text file of "key fmt param" lines).
-// Candidates: CSR rows with 8 or 32 rows per item, SELL-64 with sigma 1 or
-// 4096 and 1 slice per item.
+// Candidates: CSR rows with the slot count chosen from the mean row length,
+// with a quarter and with four times as many slots, and SELL-64 with sigma
+// 1 or 4096.
+//
+// Work items: after all matrices are laid out, each matrix is cut into runs
+// of rows (CSR) or slices (SELL) of about equal cost, about 8 items per
+// wavefront of the persistent grid, so the queue costs few atomics and the
+// last items finish together.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,8 +36,10 @@ namespace {
 
 struct Choice {
   int fmt = 0;    // 0 = CSR rows, 1 = SELL-64 slices
-  int param = 8;  // CSR: rows per item; SELL: sigma
+  int param = 0;  // CSR: slots per row (0 = from the mean row length); SELL: sigma
 };
+
+constexpr int kBatchGrid = 2048;  // persistent workgroups (4 wavefronts each)
 
 std::mutex g_tune_mu;
 std::map<std::string, Choice> g_tune_cache;
@@ -108,7 +116,20 @@ struct HostMat {
   int k;
 };
 
-// Appends one matrix under a given choice; returns its BatchMat.
+// Right-hand sides per lane: the largest power of two <= vmax dividing k.
+static int vec_width(int k, int vmax) {
+  int v = 1;
+  while (v * 2 <= vmax && k % (v * 2) == 0) v *= 2;
+  return v;
+}
+
+static int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Appends matrices under their choices, then cuts the work items.
 template <class V>
 struct Builder {
   std::vector<int32_t> rst, col, cl, cs, scol, perm;
@@ -122,22 +143,31 @@ struct Builder {
     M.m = (int32_t)h.m;
     M.n = (int32_t)h.n;
     M.k = h.k;
-    M.kp = 1;
-    while (M.kp < h.k) M.kp <<= 1;
     M.fmt = c.fmt;
+    M.vw = vec_width(h.k, sizeof(V) == 8 ? 2 : 4);
+    M.lpe = pow2_ceil((h.k + M.vw - 1) / M.vw);
+    // X and Y blocks start on 4-element boundaries (vector loads)
+    x_elems = (x_elems + 3) & ~int64_t(3);
+    y_elems = (y_elems + 3) & ~int64_t(3);
     M.x_off = x_elems;
     M.y_off = y_elems;
     x_elems += h.n * h.k;
     y_elems += h.m * h.k;
-    const int32_t mi = (int32_t)mats.size();
     if (c.fmt == 0) {
+      const int64_t nnz = h.rst[h.m];
+      int slots = c.param;
+      if (slots <= 0) {
+        const double mean = h.m ? (double)nnz / (double)h.m : 1.0;
+        slots = 1;
+        while (slots * 2 <= 64 && slots * 2 * 4 <= mean) slots *= 2;  // ~4 entries per lane
+      }
+      slots = std::max(1, std::min(pow2_ceil(slots), 64 / M.lpe));
+      M.gw = M.lpe * slots;
       M.rst_off = (int64_t)rst.size();
       M.ent_off = (int64_t)col.size();
       rst.insert(rst.end(), h.rst, h.rst + h.m + 1);
-      col.insert(col.end(), h.col, h.col + h.rst[h.m]);
-      val.insert(val.end(), h.val, h.val + h.rst[h.m]);
-      for (int64_t r = 0; r < h.m; r += c.param)
-        items.push_back(BatchItem{mi, (int32_t)r, (int32_t)c.param, 0});
+      col.insert(col.end(), h.col, h.col + nnz);
+      val.insert(val.end(), h.val, h.val + nnz);
     } else {
       spMtxCRS<V> a((size_t)h.m, (size_t)h.n, (size_t)h.rst[h.m], h.rst, h.col, h.val);
       spMtxSELL_C_Sigma<V> s;
@@ -150,14 +180,49 @@ struct Builder {
       scol.insert(scol.end(), s.vCol.begin(), s.vCol.end());
       sval.insert(sval.end(), s.vVal.begin(), s.vVal.end());
       perm.insert(perm.end(), s.vPerm.begin(), s.vPerm.end());
-      const int64_t nsl = (h.m + 63) / 64;
-      for (int64_t q = 0; q < nsl; ++q) items.push_back(BatchItem{mi, (int32_t)q, 1, 0});
     }
     mats.push_back(M);
   }
 
+  // Cost units (wavefront load steps): a CSR row costs len * lpe / 64 plus a
+  // fixed share for its pointers and reduction, a SELL slice width * k.
+  double unit_cost(const BatchMat& M, int64_t u) const {
+    if (M.fmt == 0) {
+      const int64_t len = rst[(size_t)(M.rst_off + u + 1)] - rst[(size_t)(M.rst_off + u)];
+      return ((double)len * M.lpe + 4.0 * M.gw) / 64.0;
+    }
+    return (double)cl[(size_t)(M.sl_off + u)] * M.k + 2.0;
+  }
+
+  void cut_items(int waves) {
+    items.clear();
+    double total = 0;
+    for (const BatchMat& M : mats) {
+      const int64_t units = M.fmt == 0 ? M.m : (M.m + 63) / 64;
+      for (int64_t u = 0; u < units; ++u) total += unit_cost(M, u);
+    }
+    const double target = std::max(16.0, total / (8.0 * std::max(1, waves)));
+    for (size_t i = 0; i < mats.size(); ++i) {
+      const BatchMat& M = mats[i];
+      const int64_t units = M.fmt == 0 ? M.m : (M.m + 63) / 64;
+      const int64_t step = M.fmt == 0 ? 64 / M.gw : 1;  // rows per wavefront step
+      int64_t u0 = 0;
+      double acc = 0;
+      for (int64_t u = 0; u < units; ++u) {
+        acc += unit_cost(M, u);
+        const bool cut = acc >= target && (u + 1 - u0) % step == 0;
+        if (cut || u + 1 == units) {
+          items.push_back(BatchItem{(int32_t)i, (int32_t)u0, (int32_t)(u + 1 - u0), 0});
+          u0 = u + 1;
+          acc = 0;
+        }
+      }
+    }
+  }
+
   // move everything to the device
   void upload(rvvla_batch_s& B, int dtype) {
+    cut_items(kBatchGrid * (rvvla::kWG / 64));
     B.mats = mats;
     B.items = items;
     B.args.mats = (const BatchMat*)B.put(mats.data(), mats.size() * sizeof(BatchMat));
@@ -179,8 +244,7 @@ struct Builder {
     if (rc) throw Error(rc, rvvla::last_error());
     B.dtype = dtype;
     B.count = (int)mats.size();
-    const int64_t waves = ((int64_t)items.size() + 1) / 2;
-    B.grid = (int)std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, 256 * 8));
+    B.grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)items.size() + 3) / 4, kBatchGrid));
   }
 };
 
@@ -244,11 +308,17 @@ rvvla_batch_t create(int dtype, int count, const int64_t* m, const int64_t* n,
     Choice c;
     if (force_fmt >= 0) {
       c.fmt = force_fmt;
-      c.param = rvvla::param_int(P, "param", force_fmt == 0 ? 8 : 1);
+      c.param = rvvla::param_int(P, "param", force_fmt == 0 ? 0 : 1);
     } else if (g_tune_cache.count(key)) {
       c = g_tune_cache[key];
     } else if (tune) {
-      const Choice cands[] = {{0, 8}, {0, 32}, {1, 1}, {1, 4096}};
+      // CSR slot counts around the one the mean row length suggests
+      const int64_t nnz = h.rst[h.m];
+      const double mean = h.m ? (double)nnz / (double)h.m : 1.0;
+      int s0 = 1;
+      while (s0 * 2 <= 64 && s0 * 2 * 4 <= mean) s0 *= 2;
+      const Choice cands[] = {{0, s0}, {0, std::max(1, s0 / 4)}, {0, std::min(64, s0 * 4)},
+                              {1, 1}, {1, 4096}};
       double best = 1e30;
       for (const Choice& cand : cands) {
         const double t = time_choice<V>(h, cand, dtype);

@@ -201,34 +201,6 @@ void check_csr_public(int64_t m, int64_t n, int64_t nnz, const int* rst, const i
   check_csr(m, n, nnz, rst, col);
 }
 
-// Adaptive row blocks (see k_csr.hip).
-static std::vector<int32_t> csr_blocks(int64_t m, const int* rst) {
-  std::vector<int32_t> b;
-  b.reserve((size_t)(m / 64 + 2));
-  b.push_back(0);
-  int64_t r = 0;
-  while (r < m) {
-    if (rst[r + 1] - rst[r] > kLongRow) {
-      b.push_back((int32_t)++r);
-      continue;
-    }
-    const int64_t start = r;
-    int64_t cnt = 0;
-    while (r < m && r - start < kStreamRows) {
-      const int64_t len = rst[r + 1] - rst[r];
-      if (len > kLongRow || cnt + len > kStreamCap) break;
-      cnt += len;
-      ++r;
-    }
-    b.push_back((int32_t)r);
-  }
-  return b;
-}
-
-// Number of column slices for the sliced CSR image: `request` > 0 is taken
-// as given (1 = no slicing), otherwise one slice per XCD when x is larger
-// than one XCD's L2 (4 MiB) and more than a fifth of the entries lie over
-// 65536 columns away from the row's diagonal position.
 template <class V>
 static int choose_xslices(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request) {
   if (request > 8) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xslices must be at most 8");
@@ -250,12 +222,12 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   check_csr((int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, a.Rst, a.Col);
   auto M = new_mat(RVVLA_FMT_CSR, dtype, (int64_t)a.m, (int64_t)a.n, (int64_t)a.nz);
-  const std::vector<int32_t> blocks = csr_blocks((int64_t)a.m, a.Rst);
+  const std::vector<int32_t> blocks = csr_row_blocks(a.Rst, 0, (int64_t)a.m);
   M->csr.rst = (const int32_t*)M->put(a.Rst, (a.m + 1) * sizeof(int));
   M->csr.col = (const int32_t*)M->put(a.Col, a.nz * sizeof(int));
   M->csr.val = put_values(*M, dtype, a.Val, a.nz);
   M->csr.blocks = M->put_vec(blocks);
-  M->csr.nblocks = (int64_t)blocks.size() - 1;
+  M->csr.nblocks = (int64_t)blocks.size() / 2;
   const int P = choose_xslices(*M, a, xslices);
   if (P > 0) {
     XSliceHost<V> X;
@@ -399,6 +371,8 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
   d.sbs = M->put_vec(a.vSBs);
   d.col = M->put_vec(a.vCol);
   d.perm = M->put_vec(a.vPerm);
+  d.perm_identity = true;
+  for (int64_t s = 0; s < m && d.perm_identity; ++s) d.perm_identity = a.vPerm[(size_t)s] == (uint32_t)s;
   d.val = put_values(*M, dtype, a.vVal.data(), a.vVal.size());
   return M.release();
 }
@@ -652,12 +626,12 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
     }
   }
   auto M = new_mat(RVVLA_FMT_LAV, dtype, m, n, a.nz);
-  const std::vector<int32_t> blocks = csr_blocks(m, sp.Rst);
+  const std::vector<int32_t> blocks = csr_row_blocks(sp.Rst, 0, m);
   M->csr.rst = (const int32_t*)M->put(sp.Rst, (sp.m + 1) * sizeof(int));
   M->csr.col = (const int32_t*)M->put(sp.Col, sp.nz * sizeof(int));
   M->csr.val = put_values(*M, dtype, sp.Val, sp.nz);
   M->csr.blocks = M->put_vec(blocks);
-  M->csr.nblocks = (int64_t)blocks.size() - 1;
+  M->csr.nblocks = (int64_t)blocks.size() / 2;
   DevLAV& d = M->lav;
   d.nseg = S;
   d.nchunks = cc;
@@ -807,7 +781,7 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
     case RVVLA_FMT_SELL: {
       auto s = std::make_shared<spMtxSELL_C_Sigma<V>>();
       build_sell<V>(*s, *crs, param_int(P, "C", 64),
-                    param_int(P, "sigma", default_sell_sigma((int64_t)crs->m, (int64_t)crs->nz)));
+                    param_int(P, "sigma", default_sell_sigma((int64_t)crs->m, (int64_t)crs->nz, crs->Rst)));
       L.obj = s;
       reg(L, "vCl", s->vCl);
       reg(L, "vCs", s->vCs);
@@ -1153,7 +1127,7 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
   const rvvla::DevCSR& c = A->csr;
   const rvvla::DevXSlice& x = c.xs;
   const int vb = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
-  const int64_t nblk = c.nblocks ? c.nblocks + 1 : 0;
+  const int64_t nblk = 2 * c.nblocks;
   const int64_t xnnz = x.P ? A->nnz : 0, xpairs = x.P ? x.pairs + 1 : 0;
   const std::string f = field;
   struct V { const void* p; int64_t n; int e; };

@@ -20,10 +20,19 @@ namespace rvvla {
 
 using namespace SparseMatrixLib;
 
-int default_sell_sigma(int64_t m, int64_t nnz) {
-  (void)nnz;
-  (void)m;
-  return 16384;
+int default_sell_sigma(int64_t m, int64_t nnz, const int* rst) {
+  // sigma = 1 keeps the row order (the permutation is the identity, so the
+  // multiply needs no vPerm read); take it when 64-row slices in natural
+  // order pad by at most 1 % of the entries, else sort in windows of 16384
+  if (m == 0 || nnz == 0 || !rst) return 16384;
+  int64_t padded = 0;
+  for (int64_t r0 = 0; r0 < m; r0 += 64) {
+    const int64_t r1 = std::min(m, r0 + 64);
+    int64_t w = 0;
+    for (int64_t r = r0; r < r1; ++r) w = std::max<int64_t>(w, rst[r + 1] - rst[r]);
+    padded += w * (r1 - r0);
+  }
+  return (double)(padded - nnz) <= 0.01 * (double)nnz ? 1 : 16384;
 }
 
 template <class V>

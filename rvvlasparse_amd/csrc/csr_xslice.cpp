@@ -18,7 +18,8 @@
 //                    rows ascending within a slice
 //   col[nnz], val[nnz]
 //   blk[2 * nb * P]  (first, end) compressed-row range of each workgroup;
-//                    entry b belongs to slice b mod P (empty ranges pad)
+//                    entry b belongs to slice b mod P (empty ranges pad);
+//                    within a slice the long-row blocks come first
 //   mask[m]          bit p set when row r has entries in slice p
 //   base[P + 1]      first pair of each slice
 //   pref[P * nq]     present rows of slice p in rows [0, 64 q), nq = m/64
@@ -35,28 +36,38 @@
 
 namespace rvvla {
 
-// Adaptive row blocks over rows [r0, r1) of a CSR (same rules as the plain
-// CSR image: one row of more than kLongRow entries, or up to kStreamRows
-// rows holding at most kStreamCap entries).
-static void row_blocks(const std::vector<int32_t>& rst, int64_t r0, int64_t r1,
-                       std::vector<int32_t>& out) {
+std::vector<int32_t> csr_row_blocks(const int* rst, int64_t r0, int64_t r1) {
+  std::vector<int32_t> longs, streams;
   int64_t r = r0;
   while (r < r1) {
-    out.push_back((int32_t)r);
-    if (rst[(size_t)r + 1] - rst[(size_t)r] > kLongRow) {
-      out.push_back((int32_t)++r);
+    if (rst[r + 1] - rst[r] > kLongRow) {
+      longs.push_back((int32_t)r);
+      ++r;
       continue;
     }
     const int64_t start = r;
     int64_t cnt = 0;
     while (r < r1 && r - start < kStreamRows) {
-      const int64_t len = rst[(size_t)r + 1] - rst[(size_t)r];
+      const int64_t len = rst[r + 1] - rst[r];
       if (len > kLongRow || cnt + len > kStreamCap) break;
       cnt += len;
       ++r;
     }
-    out.push_back((int32_t)r);
+    streams.push_back((int32_t)start);
+    streams.push_back((int32_t)r);
   }
+  // longest rows first: a long row is the longest-running block, so it
+  // should not start last
+  std::stable_sort(longs.begin(), longs.end(),
+                   [&](int32_t a, int32_t b) { return rst[a + 1] - rst[a] > rst[b + 1] - rst[b]; });
+  std::vector<int32_t> out;
+  out.reserve(2 * longs.size() + streams.size());
+  for (int32_t l : longs) {
+    out.push_back(l);
+    out.push_back(l + 1);
+  }
+  out.insert(out.end(), streams.begin(), streams.end());
+  return out;
 }
 
 double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window) {
@@ -144,7 +155,7 @@ void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const 
   std::vector<std::vector<int32_t>> per((size_t)P);
   size_t nb = 0;
   for (int p = 0; p < P; ++p) {
-    row_blocks(X.rst, X.base[(size_t)p], X.base[(size_t)p + 1], per[(size_t)p]);
+    per[(size_t)p] = csr_row_blocks(X.rst.data(), X.base[(size_t)p], X.base[(size_t)p + 1]);
     nb = std::max(nb, per[(size_t)p].size() / 2);
   }
   X.blk.assign(2 * nb * (size_t)P, 0);

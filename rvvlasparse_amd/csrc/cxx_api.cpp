@@ -193,7 +193,7 @@ void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>&
 // SELL-C-sigma keys "C" (64), "sigma"
 RVVLA_CONVERT_BOTH(spMtxSELL_C_Sigma,
                    rvvla::build_sell(d, s, iparam(p, "C", 64),
-                                     iparam(p, "sigma", rvvla::default_sell_sigma((int64_t)s.m, (int64_t)s.nz))),
+                                     iparam(p, "sigma", rvvla::default_sell_sigma((int64_t)s.m, (int64_t)s.nz, s.Rst))),
                    rvvla::upload_sell(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
 // CSR5 keys "omega" (64), "sigma" (0 = chosen from the row lengths)
 RVVLA_CONVERT_BOTH(spMtxCSR5, rvvla::build_csr5(d, s, iparam(p, "omega", 64), iparam(p, "sigma", 0)),

@@ -64,6 +64,7 @@ struct DevSELL {
   const uint32_t* col = nullptr;
   const uint32_t* perm = nullptr;
   const void* val = nullptr;
+  bool perm_identity = false;  // vPerm[s] == s: rows written in place
 };
 
 // CSR5 (omega = 64): tiles as the host container holds them, plus the
@@ -154,8 +155,12 @@ struct BatchMat {
   int64_t sl_off = 0;    // into the concatenated slice tables (SELL item kernel)
   int64_t slot_off = 0;  // into the concatenated SELL slots
   int64_t row_off = 0;   // into the concatenated SELL permutations
-  int64_t x_off = 0, y_off = 0;
-  int32_t m = 0, n = 0, k = 0, kp = 1, fmt = 0, pad = 0;
+  int64_t x_off = 0, y_off = 0;  // multiples of 4 elements
+  int32_t m = 0, n = 0, k = 0, fmt = 0;
+  int32_t vw = 1;   // right-hand sides per lane: a power of two (<= 4) dividing k
+  int32_t lpe = 1;  // lanes per entry: power of two >= k / vw
+  int32_t gw = 1;   // lanes per row in the CSR item kernel: lpe * slots, <= 64
+  int32_t pad = 0;
 };
 struct BatchItem {
   int32_t mat, first, count, pad;

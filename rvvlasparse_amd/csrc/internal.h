@@ -56,7 +56,7 @@ void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCR
                 int nparts);
 
 // Default partition counts and parameters chosen for gfx950 (256 CUs).
-int default_sell_sigma(int64_t m, int64_t nnz);
+int default_sell_sigma(int64_t m, int64_t nnz, const int* rst);
 int default_csr5_sigma(int64_t m, int64_t nnz);
 int default_parts(int64_t m, int64_t nnz, int64_t items_per_part);
 // Rows [b[t], b[t+1]) of part t: contiguous, about nnz/parts entries each.
@@ -84,6 +84,12 @@ rvvla_mat_t upload_vhcc(int dtype, const SparseMatrixLib::spMtxVHCC<V>& a);
 template <class V>
 rvvla_mat_t upload_vnec(int dtype, const SparseMatrixLib::spMtxVNEC<V>& a,
                         const SparseMatrixLib::spMtxCRS<V>& src);
+
+// Adaptive CSR row blocks over rows [r0, r1) as (first, end) pairs: one
+// row of more than kLongRow entries, or up to kStreamRows rows holding at
+// most kStreamCap entries; the long-row blocks first, longest first, then
+// the others in row order.
+std::vector<int32_t> csr_row_blocks(const int* rst, int64_t r0, int64_t r1);
 
 // Column-sliced CSR image (csr_xslice.cpp).
 template <class V>

@@ -3,16 +3,17 @@
 // queue (no reference counterpart: the reference multiplies one matrix with
 // one vector per call).
 //
-// Every matrix is cut into work items (a run of rows for the CSR item
-// kernel, a run of 64-row slices for the SELL item kernel); a persistent
-// grid of wavefronts pops items from a global counter with one atomic per
-// item and runs them until the queue is empty.  A small kernel clears the
-// counter first (a fill kernel, not a memset, so the pair can be captured in
-// a graph and replayed).
-//  * CSR item: one row at a time per wavefront; lanes are (entry slot, rhs)
-//    pairs with kp = next power of two >= k lanes per entry, so 64/kp entries
-//    are in flight per step and each entry reads kp consecutive values of X;
-//    lanes of one rhs are combined by a butterfly over the slot bits.
+// Every matrix is cut into work items of about equal cost (a run of rows
+// for the CSR item kernel, a run of 64-row slices for the SELL item
+// kernel); a persistent grid of wavefronts pops items from a global counter
+// with one atomic per item and runs them until the queue is empty.  A small
+// kernel clears the counter first (a fill kernel, not a memset, so the pair
+// can be captured in a graph and replayed).
+//  * CSR item: a wavefront works on 64/gw rows at once, gw lanes per row.
+//    The lanes of a row are (slot, rhs group) pairs: lpe lanes cover the k
+//    right-hand sides of one entry, vw consecutive ones each (one 16-byte
+//    or narrower load of X per entry), and the gw/lpe slots stride over the
+//    row's entries; the slots are combined by a butterfly.
 //  * SELL item: lane per row of a 64-row slice, k accumulators per lane.
 // Y = alpha * A X + beta * Y, X n x k and Y m x k row-major per matrix.
 #include "kcommon.h"
@@ -24,28 +25,74 @@ __global__ void k_batch_reset(int* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
-template <class T>
-__device__ void batch_csr_rows(const BatchMat& M, int first, int count, const int32_t* __restrict__ rst,
-                               const int32_t* __restrict__ col, const T* __restrict__ val,
-                               const T* __restrict__ X, T* __restrict__ Y, T alpha, T beta) {
+template <class T, int V>
+struct VecOf {
+  typedef T type __attribute__((ext_vector_type(V)));
+};
+
+template <class T, int V>
+__device__ __forceinline__ void load_v(const T* __restrict__ p, T (&o)[V]) {
+  if constexpr (V == 1) {
+    o[0] = *p;
+  } else {
+    const typename VecOf<T, V>::type v = *(const typename VecOf<T, V>::type*)p;
+#pragma unroll
+    for (int q = 0; q < V; ++q) o[q] = v[q];
+  }
+}
+
+template <class T, int V>
+__device__ void batch_csr_rows(const BatchMat& M, int first, int count,
+                               const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
+                               const T* __restrict__ val, const T* __restrict__ X,
+                               T* __restrict__ Y, T alpha, T beta) {
   const int lane = threadIdx.x & 63;
-  const int k = M.k, kp = M.kp;
-  const int slot = lane / kp, r = lane & (kp - 1), per = 64 / kp;
+  const int k = M.k, lpe = M.lpe, G = M.gw, S = G / lpe;
+  const int g = lane / G, t = lane & (G - 1), slot = t / lpe, c = (t & (lpe - 1)) * V;
+  const bool act = c < k;
   const int32_t* rp = rst + M.rst_off;
   const int32_t* cp = col + M.ent_off;
   const T* vp = val + M.ent_off;
-  const T* xp = X + M.x_off;
-  T* yp = Y + M.y_off;
+  const T* xp = X + M.x_off + c;
+  T* yp = Y + M.y_off + c;
   const int last = min(first + count, M.m);
-  for (int row = first; row < last; ++row) {
-    const int e0 = rp[row], e1 = rp[row + 1];
-    T acc = 0;
-    if (r < k)
-      for (int e = e0 + slot; e < e1; e += per) acc += vp[e] * xp[(int64_t)cp[e] * k + r];
-    for (int off = 32; off >= kp; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (slot == 0 && r < k) {
-      T* yo = yp + (int64_t)row * k + r;
-      *yo = alpha * acc + scaled_y(beta, yo, 0);
+  for (int row0 = first; row0 < last; row0 += 64 / G) {
+    const int row = row0 + g;
+    T a[V], b[V];
+#pragma unroll
+    for (int q = 0; q < V; ++q) a[q] = b[q] = T(0);
+    if (row < last && act) {
+      const int e1 = rp[row + 1];
+      int e = rp[row] + slot;
+      for (; e + S < e1; e += 2 * S) {
+        const int c0 = cp[e], c1 = cp[e + S];
+        const T v0 = vp[e], v1 = vp[e + S];
+        T x0[V], x1[V];
+        load_v<T, V>(xp + (int64_t)c0 * k, x0);
+        load_v<T, V>(xp + (int64_t)c1 * k, x1);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          a[q] += v0 * x0[q];
+          b[q] += v1 * x1[q];
+        }
+      }
+      if (e < e1) {
+        const T v0 = vp[e];
+        T x0[V];
+        load_v<T, V>(xp + (int64_t)cp[e] * k, x0);
+#pragma unroll
+        for (int q = 0; q < V; ++q) a[q] += v0 * x0[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < V; ++q) a[q] += b[q];
+    for (int off = G >> 1; off >= lpe; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < V; ++q) a[q] += __shfl_xor(a[q], off, 64);
+    if (row < last && act && slot == 0) {
+      T* yo = yp + (int64_t)row * k;
+#pragma unroll
+      for (int q = 0; q < V; ++q) yo[q] = alpha * a[q] + scaled_y(beta, yo, q);
     }
   }
 }
@@ -100,11 +147,17 @@ __global__ __launch_bounds__(kWG) void k_batch(BatchArgs a, const T* __restrict_
     if (item >= a.nitems) return;  // every wavefront leaves once the queue is empty
     const BatchItem it = a.items[item];
     const BatchMat M = a.mats[it.mat];
-    if (M.fmt == 0)
-      batch_csr_rows<T>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
-    else
+    if (M.fmt == 0) {
+      if (M.vw == 4)
+        batch_csr_rows<T, 4>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+      else if (M.vw == 2)
+        batch_csr_rows<T, 2>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+      else
+        batch_csr_rows<T, 1>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+    } else {
       batch_sell_slices<T>(M, it.first, it.count, a.cl, a.cs, a.scol, sval, a.perm, X, Y, alpha,
                            beta);
+    }
   }
 }
 

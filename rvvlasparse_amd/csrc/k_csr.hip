@@ -8,23 +8,29 @@
 // upload:
 //  * stream block: up to 256 consecutive rows with <= 2048 entries in all.
 //    The entries are read fully coalesced, multiplied by x and staged in
-//    LDS; then each row is summed out of LDS by a group of 1..64 lanes
-//    (group width = largest power of two <= 256 / rows), in entry order
-//    within a lane and by a butterfly across lanes.
+//    LDS; then a row of at most kShortSum entries is summed out of LDS by
+//    its own thread in entry order, a longer one by a whole wavefront
+//    (strided lanes, then a butterfly).
 //  * long block: one row with more than 1024 entries; all 256 threads
 //    stride over it, then wave and workgroup reductions.  In the 16-bit
 //    value forms the partial sums are compensated (TwoSum) there.
 //
+// Blocks are (first, end) row pairs, long-row blocks first (longest first)
+// so that no long row starts at the end of the launch.
+//
 // Column-sliced image (csr_xslice.cpp): the same kernel runs over the
-// compressed (row, slice) rows with blocks given as (first, end) pairs,
-// block b of slice b mod P so that each XCD gathers from its own slice of x,
-// and writes one partial per pair (alpha = 1, beta = 0); k_xs_combine then
-// adds each row's partials in slice order and applies alpha and beta.
+// compressed (row, slice) rows, block b of slice b mod P so that each XCD
+// gathers from its own slice of x (entries and columns are then streamed
+// with nontemporal loads, to leave L2 to x), and writes one partial per pair
+// (alpha = 1, beta = 0); k_xs_combine then adds each row's partials in
+// slice order and applies alpha and beta.
 #include "kcommon.h"
 
 namespace rvvla {
 
-template <class VT, bool COMP, bool PAIRS>
+constexpr int kShortSum = 16;
+
+template <class VT, bool COMP, bool NT>
 __global__ __launch_bounds__(kWG) void k_csr_adaptive(
     const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
     const VT* __restrict__ val, const int32_t* __restrict__ blocks,
@@ -34,8 +40,7 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
   __shared__ A prod[kStreamCap];
   __shared__ A wsum[2 * (kWG / 64)];
   const int tid = threadIdx.x;
-  const int r0 = PAIRS ? blocks[2 * blockIdx.x] : blocks[blockIdx.x];
-  const int r1 = PAIRS ? blocks[2 * blockIdx.x + 1] : blocks[blockIdx.x + 1];
+  const int r0 = blocks[2 * blockIdx.x], r1 = blocks[2 * blockIdx.x + 1];
   if (r0 == r1) return;  // padding block of a sliced image (uniform)
   const int e0 = rst[r0], e1 = rst[r1];
   const int nnz = e1 - e0, nrows = r1 - r0;
@@ -44,15 +49,16 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
     Acc<A, COMP> a;
     int e = e0 + tid;
     for (; e + 3 * kWG < e1; e += 4 * kWG) {
-      const int c0 = col[e], c1 = col[e + kWG], c2 = col[e + 2 * kWG], c3 = col[e + 3 * kWG];
-      const A v0 = Val<VT>::get(val, e), v1 = Val<VT>::get(val, e + kWG);
-      const A v2 = Val<VT>::get(val, e + 2 * kWG), v3 = Val<VT>::get(val, e + 3 * kWG);
+      const int c0 = ldi<NT>(col, e), c1 = ldi<NT>(col, e + kWG);
+      const int c2 = ldi<NT>(col, e + 2 * kWG), c3 = ldi<NT>(col, e + 3 * kWG);
+      const A v0 = ldv<NT>(val, e), v1 = ldv<NT>(val, e + kWG);
+      const A v2 = ldv<NT>(val, e + 2 * kWG), v3 = ldv<NT>(val, e + 3 * kWG);
       a.add(v0 * (A)x[c0]);
       a.add(v1 * (A)x[c1]);
       a.add(v2 * (A)x[c2]);
       a.add(v3 * (A)x[c3]);
     }
-    for (; e < e1; e += kWG) a.add(Val<VT>::get(val, e) * (A)x[col[e]]);
+    for (; e < e1; e += kWG) a.add(ldv<NT>(val, e) * (A)x[ldi<NT>(col, e)]);
     a = group_reduce(a, 64);
     const int w = tid >> 6;
     if ((tid & 63) == 0) {
@@ -75,66 +81,79 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
     return;
   }
 
-  // stream block: coalesced products into LDS
+  // stream block: row bounds of thread tid's row (fetched before the
+  // barrier), then coalesced products into LDS
+  const int t_row = r0 + tid;
+  int lo = 0, hi = 0;
+  if (tid < nrows) {
+    lo = rst[t_row] - e0;
+    hi = rst[t_row + 1] - e0;
+  }
   {
     int k = tid;
     for (; k + 3 * kWG < nnz; k += 4 * kWG) {
-      const int c0 = col[e0 + k], c1 = col[e0 + k + kWG];
-      const int c2 = col[e0 + k + 2 * kWG], c3 = col[e0 + k + 3 * kWG];
-      const A v0 = Val<VT>::get(val, e0 + k), v1 = Val<VT>::get(val, e0 + k + kWG);
-      const A v2 = Val<VT>::get(val, e0 + k + 2 * kWG), v3 = Val<VT>::get(val, e0 + k + 3 * kWG);
+      const int c0 = ldi<NT>(col, e0 + k), c1 = ldi<NT>(col, e0 + k + kWG);
+      const int c2 = ldi<NT>(col, e0 + k + 2 * kWG), c3 = ldi<NT>(col, e0 + k + 3 * kWG);
+      const A v0 = ldv<NT>(val, e0 + k), v1 = ldv<NT>(val, e0 + k + kWG);
+      const A v2 = ldv<NT>(val, e0 + k + 2 * kWG), v3 = ldv<NT>(val, e0 + k + 3 * kWG);
       prod[k] = v0 * (A)x[c0];
       prod[k + kWG] = v1 * (A)x[c1];
       prod[k + 2 * kWG] = v2 * (A)x[c2];
       prod[k + 3 * kWG] = v3 * (A)x[c3];
     }
-    for (; k < nnz; k += kWG) prod[k] = Val<VT>::get(val, e0 + k) * (A)x[col[e0 + k]];
+    for (; k < nnz; k += kWG) prod[k] = ldv<NT>(val, e0 + k) * (A)x[ldi<NT>(col, e0 + k)];
   }
   __syncthreads();
-  if (nrows > kWG / 2) {
-    if (tid < nrows) {
-      const int r = r0 + tid;
-      const int lo = rst[r] - e0, hi = rst[r + 1] - e0;
-      A s = 0;
-      for (int k = lo; k < hi; ++k) s += prod[k];
-      y[r] = alpha * s + scaled_y(beta, y, r);
+  // rows of at most kShortSum products: their own thread, in entry order;
+  // longer rows: the whole wavefront of the row's thread, one row at a time
+  const bool is_long = tid < nrows && hi - lo > kShortSum;
+  if (tid < nrows && !is_long) {
+    A s = 0;
+    for (int k = lo; k < hi; ++k) s += prod[k];
+    y[t_row] = alpha * s + scaled_y(beta, y, t_row);
+  }
+  const int lane = tid & 63;
+  uint64_t lm = __ballot(is_long);
+  while (lm) {
+    const int l = __ffsll((unsigned long long)lm) - 1;
+    lm &= lm - 1;
+    const int rlo = __shfl(lo, l, 64), rhi = __shfl(hi, l, 64);
+    Acc<A, false> a;
+    for (int k = rlo + lane; k < rhi; k += 64) a.add(prod[k]);
+    a = group_reduce(a, 64);
+    if (lane == 0) {
+      const int r = r0 + (tid - lane) + l;
+      y[r] = alpha * a.value() + scaled_y(beta, y, r);
     }
-    return;
-  }
-  int width = 64;
-  while (width * nrows > kWG) width >>= 1;
-  const int g = tid / width, l = tid & (width - 1);
-  Acc<A, false> a;
-  if (g < nrows) {
-    const int lo = rst[r0 + g] - e0, hi = rst[r0 + g + 1] - e0;
-    for (int k = lo + l; k < hi; k += width) a.add(prod[k]);
-  }
-  a = group_reduce(a, width);
-  if (g < nrows && l == 0) {
-    const int r = r0 + g;
-    y[r] = alpha * a.value() + scaled_y(beta, y, r);
   }
 }
 
 // y[r] = alpha * (sum over slices p in order of part[pair(r, p)]) + beta*y[r].
 // One wavefront per 64 rows; the pair of row r in slice p is base[p] +
-// pref[p*nq + q] + (lower lanes of the wavefront with bit p set).
+// pref[p*nq + q] + (lower lanes of the wavefront with bit p set).  All
+// partial loads are issued before the (in-order) sum.
 template <class IO, class A>
 __global__ __launch_bounds__(kWG) void k_xs_combine(
     const uint8_t* __restrict__ mask, const int32_t* __restrict__ base,
     const int32_t* __restrict__ pref, const IO* __restrict__ part, IO* __restrict__ y,
     int64_t m, int64_t nq, int P, A alpha, A beta) {
   const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  const int64_t q = r >> 6;
+  const int q = __builtin_amdgcn_readfirstlane((int)(r >> 6));
   const int lane = threadIdx.x & 63;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const unsigned mk = r < m ? mask[r] : 0u;
-  A s = 0;
-  for (int p = 0; p < P; ++p) {
-    const bool has = (mk >> p) & 1u;
+  A v[8];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const bool has = p < P && ((mk >> p) & 1u);
     const uint64_t bal = __ballot(has);
-    if (has) s += (A)part[base[p] + pref[p * nq + q] + __popcll(bal & below)];
+    v[p] = A(0);
+    if (has) v[p] = (A)part[base[p] + pref[(int64_t)p * nq + q] + __popcll(bal & below)];
   }
+  A s = A(0);
+#pragma unroll
+  for (int p = 0; p < 8; ++p)
+    if ((mk >> p) & 1u) s += v[p];
   if (r < m) y[r] = alpha * s + scaled_y(beta, y, r);
 }
 
@@ -165,6 +184,7 @@ __global__ __launch_bounds__(kWG) void k_csr_spmm(
 template <class VT>
 static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                    hipStream_t s) {
+  const bool xs_nt = nt_enabled();
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevCSR& d = A.csr;
@@ -177,8 +197,12 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
         hipLaunchKernelGGL((k_csr_adaptive<VT, true, true>), dim3((unsigned)xs.nblk), dim3(kWG), 0,
                            s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
                            (IO*)xs.part, (AC)1, (AC)0);
-      else
+      else if (xs_nt)
         hipLaunchKernelGGL((k_csr_adaptive<VT, false, true>), dim3((unsigned)xs.nblk), dim3(kWG),
+                           0, s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
+                           (IO*)xs.part, (AC)1, (AC)0);
+      else
+        hipLaunchKernelGGL((k_csr_adaptive<VT, false, false>), dim3((unsigned)xs.nblk), dim3(kWG),
                            0, s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
                            (IO*)xs.part, (AC)1, (AC)0);
       if (int rc = hip_check(hipGetLastError(), "k_csr_adaptive (sliced) launch")) return rc;

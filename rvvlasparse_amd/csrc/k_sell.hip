@@ -6,12 +6,14 @@
 // straight to the original row through vPerm, fusing the reference's
 // PREPARATION / GET_RESULTS permutations into the multiply as its vector
 // path does (reference src/rvv/spmv_mtx_sell_c_sigma.cpp:60-85):
-// y[perm[s]] = alpha * acc + beta * y[perm[s]].
+// y[perm[s]] = alpha * acc + beta * y[perm[s]].  When the permutation is
+// the identity (sigma = 1) it is not read at all.  FP64 values and their
+// columns are read with nontemporal loads.
 #include "kcommon.h"
 
 namespace rvvla {
 
-template <class VT>
+template <class VT, bool IDENT, bool NT>
 __global__ __launch_bounds__(kWG) void k_sell64(
     const int32_t* __restrict__ cl, const int32_t* __restrict__ cs,
     const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
@@ -35,19 +37,20 @@ __global__ __launch_bounds__(kWG) void k_sell64(
   int j = 0;
   for (; j + 3 < width; j += 4) {
     // all column and value loads first, then the four gathers
-    const uint32_t c0 = col[idx], c1 = col[idx + height];
-    const uint32_t c2 = col[idx + 2 * (int64_t)height], c3 = col[idx + 3 * (int64_t)height];
-    const A v0 = Val<VT>::get(val, idx), v1 = Val<VT>::get(val, idx + height);
-    const A v2 = Val<VT>::get(val, idx + 2 * (int64_t)height);
-    const A v3 = Val<VT>::get(val, idx + 3 * (int64_t)height);
+    const uint32_t c0 = ldu<NT>(col, idx), c1 = ldu<NT>(col, idx + height);
+    const uint32_t c2 = ldu<NT>(col, idx + 2 * (int64_t)height);
+    const uint32_t c3 = ldu<NT>(col, idx + 3 * (int64_t)height);
+    const A v0 = ldv<NT>(val, idx), v1 = ldv<NT>(val, idx + height);
+    const A v2 = ldv<NT>(val, idx + 2 * (int64_t)height);
+    const A v3 = ldv<NT>(val, idx + 3 * (int64_t)height);
     a0 += v0 * (A)x[c0];
     a1 += v1 * (A)x[c1];
     a2 += v2 * (A)x[c2];
     a3 += v3 * (A)x[c3];
     idx += 4 * (int64_t)height;
   }
-  for (; j < width; ++j, idx += height) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
-  const uint32_t r = perm[srow + lane];
+  for (; j < width; ++j, idx += height) a0 += ldv<NT>(val, idx) * (A)x[ldu<NT>(col, idx)];
+  const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
   y[r] = alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r);
 }
 
@@ -59,9 +62,21 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
   const DevSELL& d = A.sell;
   if (d.nslices == 0) return RVVLA_OK;
   const unsigned grid = (unsigned)((d.nslices + kWG / 64 - 1) / (kWG / 64));
-  hipLaunchKernelGGL((k_sell64<VT>), dim3(grid), dim3(kWG), 0, s, d.cl, d.cs, d.bs, d.sbs, d.col,
-                     (const VT*)d.val, d.perm, (const IO*)x, (IO*)y, d.nslices, (AC)alpha,
-                     (AC)beta);
+  // nontemporal streaming pays for 8-byte values; narrower values run
+  // faster with ordinary loads
+  const bool nt = nt_enabled() && sizeof(VT) == 8;
+#define RVVLA_SELL_LAUNCH(ID, NT_)                                                              \
+  hipLaunchKernelGGL((k_sell64<VT, ID, NT_>), dim3(grid), dim3(kWG), 0, s, d.cl, d.cs, d.bs,    \
+                     d.sbs, d.col, (const VT*)d.val, d.perm, (const IO*)x, (IO*)y, d.nslices,   \
+                     (AC)alpha, (AC)beta)
+  if (d.perm_identity) {
+    if (nt) RVVLA_SELL_LAUNCH(true, true);
+    else RVVLA_SELL_LAUNCH(true, false);
+  } else {
+    if (nt) RVVLA_SELL_LAUNCH(false, true);
+    else RVVLA_SELL_LAUNCH(false, false);
+  }
+#undef RVVLA_SELL_LAUNCH
   return hip_check(hipGetLastError(), "k_sell64 launch");
 }
 

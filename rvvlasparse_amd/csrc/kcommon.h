@@ -3,6 +3,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <cstdint>
 
 #include "devmat.h"
@@ -21,12 +23,18 @@ struct Val<double> {
   using acc = double;
   using io = double;
   static __device__ __forceinline__ double get(const double* p, int64_t i) { return p[i]; }
+  static __device__ __forceinline__ double get_nt(const double* p, int64_t i) {
+    return __builtin_nontemporal_load(p + i);
+  }
 };
 template <>
 struct Val<float> {
   using acc = float;
   using io = float;
   static __device__ __forceinline__ float get(const float* p, int64_t i) { return p[i]; }
+  static __device__ __forceinline__ float get_nt(const float* p, int64_t i) {
+    return __builtin_nontemporal_load(p + i);
+  }
 };
 template <>
 struct Val<bf16_t> {
@@ -35,13 +43,47 @@ struct Val<bf16_t> {
   static __device__ __forceinline__ float get(const bf16_t* p, int64_t i) {
     return __uint_as_float((uint32_t)p[i].bits << 16);
   }
+  static __device__ __forceinline__ float get_nt(const bf16_t* p, int64_t i) {
+    return __uint_as_float((uint32_t)__builtin_nontemporal_load((const uint16_t*)p + i) << 16);
+  }
 };
 template <>
 struct Val<_Float16> {
   using acc = float;
   using io = float;
   static __device__ __forceinline__ float get(const _Float16* p, int64_t i) { return (float)p[i]; }
+  static __device__ __forceinline__ float get_nt(const _Float16* p, int64_t i) {
+    return (float)__builtin_nontemporal_load(p + i);
+  }
 };
+
+// Streamed loads: the nontemporal form marks the line as not worth keeping
+// in the caches (used where x must stay resident in L2).
+template <bool NT, class VT>
+__device__ __forceinline__ typename Val<VT>::acc ldv(const VT* p, int64_t i) {
+  if constexpr (NT) return Val<VT>::get_nt(p, i);
+  else return Val<VT>::get(p, i);
+}
+template <bool NT>
+__device__ __forceinline__ int32_t ldi(const int32_t* p, int64_t i) {
+  if constexpr (NT) return __builtin_nontemporal_load(p + i);
+  else return p[i];
+}
+template <bool NT>
+__device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) {
+  if constexpr (NT) return __builtin_nontemporal_load(p + i);
+  else return p[i];
+}
+
+// Nontemporal streaming of the matrix arrays is on unless the environment
+// sets RVVLA_NT=0 (for comparisons).
+static inline bool nt_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("RVVLA_NT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 
 // beta * y[i], with y not read at all when beta == 0 (BLAS convention: y
 // is then write-only, so NaN/Inf already in y do not propagate).

@@ -92,12 +92,20 @@ def test_batch_autotuned(dtype):
     assert len(keys) >= 1
 
 
-@pytest.mark.parametrize("params", ["format=0,param=8", "format=0,param=32", "format=1,param=1",
-                                    "format=1,param=4096"])
+@pytest.mark.parametrize("params", ["format=0,param=1", "format=0,param=4", "format=0,param=64",
+                                    "format=1,param=1", "format=1,param=4096"])
 def test_batch_forced_item_kernels(params):
     mats, ks = _batch_inputs(12, seed=5)
     B = R.Batch(mats, ks, R.F32, params)
     _run_and_check(B, mats, ks, R.F32, 1.5, 0.5)
+
+
+@pytest.mark.parametrize("params", ["format=0,param=2", "format=0,param=16"])
+def test_batch_forced_csr_f64(params):
+    """FP64: two right-hand sides per lane when k is even, one otherwise."""
+    mats, ks = _batch_inputs(12, seed=6)
+    B = R.Batch(mats, ks, R.F64, params)
+    _run_and_check(B, mats, ks, R.F64, -1.0, 0.25)
 
 
 def test_batch_graph_replays():

@@ -30,6 +30,32 @@ def _matrix():
     return R.CSR(4000, 30000, rst, col, val)
 
 
+def _check_blocks(ne, plen, lo_row, hi_row):
+    """Non-empty (first, end) blocks: long-row blocks first, longest first,
+    then stream blocks in row order; together they tile [lo_row, hi_row)."""
+    is_long = np.array([hi - lo == 1 and plen[lo] > LONG for lo, hi in ne], bool)
+    nl = int(is_long.sum())
+    assert np.all(is_long[:nl]) and not np.any(is_long[nl:])
+    lens = plen[ne[:nl, 0]]
+    assert np.all(lens[:-1] >= lens[1:])
+    st = ne[nl:]
+    assert np.all(st[1:, 0] > st[:-1, 0])
+    order = ne[np.argsort(ne[:, 0], kind="stable")]
+    if hi_row > lo_row:
+        assert order[0, 0] == lo_row and order[-1, 1] == hi_row
+        assert np.array_equal(order[1:, 0], order[:-1, 1])
+    for lo, hi in st:
+        assert hi - lo <= ROWS and plen[lo:hi].sum() <= CAP and plen[lo:hi].max() <= LONG
+
+
+def test_plain_blocks():
+    a = _matrix()
+    M = R.convert("csr", a, R.F64, "xslices=0").upload()
+    blk = M.view("blocks").astype(np.int64).reshape(-1, 2)
+    _check_blocks(blk, np.diff(a.rst), 0, a.m)
+    assert len(M.view("xs.base")) == 0
+
+
 def _check_image(a: R.CSR, M: R.DeviceMatrix, P: int):
     w = -(-a.n // P)
     mask = M.view("xs.mask")
@@ -65,19 +91,12 @@ def _check_image(a: R.CSR, M: R.DeviceMatrix, P: int):
             assert np.array_equal(xcol[xrst[k]:xrst[k + 1]], a.col[sel]), (r, p)
             assert np.array_equal(xval[xrst[k]:xrst[k + 1]], vals[sel]), (r, p)
     # blocks: entry b belongs to slice b mod P; a slice's non-empty blocks
-    # tile its pairs in order and keep the adaptive limits
+    # tile its pairs and keep the adaptive limits
     plen = np.diff(xrst)
     for p in range(P):
         b = blk[p::P]
-        ne = b[b[:, 1] > b[:, 0]]
         assert np.all(b[:, 0] >= base[p]) and np.all(b[:, 1] <= base[p + 1])
-        if base[p + 1] > base[p]:
-            assert ne[0, 0] == base[p] and ne[-1, 1] == base[p + 1]
-            assert np.array_equal(ne[1:, 0], ne[:-1, 1])
-        for lo, hi in ne:
-            if hi - lo == 1 and plen[lo] > LONG:
-                continue
-            assert hi - lo <= ROWS and plen[lo:hi].sum() <= CAP and plen[lo:hi].max() <= LONG
+        _check_blocks(b[b[:, 1] > b[:, 0]], plen, base[p], base[p + 1])
     return base, pref, xrst, xcol, xval
 
 
