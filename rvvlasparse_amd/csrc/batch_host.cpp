@@ -194,41 +194,70 @@ struct Builder {
     return (double)cl[(size_t)(M.sl_off + u)] * M.k + 2.0;
   }
 
-  void cut_items(int waves) {
+  // Cuts every matrix into items of about `target` cost and packs the
+  // matrices into `nq` queues, longest processing time first (each matrix
+  // to the least loaded queue); a queue lists its matrices' items in order.
+  void cut_items(int waves, int nq, std::vector<int64_t>& qbase) {
     items.clear();
+    std::vector<double> cost(mats.size(), 0.0);
     double total = 0;
-    for (const BatchMat& M : mats) {
-      const int64_t units = M.fmt == 0 ? M.m : (M.m + 63) / 64;
-      for (int64_t u = 0; u < units; ++u) total += unit_cost(M, u);
-    }
-    const double target = std::max(16.0, total / (8.0 * std::max(1, waves)));
     for (size_t i = 0; i < mats.size(); ++i) {
       const BatchMat& M = mats[i];
       const int64_t units = M.fmt == 0 ? M.m : (M.m + 63) / 64;
-      const int64_t step = M.fmt == 0 ? 64 / M.gw : 1;  // rows per wavefront step
-      int64_t u0 = 0;
-      double acc = 0;
-      for (int64_t u = 0; u < units; ++u) {
-        acc += unit_cost(M, u);
-        const bool cut = acc >= target && (u + 1 - u0) % step == 0;
-        if (cut || u + 1 == units) {
-          items.push_back(BatchItem{(int32_t)i, (int32_t)u0, (int32_t)(u + 1 - u0), 0});
-          u0 = u + 1;
-          acc = 0;
+      for (int64_t u = 0; u < units; ++u) cost[i] += unit_cost(M, u);
+      total += cost[i];
+    }
+    const double target = std::max(16.0, total / (8.0 * std::max(1, waves)));
+    std::vector<size_t> order(mats.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<double> load((size_t)nq, 0.0);
+    std::vector<std::vector<size_t>> queue((size_t)nq);
+    for (size_t i : order) {
+      const size_t q = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[q] += cost[i];
+      queue[q].push_back(i);
+    }
+    qbase.assign((size_t)nq + 1, 0);
+    for (int q = 0; q < nq; ++q) {
+      for (size_t i : queue[(size_t)q]) {
+        const BatchMat& M = mats[i];
+        const int64_t units = M.fmt == 0 ? M.m : (M.m + 63) / 64;
+        const int64_t step = M.fmt == 0 ? 64 / M.gw : 1;  // rows per wavefront step
+        int64_t u0 = 0;
+        double acc = 0;
+        for (int64_t u = 0; u < units; ++u) {
+          acc += unit_cost(M, u);
+          const bool cut = acc >= target && (u + 1 - u0) % step == 0;
+          if (cut || u + 1 == units) {
+            items.push_back(BatchItem{(int32_t)i, (int32_t)u0, (int32_t)(u + 1 - u0), 0});
+            u0 = u + 1;
+            acc = 0;
+          }
         }
       }
+      qbase[(size_t)q + 1] = (int64_t)items.size();
     }
   }
 
   // move everything to the device
   void upload(rvvla_batch_s& B, int dtype) {
-    cut_items(kBatchGrid * (rvvla::kWG / 64));
+    int xcc = 1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess)
+      xcc = 1;
+    const int nq = std::max(1, std::min(xcc, rvvla::kMaxQueues));
+    std::vector<int64_t> qbase;
+    cut_items(kBatchGrid * (rvvla::kWG / 64), nq, qbase);
+    B.args.nqueues = nq;
+    for (int q = 0; q <= nq; ++q) B.args.qbase[q] = qbase[(size_t)q];
     B.mats = mats;
     B.items = items;
     B.args.mats = (const BatchMat*)B.put(mats.data(), mats.size() * sizeof(BatchMat));
     B.args.items = (const BatchItem*)B.put(items.data(), items.size() * sizeof(BatchItem));
     B.args.nitems = (int64_t)items.size();
-    B.args.counter = (int*)B.put(nullptr, 64);
+    B.args.counter = (int*)B.put(nullptr, (size_t)rvvla::kMaxQueues * rvvla::kQueueStride * 4);
     B.args.rst = (const int32_t*)B.put(rst.data(), rst.size() * 4);
     B.args.col = (const int32_t*)B.put(col.data(), col.size() * 4);
     B.args.cl = (const int32_t*)B.put(cl.data(), cl.size() * 4);

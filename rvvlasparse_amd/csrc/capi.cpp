@@ -201,20 +201,30 @@ void check_csr_public(int64_t m, int64_t n, int64_t nnz, const int* rst, const i
   check_csr(m, n, nnz, rst, col);
 }
 
+// XCDs of the current device (8 assumed in host-validation mode).
+static int device_xcds(const rvvla_mat_s& M) {
+  int xcc = 8;
+  if (!M.host_only &&
+      hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, M.device) != hipSuccess)
+    xcc = 1;
+  return std::max(1, std::min(xcc, 8));
+}
+
+// Number of column slices for the sliced CSR image: `request` > 0 is taken
+// as given (1 = no slicing), otherwise (-1) slicing is used when x is
+// larger than one XCD's L2 (4 MiB) and more than a fifth of the entries lie
+// over 65536 columns away from the row's diagonal position: one slice per
+// XCD, or two per XCD when a slice of x would exceed the 4 MiB L2.
 template <class V>
 static int choose_xslices(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request) {
-  if (request > 8) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xslices must be at most 8");
+  if (request > 16) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xslices must be at most 16");
   if (request == 0 || request == 1) return 0;
   if (request > 1) return request;
-  int xcc = 8;
-  if (!M.host_only) {
-    if (hipDeviceGetAttribute(&xcc, hipDeviceAttributeNumberOfXccs, M.device) != hipSuccess)
-      xcc = 1;
-  }
-  xcc = std::min(xcc, 8);
-  const int64_t sx = M.dtype == RVVLA_F64 ? 8 : 4;
-  if (xcc < 2 || (int64_t)a.n * sx <= (int64_t(4) << 20)) return 0;
-  return far_fraction((int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, 65536) > 0.2 ? xcc : 0;
+  const int xcc = device_xcds(M);
+  const int64_t xbytes = (int64_t)a.n * (M.dtype == RVVLA_F64 ? 8 : 4);
+  if (xcc < 2 || xbytes <= (int64_t(4) << 20)) return 0;
+  if (far_fraction((int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, 65536) <= 0.2) return 0;
+  return xbytes / xcc > (int64_t(4) << 20) ? 2 * xcc : xcc;
 }
 
 template <class V>
@@ -231,7 +241,8 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices) {
   const int P = choose_xslices(*M, a, xslices);
   if (P > 0) {
     XSliceHost<V> X;
-    build_xslice<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, P);
+    build_xslice<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, P,
+                    std::min(P, device_xcds(*M)));
     DevXSlice& d = M->csr.xs;
     d.P = P;
     d.nq = X.nq;
@@ -1138,7 +1149,7 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
             : f == "xs.col"  ? V{x.col, xnnz, 4}
             : f == "xs.val"  ? V{x.val, xnnz, vb}
             : f == "xs.blk"  ? V{x.blk, 2 * x.nblk, 4}
-            : f == "xs.mask" ? V{x.mask, x.P ? A->m : 0, 1}
+            : f == "xs.mask" ? V{x.mask, x.P ? A->m : 0, 2}
             : f == "xs.base" ? V{x.base, x.P ? x.P + 1 : 0, 4}
             : f == "xs.pref" ? V{x.pref, x.P * x.nq, 4}
                              : V{nullptr, -1, 0};

@@ -4,23 +4,26 @@
 // round-robin by workgroup index.  A matrix whose columns are spread over
 // an x larger than one L2 (e.g. a power-law graph with uniform columns)
 // turns every x gather into a miss to the memory-side cache.  The sliced
-// image cuts the columns into P contiguous slices (P = number of XCDs) and
-// re-lays the entries slice-major: slice p holds, for each row with entries
-// in columns of slice p, those entries in CSR order.  The multiply then
-// runs slice p only on the workgroups of XCD p (workgroup b takes a block of
-// slice b mod P), so each XCD's gathers stay inside its own 1/P of x, which
-// fits its L2.  Each (row, slice) pair leaves one partial sum; a second
-// kernel adds a row's partials in slice order (deterministic) and applies
-// alpha and beta.
+// image cuts the columns into P contiguous slices (P = the number of XCDs,
+// or twice that) and re-lays the entries slice-major: slice p holds, for
+// each row with entries in columns of slice p, those entries in CSR order.
+// The multiply runs slice p only on the workgroups of XCD p mod X (X = the
+// XCD count; workgroup b takes a block of slice b mod X of its phase), so
+// each XCD's gathers stay inside one 1/P of x, which fits its L2.  With
+// P = 2X the launch has two phases (slices 0..X-1, then X..2X-1) in
+// workgroup order, which the hardware dispatches roughly in order.  Each
+// (row, slice) pair leaves one partial sum; a second kernel adds a row's
+// partials in slice order (deterministic) and applies alpha and beta.
 //
 // Arrays (pairs = number of (row, slice) pairs with at least one entry):
 //   rst[pairs + 1]   entry offsets of the compressed rows, slice-major,
 //                    rows ascending within a slice
 //   col[nnz], val[nnz]
-//   blk[2 * nb * P]  (first, end) compressed-row range of each workgroup;
-//                    entry b belongs to slice b mod P (empty ranges pad);
-//                    within a slice the long-row blocks come first
-//   mask[m]          bit p set when row r has entries in slice p
+//   blk[...]         (first, end) compressed-row range of each workgroup:
+//                    per phase h, nb_h * X entries, entry b of the phase in
+//                    slice h*X + b mod X (empty ranges pad); within a slice
+//                    the long-row blocks come first
+//   mask[m]          (16 bits) bit p set when row r has entries in slice p
 //   base[P + 1]      first pair of each slice
 //   pref[P * nq]     present rows of slice p in rows [0, 64 q), nq = m/64
 //                    rounded up; a wavefront of 64 rows finds the pair of
@@ -85,8 +88,9 @@ double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_
 
 template <class V>
 void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
-                  const V* val, int P) {
-  if (P < 2 || P > 8) throw Error(RVVLA_ERR_INVALID_ARG, "column slices: P in [2, 8]");
+                  const V* val, int P, int xcd) {
+  if (P < 2 || P > 16) throw Error(RVVLA_ERR_INVALID_ARG, "column slices: P in [2, 16]");
+  if (xcd < 1 || xcd > P) xcd = P;
   const int64_t nnz = m ? rst[m] : 0;
   const int64_t w = (n + P - 1) / P;
   X.P = P;
@@ -98,8 +102,8 @@ void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const 
   for (int64_t r = 0; r < m; ++r) {
     int32_t* c = &cnt[(size_t)(r * P)];
     for (int e = rst[r]; e < rst[r + 1]; ++e) c[col[e] / w]++;
-    uint8_t mk = 0;
-    for (int p = 0; p < P; ++p) mk |= (uint8_t)((c[p] > 0) << p);
+    uint16_t mk = 0;
+    for (int p = 0; p < P; ++p) mk |= (uint16_t)((c[p] > 0) << p);
     X.mask[(size_t)r] = mk;
   }
   // pairs per slice and per 64-row group
@@ -136,11 +140,11 @@ void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const 
   X.val.assign((size_t)nnz, V(0));
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t q = 0; q < nq; ++q) {
-    int32_t rank[8];
+    int32_t rank[16];
     for (int p = 0; p < P; ++p) rank[p] = X.base[(size_t)p] + X.pref[(size_t)(p * nq + q)];
     const int64_t hi = std::min(m, (q + 1) * 64);
     for (int64_t r = q * 64; r < hi; ++r) {
-      int32_t fill[8];
+      int32_t fill[16];
       for (int p = 0; p < P; ++p) fill[p] = X.rst[(size_t)rank[p]];
       for (int e = rst[r]; e < rst[r + 1]; ++e) {
         const int p = (int)(col[e] / w);
@@ -150,31 +154,35 @@ void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const 
       for (int p = 0; p < P; ++p) rank[p] += (X.mask[(size_t)r] >> p) & 1;
     }
   }
-  // workgroup blocks: per slice, then interleaved so that block b is in
-  // slice b mod P
+  // workgroup blocks: per slice, then interleaved phase by phase so that
+  // entry b of a phase belongs to slice (phase * xcd + b mod xcd)
   std::vector<std::vector<int32_t>> per((size_t)P);
-  size_t nb = 0;
-  for (int p = 0; p < P; ++p) {
+  for (int p = 0; p < P; ++p)
     per[(size_t)p] = csr_row_blocks(X.rst.data(), X.base[(size_t)p], X.base[(size_t)p + 1]);
-    nb = std::max(nb, per[(size_t)p].size() / 2);
-  }
-  X.blk.assign(2 * nb * (size_t)P, 0);
-  for (size_t i = 0; i < nb; ++i)
-    for (int p = 0; p < P; ++p) {
-      const std::vector<int32_t>& v = per[(size_t)p];
-      const size_t o = 2 * (i * (size_t)P + (size_t)p);
-      if (2 * i + 1 < v.size()) {
-        X.blk[o] = v[2 * i];
-        X.blk[o + 1] = v[2 * i + 1];
-      } else {
-        X.blk[o] = X.blk[o + 1] = X.base[(size_t)p + 1];  // empty
+  X.blk.clear();
+  for (int p0 = 0; p0 < P; p0 += xcd) {
+    const int np = std::min(xcd, P - p0);
+    size_t nb = 0;
+    for (int p = p0; p < p0 + np; ++p) nb = std::max(nb, per[(size_t)p].size() / 2);
+    for (size_t i = 0; i < nb; ++i)
+      for (int j = 0; j < xcd; ++j) {
+        const int p = p0 + j;
+        int32_t lo, hi;
+        if (j < np && 2 * i + 1 < per[(size_t)p].size()) {
+          lo = per[(size_t)p][2 * i];
+          hi = per[(size_t)p][2 * i + 1];
+        } else {
+          lo = hi = X.base[(size_t)std::min(p, P)];  // empty
+        }
+        X.blk.push_back(lo);
+        X.blk.push_back(hi);
       }
-    }
+  }
 }
 
 template void build_xslice<double>(XSliceHost<double>&, int64_t, int64_t, const int*, const int*,
-                                   const double*, int);
+                                   const double*, int, int);
 template void build_xslice<float>(XSliceHost<float>&, int64_t, int64_t, const int*, const int*,
-                                  const float*, int);
+                                  const float*, int, int);
 
 }  // namespace rvvla

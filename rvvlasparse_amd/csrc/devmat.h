@@ -11,8 +11,8 @@
 
 namespace rvvla {
 
-// Column-sliced image of a CSR matrix (see csr_xslice.cpp): P slices, one
-// per XCD; compressed (row, slice) rows, workgroup blocks as (first, end)
+// Column-sliced image of a CSR matrix (see csr_xslice.cpp): P slices (one
+// or two per XCD); compressed (row, slice) rows, workgroup blocks as (first, end)
 // pairs with block b in slice b mod P, presence masks, per-slice bases and
 // per-64-row prefixes, and the partial-sum scratch (one per pair).
 struct DevXSlice {
@@ -22,7 +22,7 @@ struct DevXSlice {
   const int32_t* col = nullptr;
   const void* val = nullptr;
   const int32_t* blk = nullptr;
-  const uint8_t* mask = nullptr;
+  const uint16_t* mask = nullptr;
   const int32_t* base = nullptr;
   const int32_t* pref = nullptr;
   void* part = nullptr;
@@ -165,11 +165,17 @@ struct BatchMat {
 struct BatchItem {
   int32_t mat, first, count, pad;
 };
+// One work queue per XCD: queue q holds items[qbase[q] .. qbase[q + 1]);
+// its counter is counter[q * kQueueStride] (one cache line each).
+constexpr int kMaxQueues = 8;
+constexpr int kQueueStride = 32;
 struct BatchArgs {
   const BatchMat* mats;
   const BatchItem* items;
   int64_t nitems;
   int* counter;
+  int nqueues;
+  int64_t qbase[kMaxQueues + 1];
   const int32_t* rst;
   const int32_t* col;
   const int32_t* cl;

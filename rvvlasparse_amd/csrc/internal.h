@@ -98,11 +98,12 @@ struct XSliceHost {
   int64_t width = 0, nq = 0;
   std::vector<int32_t> rst, col, blk, base, pref;
   std::vector<V> val;
-  std::vector<uint8_t> mask;
+  std::vector<uint16_t> mask;
 };
+// P slices (2..16) on `xcd` XCDs (slices are grouped into phases of xcd).
 template <class V>
 void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
-                  const V* val, int P);
+                  const V* val, int P, int xcd);
 // Fraction of entries whose column is more than `window` away from the
 // row's diagonal position r * n / m.
 double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window);

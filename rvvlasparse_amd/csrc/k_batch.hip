@@ -5,10 +5,13 @@
 //
 // Every matrix is cut into work items of about equal cost (a run of rows
 // for the CSR item kernel, a run of 64-row slices for the SELL item
-// kernel); a persistent grid of wavefronts pops items from a global counter
-// with one atomic per item and runs them until the queue is empty.  A small
-// kernel clears the counter first (a fill kernel, not a memset, so the pair
-// can be captured in a graph and replayed).
+// kernel).  The matrices are packed into one queue per XCD (longest
+// processing time first), so that the matrices in flight on an XCD, and
+// their X, are few and stay in that XCD's L2.  A persistent grid of
+// wavefronts pops items with one atomic per item: workgroup b serves queue
+// b mod nqueues first, then helps with the others until all are empty.  A
+// small kernel clears the counters first (a fill kernel, not a memset, so
+// the pair can be captured in a graph and replayed).
 //  * CSR item: a wavefront works on 64/gw rows at once, gw lanes per row.
 //    The lanes of a row are (slot, rhs group) pairs: lpe lanes cover the k
 //    right-hand sides of one entry, vw consecutive ones each (one 16-byte
@@ -21,8 +24,8 @@
 namespace rvvla {
 
 template <class T>
-__global__ void k_batch_reset(int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+__global__ void k_batch_reset(int* counter, int nqueues) {
+  if (blockIdx.x == 0 && (int)threadIdx.x < nqueues) counter[threadIdx.x * kQueueStride] = 0;
 }
 
 template <class T, int V>
@@ -135,28 +138,42 @@ __device__ void batch_sell_slices(const BatchMat& M, int first, int count,
 }
 
 template <class T>
+__device__ __forceinline__ void run_item(const BatchArgs& a, const BatchItem& it,
+                                         const T* __restrict__ val, const T* __restrict__ sval,
+                                         const T* __restrict__ X, T* __restrict__ Y, T alpha,
+                                         T beta) {
+  const BatchMat M = a.mats[it.mat];
+  if (M.fmt == 0) {
+    if (M.vw == 4)
+      batch_csr_rows<T, 4>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+    else if (M.vw == 2)
+      batch_csr_rows<T, 2>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+    else
+      batch_csr_rows<T, 1>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
+  } else {
+    batch_sell_slices<T>(M, it.first, it.count, a.cl, a.cs, a.scol, sval, a.perm, X, Y, alpha,
+                         beta);
+  }
+}
+
+template <class T>
 __global__ __launch_bounds__(kWG) void k_batch(BatchArgs a, const T* __restrict__ val,
                                                 const T* __restrict__ sval,
                                                 const T* __restrict__ X, T* __restrict__ Y,
                                                 T alpha, T beta) {
   const int lane = threadIdx.x & 63;
-  for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(a.counter, 1);
-    item = __shfl(item, 0, 64);
-    if (item >= a.nitems) return;  // every wavefront leaves once the queue is empty
-    const BatchItem it = a.items[item];
-    const BatchMat M = a.mats[it.mat];
-    if (M.fmt == 0) {
-      if (M.vw == 4)
-        batch_csr_rows<T, 4>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
-      else if (M.vw == 2)
-        batch_csr_rows<T, 2>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
-      else
-        batch_csr_rows<T, 1>(M, it.first, it.count, a.rst, a.col, val, X, Y, alpha, beta);
-    } else {
-      batch_sell_slices<T>(M, it.first, it.count, a.cl, a.cs, a.scol, sval, a.perm, X, Y, alpha,
-                           beta);
+  const int home = (int)(blockIdx.x % (unsigned)a.nqueues);
+  // every wavefront leaves once it has found every queue empty
+  for (int qi = 0; qi < a.nqueues; ++qi) {
+    const int q = (home + qi) % a.nqueues;
+    const int64_t q0 = a.qbase[q], qn = a.qbase[q + 1] - q0;
+    int* cnt = a.counter + q * kQueueStride;
+    for (;;) {
+      int item = 0;
+      if (lane == 0) item = atomicAdd(cnt, 1);
+      item = __shfl(item, 0, 64);
+      if (item >= qn) break;
+      run_item<T>(a, a.items[q0 + item], val, sval, X, Y, alpha, beta);
     }
   }
 }
@@ -164,7 +181,7 @@ __global__ __launch_bounds__(kWG) void k_batch(BatchArgs a, const T* __restrict_
 template <class T>
 static int run_batch(const BatchArgs& a, const void* val, const void* sval, const void* X, void* Y,
                      double alpha, double beta, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch_reset<T>), dim3(1), dim3(64), 0, s, a.counter);
+  hipLaunchKernelGGL((k_batch_reset<T>), dim3(1), dim3(64), 0, s, a.counter, a.nqueues);
   if (int rc = hip_check(hipGetLastError(), "k_batch_reset launch")) return rc;
   if (a.nitems == 0) return RVVLA_OK;
   hipLaunchKernelGGL((k_batch<T>), dim3((unsigned)grid), dim3(kWG), 0, s, a, (const T*)val,

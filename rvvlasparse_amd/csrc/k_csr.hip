@@ -19,8 +19,8 @@
 // so that no long row starts at the end of the launch.
 //
 // Column-sliced image (csr_xslice.cpp): the same kernel runs over the
-// compressed (row, slice) rows, block b of slice b mod P so that each XCD
-// gathers from its own slice of x (entries and columns are then streamed
+// compressed (row, slice) rows, each block on the XCD of its slice so that
+// each XCD gathers from its own slice of x (entries and columns are then streamed
 // with nontemporal loads, to leave L2 to x), and writes one partial per pair
 // (alpha = 1, beta = 0); k_xs_combine then adds each row's partials in
 // slice order and applies alpha and beta.
@@ -129,32 +129,47 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
 }
 
 // y[r] = alpha * (sum over slices p in order of part[pair(r, p)]) + beta*y[r].
-// One wavefront per 64 rows; the pair of row r in slice p is base[p] +
-// pref[p*nq + q] + (lower lanes of the wavefront with bit p set).  All
-// partial loads are issued before the (in-order) sum.
+// A wavefront covers kCombineGroups consecutive groups of 64 rows, one row
+// of each per lane; the pair of row r (group q) in slice p is base[p] +
+// pref[p*nq + q] + (lower lanes with bit p set).  All partial loads of all
+// groups are issued before the (in-order) sums.
+constexpr int kCombineGroups = 2;
 template <class IO, class A>
 __global__ __launch_bounds__(kWG) void k_xs_combine(
-    const uint8_t* __restrict__ mask, const int32_t* __restrict__ base,
+    const uint16_t* __restrict__ mask, const int32_t* __restrict__ base,
     const int32_t* __restrict__ pref, const IO* __restrict__ part, IO* __restrict__ y,
     int64_t m, int64_t nq, int P, A alpha, A beta) {
-  const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  const int q = __builtin_amdgcn_readfirstlane((int)(r >> 6));
+  constexpr int G = kCombineGroups;
   const int lane = threadIdx.x & 63;
+  const int64_t q0 =
+      ((int64_t)blockIdx.x * (kWG / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * G;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const unsigned mk = r < m ? mask[r] : 0u;
-  A v[8];
+  unsigned mk[G];
+  A v[G][16];
 #pragma unroll
-  for (int p = 0; p < 8; ++p) {
-    const bool has = p < P && ((mk >> p) & 1u);
-    const uint64_t bal = __ballot(has);
-    v[p] = A(0);
-    if (has) v[p] = (A)part[base[p] + pref[(int64_t)p * nq + q] + __popcll(bal & below)];
+  for (int g = 0; g < G; ++g) {
+    const int64_t r = (q0 + g) * 64 + lane;
+    mk[g] = r < m ? mask[r] : 0u;
   }
-  A s = A(0);
 #pragma unroll
-  for (int p = 0; p < 8; ++p)
-    if ((mk >> p) & 1u) s += v[p];
-  if (r < m) y[r] = alpha * s + scaled_y(beta, y, r);
+  for (int g = 0; g < G; ++g) {
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+      const bool has = p < P && ((mk[g] >> p) & 1u);
+      const uint64_t bal = __ballot(has);
+      v[g][p] = A(0);
+      if (has) v[g][p] = (A)part[base[p] + pref[(int64_t)p * nq + q0 + g] + __popcll(bal & below)];
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int64_t r = (q0 + g) * 64 + lane;
+    A s = A(0);
+#pragma unroll
+    for (int p = 0; p < 16; ++p)
+      if ((mk[g] >> p) & 1u) s += v[g][p];
+    if (r < m) y[r] = alpha * s + scaled_y(beta, y, r);
+  }
 }
 
 // SpMM: Y = alpha*A*X + beta*Y, X n x k, Y m x k (row-major).  One
@@ -207,7 +222,8 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
                            (IO*)xs.part, (AC)1, (AC)0);
       if (int rc = hip_check(hipGetLastError(), "k_csr_adaptive (sliced) launch")) return rc;
     }
-    const unsigned g = (unsigned)((A.m + kWG - 1) / kWG);
+    const int64_t per_wg = (int64_t)(kWG / 64) * kCombineGroups * 64;
+    const unsigned g = (unsigned)((A.m + per_wg - 1) / per_wg);
     hipLaunchKernelGGL((k_xs_combine<IO, AC>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base, xs.pref,
                        (const IO*)xs.part, (IO*)y, A.m, xs.nq, xs.P, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_xs_combine launch");

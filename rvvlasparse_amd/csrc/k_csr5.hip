@@ -74,20 +74,37 @@ __global__ __launch_bounds__(kWG) void k_csr5_tiles(
   A head = 0, cur = 0;
   uint32_t nfl = 0;
   const int64_t base = t * 64 * (int64_t)sigma + lane;
-  for (int i = 0; i < sigma; ++i) {
-    const int64_t e = base + (int64_t)i * 64;
-    const A p = Val<VT>::get(val, e) * (A)x[a.col[e]];
-    if ((fl >> i) & 1u) {
-      if (nfl == 0) {
-        head = cur;
-      } else {
-        const int64_t r = row_of(yoff + nfl);
-        y[r] = alpha * cur + scaled_y(beta, y, r);
-      }
-      cur = 0;
-      ++nfl;
+  const uint32_t* __restrict__ colp = a.col;
+  // kChunk lane steps at a time: all column and value loads, then all x
+  // gathers, then the flag walk, so that many loads are in flight
+  constexpr int kChunk = 8;
+  for (int i0 = 0; i0 < sigma; i0 += kChunk) {
+    uint32_t c[kChunk];
+    A v[kChunk], p[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int64_t e = base + (int64_t)(i0 + j) * 64;
+      c[j] = i0 + j < sigma ? colp[e] : 0u;
+      v[j] = i0 + j < sigma ? Val<VT>::get(val, e) : A(0);
     }
-    cur += p;
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) p[j] = i0 + j < sigma ? v[j] * (A)x[c[j]] : A(0);
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = i0 + j;
+      if (i >= sigma) break;
+      if ((fl >> i) & 1u) {
+        if (nfl == 0) {
+          head = cur;
+        } else {
+          const int64_t r = row_of(yoff + nfl);
+          y[r] = alpha * cur + scaled_y(beta, y, r);
+        }
+        cur = 0;
+        ++nfl;
+      }
+      cur += p[j];
+    }
   }
   const bool flagged = nfl > 0;
   if (!flagged) head = cur;

@@ -24,25 +24,43 @@ __global__ __launch_bounds__(kWG) void k_cvr(
     typename Val<VT>::io* __restrict__ y, int64_t nstreams, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= nstreams) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (uint64_t(1) << lane) - 1;
   const int64_t k0 = pack_start[t], k1 = pack_start[t + 1];
   int32_t rb = rec_start[t];
   A acc = 0;
-  for (int64_t k = k0; k < k1; ++k) {
-    const uint64_t mk = flush[k];
-    if (mk) {
-      if ((mk >> lane) & 1u) {
-        const int32_t r = rec_rows[rb + __popcll(mk & below)];
-        y[r] = alpha * acc + scaled_y(beta, y, r);
+  // kBatch packs at a time: masks, flushed rows, columns, values and the
+  // gathers are all issued before the packs are walked
+  constexpr int kBatch = 8;
+  for (int64_t kb = k0; kb < k1; kb += kBatch) {
+    uint64_t mk[kBatch];
+    uint32_t c[kBatch];
+    A v[kBatch], p[kBatch];
+    int32_t rr[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const bool in = kb + u < k1;
+      const int64_t e = (kb + u) * 64 + lane;
+      mk[u] = in ? flush[kb + u] : 0ull;
+      c[u] = in ? col[e] : 0u;
+      v[u] = in ? Val<VT>::get(val, e) : A(0);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      p[u] = v[u] * (A)x[c[u]];
+      rr[u] = ((mk[u] >> lane) & 1u) ? rec_rows[rb + __popcll(mk[u] & below)] : 0;
+      rb += __popcll(mk[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if ((mk[u] >> lane) & 1u) {
+        y[rr[u]] = alpha * acc + scaled_y(beta, y, rr[u]);
         acc = 0;
       }
-      rb += __popcll(mk);
+      acc += p[u];  // zero past the stream's last pack
     }
-    const int64_t e = k * 64 + lane;
-    acc += Val<VT>::get(val, e) * (A)x[col[e]];
   }
   const int32_t r = tail_rows[t * 64 + lane];
   if (r >= 0) y[r] = alpha * acc + scaled_y(beta, y, r);

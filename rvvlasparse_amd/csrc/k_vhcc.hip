@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
                                                typename Val<VT>::acc alpha,
                                                typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= a.nparts) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (uint64_t(1) << lane) - 1;
@@ -51,44 +51,72 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
   int64_t rank = a.rbase[t];
   const bool has_fwd = a.fwd_row[t] >= 0;
   const int64_t last_entry = v1 * 64 - 1;
+  const uint32_t* __restrict__ colp = a.col;
+  const uint32_t* __restrict__ rowp = a.row_arr;
+  const uint64_t* __restrict__ eorp = a.eor;
   A hist = 0, carry = 0;
   bool pending = false;  // hist holds sums of vectors without row ends
-  for (int64_t v = v0; v < v1; ++v) {
-    const int64_t e = v * 64 + lane;
-    const A p = Val<VT>::get(val, e) * (A)x[a.col[e]];
-    const uint64_t mk = a.eor[v];
-    if (!mk) {
-      hist += p;
-      pending = true;
-      continue;
+  // kBatch vectors at a time: all their loads (columns, values, masks, the
+  // ending rows) and gathers are issued before the scans
+  constexpr int kBatch = 4;
+  for (int64_t vb = v0; vb < v1; vb += kBatch) {
+    uint32_t c[kBatch];
+    A vv[kBatch], p[kBatch];
+    uint64_t mk[kBatch];
+    uint32_t rr[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const bool in = vb + u < v1;
+      const int64_t e = (vb + u) * 64 + lane;
+      c[u] = in ? colp[e] : 0u;
+      vv[u] = in ? Val<VT>::get(val, e) : A(0);
+      mk[u] = in ? eorp[vb + u] : 0ull;
     }
-    if (pending) {
-      A h = hist;
-      for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
-      carry += h;
-      hist = 0;
-      pending = false;
+    int64_t rk = rank;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      p[u] = vv[u] * (A)x[c[u]];
+      rr[u] = ((mk[u] >> lane) & 1u) ? rowp[rk + __popcll(mk[u] & below)] : 0u;
+      rk += __popcll(mk[u]);
     }
-    A s = p + (lane == 0 ? carry : A(0));
-    const uint64_t heads = 1u | (mk << 1);
-    const int hp = 63 - __builtin_clzll(heads & upto);  // this lane's segment start
-    for (int d = 1; d < 64; d <<= 1) {
-      const A o = __shfl_up(s, d, 64);
-      if (lane - d >= hp) s += o;
-    }
-    if ((mk >> lane) & 1u) {
-      const uint32_t r = a.row_arr[rank + __popcll(mk & below)];
-      if (r >= (uint32_t)a.m) {
-        first_part[t] = s;  // the row this partition continues
-      } else if (has_fwd && e == last_entry) {
-        fwd_part[t] = s;    // the row handed on to the next partition
-      } else {
-        y[r] = alpha * s + scaled_y(beta, y, r);
+    rank = rk;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (vb + u >= v1) break;
+      const uint64_t m = mk[u];
+      if (!m) {
+        hist += p[u];
+        pending = true;
+        continue;
       }
+      if (pending) {
+        A h = hist;
+        for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
+        carry += h;
+        hist = 0;
+        pending = false;
+      }
+      A sum = p[u] + (lane == 0 ? carry : A(0));
+      const uint64_t heads = 1u | (m << 1);
+      const int hp = 63 - __builtin_clzll(heads & upto);  // this lane's segment start
+      for (int d = 1; d < 64; d <<= 1) {
+        const A o = __shfl_up(sum, d, 64);
+        if (lane - d >= hp) sum += o;
+      }
+      if ((m >> lane) & 1u) {
+        const uint32_t r = rr[u];
+        const int64_t e = (vb + u) * 64 + lane;
+        if (r >= (uint32_t)a.m) {
+          first_part[t] = sum;  // the row this partition continues
+        } else if (has_fwd && e == last_entry) {
+          fwd_part[t] = sum;    // the row handed on to the next partition
+        } else {
+          y[r] = alpha * sum + scaled_y(beta, y, r);
+        }
+      }
+      const A last = __shfl(sum, 63, 64);
+      carry = (m >> 63) ? A(0) : last;
     }
-    rank += __popcll(mk);
-    const A last = __shfl(s, 63, 64);
-    carry = (mk >> 63) ? A(0) : last;
   }
 }
 

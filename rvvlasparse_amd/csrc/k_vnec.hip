@@ -16,6 +16,8 @@
 
 namespace rvvla {
 
+static_assert(kVnecItems <= 8 * kWG, "k_vnec stages at most 8 entries per thread");
+
 template <class VT>
 __global__ __launch_bounds__(kWG) void k_vnec(
     const int32_t* __restrict__ cx, const int32_t* __restrict__ cy,
@@ -31,11 +33,42 @@ __global__ __launch_bounds__(kWG) void k_vnec(
   const int t = blockIdx.x, tid = threadIdx.x;
   const int x0 = cx[t], x1 = cx[t + 1], y0 = cy[t], y1 = cy[t + 1];
   const int k0 = nec_ptr[t], nec = nec_ptr[t + 1] - k0;
-  for (int k = tid; k < nec; k += kWG) xs[k] = (A)x[use_x[k0 + k]];
-  __syncthreads();
-  for (int e = y0 + tid; e < y1; e += kWG) prod[e - y0] = Val<VT>::get(val, e) * xs[ecr[e]];
-  __syncthreads();
+  // four loads in flight per thread in each staging loop
+  {
+    int k = tid;
+    for (; k + 3 * kWG < nec; k += 4 * kWG) {
+      const int u0 = use_x[k0 + k], u1 = use_x[k0 + k + kWG];
+      const int u2 = use_x[k0 + k + 2 * kWG], u3 = use_x[k0 + k + 3 * kWG];
+      const A a0 = (A)x[u0], a1 = (A)x[u1], a2 = (A)x[u2], a3 = (A)x[u3];
+      xs[k] = a0;
+      xs[k + kWG] = a1;
+      xs[k + 2 * kWG] = a2;
+      xs[k + 3 * kWG] = a3;
+    }
+    for (; k < nec; k += kWG) xs[k] = (A)x[use_x[k0 + k]];
+  }
   const int nrows = x1 - x0 + (x1 < m ? 1 : 0);
+  // this thread's row bounds (thread-per-row case), fetched before the barriers
+  int lo = 0, hi = 0;
+  if (nrows > kWG / 2 && tid < nrows) {
+    const int r = x0 + tid;
+    lo = max(rst[r], y0) - y0;
+    hi = min(rst[r + 1], y1) - y0;
+  }
+  const int n = y1 - y0;  // <= kVnecItems = 8 * kWG
+  A v[8];
+  uint16_t c[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int e = y0 + tid + j * kWG;
+    c[j] = e < y1 ? ecr[e] : (uint16_t)0;
+    v[j] = e < y1 ? Val<VT>::get(val, e) : A(0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (tid + j * kWG < n) prod[tid + j * kWG] = v[j] * xs[c[j]];
+  __syncthreads();
   auto finish = [&](int r, A s) {
     if (r == x1) {
       fwd_part[t] = s;  // the row still open at the partition's end
@@ -46,11 +79,13 @@ __global__ __launch_bounds__(kWG) void k_vnec(
     }
   };
   if (nrows > kWG / 2) {
+    // a partition may hold up to kVnecItems (empty) rows: stride over them
     for (int i = tid; i < nrows; i += kWG) {
       const int r = x0 + i;
-      const int lo = max(rst[r], y0) - y0, hi = min(rst[r + 1], y1) - y0;
+      const int l = i == tid ? lo : max(rst[r], y0) - y0;
+      const int h = i == tid ? hi : min(rst[r + 1], y1) - y0;
       A s = 0;
-      for (int k = lo; k < hi; ++k) s += prod[k];
+      for (int k = l; k < h; ++k) s += prod[k];
       finish(r, s);
     }
     return;

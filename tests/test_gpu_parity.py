@@ -137,7 +137,7 @@ def test_spmv_16bit_values(fmt, dtype, name):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
-@pytest.mark.parametrize("P", [3, 8])
+@pytest.mark.parametrize("P", [3, 8, 16])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 @pytest.mark.parametrize("name", ["rand_long", "powerlaw", "one_long_row", "empty", "lap64"])
 def test_csr_column_slices(P, dtype, name):

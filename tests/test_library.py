@@ -101,6 +101,28 @@ def test_sell_builder_matches_oracle(sigma, dt):
     assert L.scalar("C") == 64 and L.scalar("Sigma") == sigma
 
 
+def test_sell_default_sigma():
+    """Default sigma: 1 (natural order, identity permutation) when 64-row
+    slices in natural order pad by at most 1 % of the entries, else 16384."""
+    lap = R.gen_laplacian5(64)  # boundary rows are shorter: tiny padding
+    L = R.convert("sell", lap, R.F64)
+    assert L.scalar("Sigma") == 1
+    assert np.array_equal(L.array("vPerm"), np.arange(lap.m, dtype=np.uint32))
+    pl = R.gen_powerlaw(20000, 1.8, 2000, 200000, 256, seed=1)
+    assert R.convert("sell", pl, R.F64).scalar("Sigma") == 16384
+    # padding just over / under 1 %: rows of 100 entries, one row per slice longer
+    m, n = 64 * 50, 4000
+    for extra, want in ((1, 1), (2, 16384)):
+        lens = np.full(m, 100)
+        lens[::64] += extra  # pads 63 * extra slots per slice
+        rst = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+        col = np.concatenate([np.arange(l) for l in lens]).astype(np.int32)
+        a = R.CSR(m, n, rst, col, np.ones(rst[-1]))
+        pad = 50 * 63 * extra
+        assert (pad <= 0.01 * a.nnz) == (want == 1)
+        assert R.convert("sell", a, R.F64).scalar("Sigma") == want
+
+
 def test_generators():
     lap = R.gen_laplacian5(9)
     rst, col, val = laplacian(9)

@@ -90,17 +90,24 @@ def _check_image(a: R.CSR, M: R.DeviceMatrix, P: int):
             sel = np.nonzero(sl[e0:e1] == p)[0] + e0
             assert np.array_equal(xcol[xrst[k]:xrst[k + 1]], a.col[sel]), (r, p)
             assert np.array_equal(xval[xrst[k]:xrst[k + 1]], vals[sel]), (r, p)
-    # blocks: entry b belongs to slice b mod P; a slice's non-empty blocks
-    # tile its pairs and keep the adaptive limits
+    # blocks: with X = min(P, 8) XCDs, entry b of the launch runs on XCD
+    # b mod X and must hold a slice p with p mod X == b mod X; slices run in
+    # phases of X (all of phase 0 first); each slice's non-empty blocks tile
+    # its pairs and keep the adaptive limits
     plen = np.diff(xrst)
+    X = min(P, 8)
+    assert len(blk) % X == 0
+    ne = np.nonzero(blk[:, 1] > blk[:, 0])[0]
+    sl_of = np.searchsorted(base, blk[ne, 0], side="right") - 1
+    assert np.all(blk[ne, 1] <= base[sl_of + 1])
+    assert np.all(sl_of % X == ne % X)
+    assert np.all(np.diff(sl_of // X) >= 0)
     for p in range(P):
-        b = blk[p::P]
-        assert np.all(b[:, 0] >= base[p]) and np.all(b[:, 1] <= base[p + 1])
-        _check_blocks(b[b[:, 1] > b[:, 0]], plen, base[p], base[p + 1])
+        _check_blocks(blk[ne[sl_of == p]], plen, base[p], base[p + 1])
     return base, pref, xrst, xcol, xval
 
 
-@pytest.mark.parametrize("P", [2, 3, 8])
+@pytest.mark.parametrize("P", [2, 3, 8, 12, 16])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_sliced_image(P, dtype):
     a = _matrix()
@@ -140,6 +147,9 @@ def test_auto_choice():
     n = 600_000
     g = R.gen_random_rows(n, 2, seed=1)  # diagonal + one uniform column
     assert len(R.convert("csr", g, R.F64).upload().view("xs.base")) == 9
+    big = R.gen_random_rows(4_400_000, 2, seed=1)  # FP64 x over 8 x 4 MiB: two per XCD
+    assert len(R.convert("csr", big, R.F64).upload().view("xs.base")) == 17
+    assert len(R.convert("csr", big, R.F32).upload().view("xs.base")) == 9
     assert len(R.convert("csr", g, R.F64, "xslices=1").upload().view("xs.base")) == 0
     lap = R.gen_laplacian5(800)  # n = 640000, banded
     assert len(R.convert("csr", lap, R.F64).upload().view("xs.base")) == 0
@@ -153,5 +163,5 @@ def test_empty_and_bad_request():
     assert np.array_equal(M.view("xs.base"), np.zeros(5, np.uint32))
     assert M.view("xs.blk").size % 8 == 0
     with pytest.raises(R.RvvlaError) as ex:
-        R.convert("csr", _matrix(), R.F64, "xslices=9").upload()
+        R.convert("csr", _matrix(), R.F64, "xslices=17").upload()
     assert ex.value.code == -10
