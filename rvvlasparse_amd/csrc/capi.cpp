@@ -652,6 +652,41 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
                                         S > 0 ? (size_t)(cc * L) * 4 : 0);
   d.col = M->put_vec(a.col_id);
   d.val = put_values(*M, dtype, vals.data(), vals.size());
+  // long chunks -> pieces of at most kLavPiece steps
+  std::vector<int32_t> whole, pc_chunk, sp_chunk, sp_first{0};
+  std::vector<int64_t> pc_k0, pc_k1;
+  for (int64_t c = 0; c < cc; ++c) {
+    int64_t steps = 0;
+    for (int64_t g = 0; g < S; ++g)
+      steps += a.chunk_offsets[(size_t)(g * (cc + 1) + c + 1)] - a.chunk_offsets[(size_t)(g * (cc + 1) + c)];
+    if (steps <= kLavPiece) {
+      whole.push_back((int32_t)c);
+      continue;
+    }
+    for (int64_t g = 0; g < S; ++g) {
+      const int64_t k0 = sptr[(size_t)g] + a.chunk_offsets[(size_t)(g * (cc + 1) + c)];
+      const int64_t k1 = sptr[(size_t)g] + a.chunk_offsets[(size_t)(g * (cc + 1) + c + 1)];
+      for (int64_t k = k0; k < k1; k += kLavPiece) {
+        pc_chunk.push_back((int32_t)c);
+        pc_k0.push_back(k);
+        pc_k1.push_back(std::min(k1, k + kLavPiece));
+      }
+    }
+    sp_chunk.push_back((int32_t)c);
+    sp_first.push_back((int32_t)pc_chunk.size());
+  }
+  d.nwhole = (int64_t)whole.size();
+  if (!sp_chunk.empty()) {
+    d.whole = M->put_vec(whole);
+    d.npieces = (int64_t)pc_chunk.size();
+    d.nsplit = (int64_t)sp_chunk.size();
+    d.pc_chunk = M->put_vec(pc_chunk);
+    d.pc_k0 = M->put_vec(pc_k0);
+    d.pc_k1 = M->put_vec(pc_k1);
+    d.sp_chunk = M->put_vec(sp_chunk);
+    d.sp_first = M->put_vec(sp_first);
+    d.part = M->put(nullptr, (size_t)d.npieces * 64 * (dtype == RVVLA_F64 ? 8 : 4));
+  }
   return M.release();
 }
 

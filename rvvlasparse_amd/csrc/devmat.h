@@ -116,6 +116,11 @@ struct DevVHCC {
 
 // LAV (SIMD_Lanes = 64): the dense part's chunk tables (the sparse part
 // lives in rvvla_mat_s::csr).
+// Chunks with more than kLavPiece steps over all segments are cut into
+// pieces of at most kLavPiece steps (within one segment), each run by its
+// own wavefront into a 64-lane partial; the chunk's partials are then added
+// in piece order.
+constexpr int kLavPiece = 64;
 struct DevLAV {
   int64_t nseg = 0, nchunks = 0;
   const int32_t* coff = nullptr;       // nseg * (nchunks + 1) step offsets
@@ -123,6 +128,15 @@ struct DevLAV {
   const uint32_t* out_order = nullptr; // nchunks * 64 rows (shared by all segments)
   const uint32_t* col = nullptr;
   const void* val = nullptr;
+  int64_t nwhole = 0;                  // chunks run whole
+  const int32_t* whole = nullptr;      // their ids (nullptr: all chunks, in order)
+  int64_t npieces = 0, nsplit = 0;
+  const int32_t* pc_chunk = nullptr;   // chunk of each piece
+  const int64_t* pc_k0 = nullptr;      // global step range of each piece
+  const int64_t* pc_k1 = nullptr;
+  const int32_t* sp_chunk = nullptr;   // the split chunks
+  const int32_t* sp_first = nullptr;   // nsplit + 1: their pieces
+  void* part = nullptr;                // npieces * 64 partial sums
 };
 
 // VNEC: merge-path partitions of at most kVnecItems rows + entries, each

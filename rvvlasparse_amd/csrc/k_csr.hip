@@ -233,6 +233,10 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
     hipLaunchKernelGGL((k_csr_adaptive<VT, true, false>), dim3((unsigned)d.nblocks), dim3(kWG), 0,
                        s, d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y,
                        (AC)alpha, (AC)beta);
+  else if (xs_nt && sizeof(VT) == 8)  // FP64 entries streamed past L2
+    hipLaunchKernelGGL((k_csr_adaptive<VT, false, true>), dim3((unsigned)d.nblocks), dim3(kWG),
+                       0, s, d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y,
+                       (AC)alpha, (AC)beta);
   else
     hipLaunchKernelGGL((k_csr_adaptive<VT, false, false>), dim3((unsigned)d.nblocks), dim3(kWG),
                        0, s, d.rst, d.col, (const VT*)d.val, d.blocks, (const IO*)x, (IO*)y,

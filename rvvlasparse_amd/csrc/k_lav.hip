@@ -8,7 +8,10 @@
 // once for all segments, a wavefront walks chunk c through every segment,
 // keeps the lane sums in registers and adds alpha * sum into y once.  Slots
 // past a step's mask were zeroed at upload, so all 64 lanes load every step
-// (one contiguous access per step) without per-lane predicates.
+// (one contiguous access per step) without per-lane predicates.  Chunks
+// longer than kLavPiece steps are cut into pieces (k_lav_pieces) whose
+// partials k_lav_combine adds in order, so no wavefront walks a chunk of
+// thousands of steps alone.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -16,25 +19,31 @@ namespace rvvla {
 template <class VT>
 __global__ __launch_bounds__(kWG) void k_lav_dense(
     const int32_t* __restrict__ coff, const int64_t* __restrict__ sptr, int64_t nseg,
-    int64_t nchunks, const uint32_t* __restrict__ out_order, const uint32_t* __restrict__ col,
+    int64_t nchunks, const int32_t* __restrict__ whole, int64_t nwhole,
+    const uint32_t* __restrict__ out_order, const uint32_t* __restrict__ col,
     const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha) {
   using A = typename Val<VT>::acc;
-  const int64_t c = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-  if (c >= nchunks) return;
+  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (w >= nwhole) return;
+  const int64_t c = whole ? (int64_t)whole[w] : w;
   const int lane = threadIdx.x & 63;
-  A a0 = 0, a1 = 0;
+  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int64_t g = 0; g < nseg; ++g) {
     const int64_t base = sptr[g];
     int64_t k = base + coff[g * (nchunks + 1) + c];
     const int64_t k1 = base + coff[g * (nchunks + 1) + c + 1];
-    for (; k + 1 < k1; k += 2) {
-      const int64_t e0 = k * 64 + lane, e1 = e0 + 64;
-      const uint32_t c0 = col[e0], c1 = col[e1];
-      a0 += Val<VT>::get(val, e0) * (A)x[c0];
-      a1 += Val<VT>::get(val, e1) * (A)x[c1];
+    for (; k + 3 < k1; k += 4) {
+      const int64_t e0 = k * 64 + lane;
+      const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
+      const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
+      const A v2 = Val<VT>::get(val, e0 + 128), v3 = Val<VT>::get(val, e0 + 192);
+      a0 += v0 * (A)x[c0];
+      a1 += v1 * (A)x[c1];
+      a2 += v2 * (A)x[c2];
+      a3 += v3 * (A)x[c3];
     }
-    if (k < k1) {
+    for (; k < k1; ++k) {
       const int64_t e0 = k * 64 + lane;
       a0 += Val<VT>::get(val, e0) * (A)x[col[e0]];
     }
@@ -42,7 +51,58 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
   const int64_t rp = c * 64 + lane;
   if (rp < m) {
     const uint32_t r = out_order[rp];
-    y[r] = (A)y[r] + alpha * (a0 + a1);
+    y[r] = (A)y[r] + alpha * ((a0 + a1) + (a2 + a3));
+  }
+}
+
+// One piece (at most kLavPiece steps of one long chunk in one segment) per
+// wavefront: its 64 lane sums go to part[piece * 64 + lane].
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_lav_pieces(
+    const int64_t* __restrict__ pc_k0, const int64_t* __restrict__ pc_k1, int64_t npieces,
+    const uint32_t* __restrict__ col, const VT* __restrict__ val,
+    const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc* __restrict__ part) {
+  using A = typename Val<VT>::acc;
+  const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (q >= npieces) return;
+  const int lane = threadIdx.x & 63;
+  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int64_t k = pc_k0[q];
+  const int64_t k1 = pc_k1[q];
+  for (; k + 3 < k1; k += 4) {
+    const int64_t e0 = k * 64 + lane;
+    const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
+    const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
+    const A v2 = Val<VT>::get(val, e0 + 128), v3 = Val<VT>::get(val, e0 + 192);
+    a0 += v0 * (A)x[c0];
+    a1 += v1 * (A)x[c1];
+    a2 += v2 * (A)x[c2];
+    a3 += v3 * (A)x[c3];
+  }
+  for (; k < k1; ++k) {
+    const int64_t e0 = k * 64 + lane;
+    a0 += Val<VT>::get(val, e0) * (A)x[col[e0]];
+  }
+  part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
+}
+
+// One split chunk per wavefront: its pieces' partials in piece order, then
+// y += alpha * sum for the chunk's rows.
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_lav_combine(
+    const int32_t* __restrict__ sp_chunk, const int32_t* __restrict__ sp_first, int64_t nsplit,
+    const typename Val<VT>::acc* __restrict__ part, const uint32_t* __restrict__ out_order,
+    typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha) {
+  using A = typename Val<VT>::acc;
+  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (w >= nsplit) return;
+  const int lane = threadIdx.x & 63;
+  A s = 0;
+  for (int32_t q = sp_first[w]; q < sp_first[w + 1]; ++q) s += part[(int64_t)q * 64 + lane];
+  const int64_t rp = (int64_t)sp_chunk[w] * 64 + lane;
+  if (rp < m) {
+    const uint32_t r = out_order[rp];
+    y[r] = (A)y[r] + alpha * s;
   }
 }
 
@@ -54,11 +114,24 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
   if (int rc = launch_csr(A, alpha, x, beta, y, s)) return rc;  // sparse_part, with beta
   const DevLAV& d = A.lav;
   if (d.nseg == 0 || d.nchunks == 0) return RVVLA_OK;
-  const unsigned g = (unsigned)((d.nchunks + kWG / 64 - 1) / (kWG / 64));
-  hipLaunchKernelGGL((k_lav_dense<VT>), dim3(g), dim3(kWG), 0, s, d.coff, d.sptr, d.nseg,
-                     d.nchunks, d.out_order, d.col, (const VT*)d.val, (const IO*)x, (IO*)y, A.m,
-                     (AC)alpha);
-  return hip_check(hipGetLastError(), "k_lav_dense launch");
+  constexpr int W = kWG / 64;
+  if (d.nwhole > 0) {
+    hipLaunchKernelGGL((k_lav_dense<VT>), dim3((unsigned)((d.nwhole + W - 1) / W)), dim3(kWG), 0,
+                       s, d.coff, d.sptr, d.nseg, d.nchunks, d.whole, d.nwhole, d.out_order, d.col,
+                       (const VT*)d.val, (const IO*)x, (IO*)y, A.m, (AC)alpha);
+    if (int rc = hip_check(hipGetLastError(), "k_lav_dense launch")) return rc;
+  }
+  if (d.nsplit > 0) {
+    hipLaunchKernelGGL((k_lav_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
+                       0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
+                       (AC*)d.part);
+    if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
+    hipLaunchKernelGGL((k_lav_combine<VT>), dim3((unsigned)((d.nsplit + W - 1) / W)), dim3(kWG),
+                       0, s, d.sp_chunk, d.sp_first, d.nsplit, (const AC*)d.part, d.out_order,
+                       (IO*)y, A.m, (AC)alpha);
+    if (int rc = hip_check(hipGetLastError(), "k_lav_combine launch")) return rc;
+  }
+  return RVVLA_OK;
 }
 
 int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,

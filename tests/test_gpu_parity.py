@@ -204,6 +204,47 @@ def test_direct_uploads_match_converted(dtype):
                             0.25, y064.astype(dt), dt, "direct upload")
 
 
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_hcsr_upload_unsorted_coo_blocks(dtype):
+    """A directly uploaded HCSR whose COO-kind blocks list their entries out
+    of row order (allowed by the format) gives the same product: the upload
+    re-sorts those blocks for the band kernel."""
+    import ctypes as C
+    a = case("rand_long")
+    x64, y064 = inputs(a, 8)
+    L = R.convert("hcsr", a, dtype, "R=300,C=400,p=2")
+    vt = np.float64 if dtype == R.F64 else np.float32
+    bp, ro = L.array("block_ptr").astype(np.int64), L.array("row_offset")
+    rp, ci, va = L.array("row_ptr").copy(), L.array("col_idx").copy(), L.array("values", vt).copy()
+    rng = np.random.default_rng(3)
+    shuffled = 0
+    for b in range(len(bp) - 1):
+        if (int(ro[b]) >> 30) != 2 or bp[b + 1] - bp[b] < 2:
+            continue
+        e0, e1, off = bp[b], bp[b + 1], int(ro[b]) & 0x3FFFFFFF
+        p = rng.permutation(e1 - e0)
+        rp[off:off + e1 - e0] = rp[off:off + e1 - e0][p]
+        ci[e0:e1], va[e0:e1] = ci[e0:e1][p], va[e0:e1][p]
+        shuffled += 1
+    assert shuffled > 0
+    h = C.c_void_p()
+    R.check(R.lib.rvvla_hcsr_upload(dtype, a.m, a.n, a.nnz, 300, 400, int(L.scalar("b_m")),
+                                    int(L.scalar("b_n")), bp.astype(np.int32).ctypes.data,
+                                    ro.ctypes.data, rp.shape[0], rp.ctypes.data, ci.ctypes.data,
+                                    va.ctypes.data, C.byref(h)))
+    M = R.DeviceMatrix(h)
+    dev = _device()
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    M.spmv(x, y, 1.0, -0.5)
+    dt = vt
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.0, -0.5, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.0, -0.5,
+                        y064.astype(dt), dt, "hcsr unsorted COO")
+
+
 def test_spmv_host_path():
     """rvvla_spmv_host: the std::vector ALL_STAGES form."""
     _device()
