@@ -254,6 +254,28 @@ int rvvla_gen_uniform(int64_t count, double lo, double hi, uint64_t seed,
                       uint64_t purpose, double* v);
 int rvvla_gen_free(rvvla_gen_csr* g);
 
+/* ---- matrix files (host) ---------------------------------------------
+ * The reference keeps a Matrix Market type code in spMtxCRS
+ * (reference include/CRS.h:8,17) and names a mtx-to-bin converter in its
+ * build options (reference cmake/TestEnvironment.cmake:62) without shipping
+ * either; these are them.  Results are canonical CSR (columns ascending in
+ * each row, duplicate entries summed) in arrays released with
+ * rvvla_gen_free; matcode (may be NULL) gets the four MM_typecode letters.
+ *
+ * rvvla_mtx_read: Matrix Market coordinate files, field real / integer /
+ * pattern (value 1), symmetry general / symmetric / skew-symmetric (the
+ * mirror of each off-diagonal entry is added).  Errors: RVVLA_ERR_NOT_FOUND
+ * (cannot open), RVVLA_ERR_INVALID_ARG (malformed), RVVLA_ERR_UNSUPPORTED_SPMV
+ * (array format, complex, hermitian), RVVLA_ERR_LENGTH (beyond int32).
+ *
+ * rvvla_bin_write / rvvla_bin_read: binary CSR, little-endian:
+ * "RVVLACSR", int32 version 1, char[4] matcode, int64 m, n, nnz, then
+ * int32 Rst[m+1], int32 Col[nnz], float64 Val[nnz].  A file that does not
+ * hold a valid CSR is refused with RVVLA_ERR_INVALID_ARG. */
+int rvvla_mtx_read(const char* path, rvvla_gen_csr* out, char matcode[4]);
+int rvvla_bin_write(const char* path, const rvvla_gen_csr* a, const char matcode[4]);
+int rvvla_bin_read(const char* path, rvvla_gen_csr* out, char matcode[4]);
+
 #ifdef __cplusplus
 }
 #endif

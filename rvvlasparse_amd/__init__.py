@@ -82,6 +82,9 @@ _sigs = {
     "rvvla_gen_random_rows_var": (_i32, [_i64, _i32, _i32, C.c_uint64, C.POINTER(GenCSR)]),
     "rvvla_gen_uniform": (_i32, [_i64, _d, _d, C.c_uint64, C.c_uint64, _vp]),
     "rvvla_gen_free": (_i32, [C.POINTER(GenCSR)]),
+    "rvvla_mtx_read": (_i32, [_cp, C.POINTER(GenCSR), C.c_char_p]),
+    "rvvla_bin_write": (_i32, [_cp, C.POINTER(GenCSR), C.c_char_p]),
+    "rvvla_bin_read": (_i32, [_cp, C.POINTER(GenCSR), C.c_char_p]),
 }
 for _name, (_res, _args) in _sigs.items():
     _f = getattr(lib, _name)
@@ -334,6 +337,31 @@ class Batch:
         if getattr(self, "_h", None):
             lib.rvvla_batch_free(self._h)
             self._h = None
+
+
+def read_mtx(path: str):
+    """Matrix Market coordinate file -> (CSR with float64 values, MM type
+    code such as "MCRG")."""
+    g, code = GenCSR(), C.create_string_buffer(4)
+    check(lib.rvvla_mtx_read(str(path).encode(), C.byref(g), code))
+    return _take(g), code.raw.decode()
+
+
+def read_bin(path: str):
+    """Binary CSR file -> (CSR, MM type code)."""
+    g, code = GenCSR(), C.create_string_buffer(4)
+    check(lib.rvvla_bin_read(str(path).encode(), C.byref(g), code))
+    return _take(g), code.raw.decode()
+
+
+def write_bin(path: str, a: "CSR", matcode: str = "MCRG") -> None:
+    val = np.ascontiguousarray(a.val, np.float64)
+    rst = np.ascontiguousarray(a.rst, np.int32)
+    col = np.ascontiguousarray(a.col, np.int32)
+    g = GenCSR(a.m, a.n, a.nnz, rst.ctypes.data_as(C.POINTER(C.c_int32)),
+               col.ctypes.data_as(C.POINTER(C.c_int32)),
+               val.ctypes.data_as(C.POINTER(C.c_double)))
+    check(lib.rvvla_bin_write(str(path).encode(), C.byref(g), matcode.encode()[:4].ljust(4)))
 
 
 def gen_random_rows_var(n: int, kmin: int, kmax: int, seed: int = 0) -> CSR:

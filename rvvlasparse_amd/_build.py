@@ -66,11 +66,11 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     _run(["g++", "-shared", "-fopenmp", "-o", str(tmp), *map(str, objs),
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
-    # the C++ drop-in example/test program, linked against the library
-    demo = ROOT / "tests" / "cxx" / "api_drop_in.cpp"
-    if demo.exists():
+    # the C++ test programs (the drop-in example, the file-I/O check),
+    # linked against the library
+    for demo in sorted((ROOT / "tests" / "cxx").glob("*.cpp")):
         _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(demo), f"-L{PKG}", "-lrvvla_hip",
-              "-Wl,-rpath,$ORIGIN/../rvvlasparse_amd", "-o", str(BUILD / "api_drop_in")])
+              "-Wl,-rpath,$ORIGIN/../rvvlasparse_amd", "-o", str(BUILD / demo.stem)])
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", flush=True)
     return LIB
