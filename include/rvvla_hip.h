@@ -147,10 +147,19 @@ int rvvla_mat_info(rvvla_mat_t A, int64_t* m, int64_t* n, int64_t* nnz,
  * compute type (double for F64, float for F32/BF16/F16).  With beta == 0
  * y is write-only (BLAS convention); the reference's scalar CSR reads y
  * there too (spmv_mtx_crs.cpp:37-43), so the two differ only when y already
- * holds NaN or Inf.  Only SPARSE_OPERATION_NON_TRANSPOSE exists, as in the
- * reference. */
+ * holds NaN or Inf. */
 int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta,
                void* y, void* stream);
+/* The product with the operation of sparse_mv's type_op
+ * (reference include/types.h:24-27; the reference accepts the flag but
+ * always computes A x): op 0 = SPARSE_OPERATION_NON_TRANSPOSE (rvvla_spmv),
+ * op 1 = SPARSE_OPERATION_TRANSPOSE, y = alpha * A^T * x + beta * y with x of
+ * m entries and y of n.  The transpose is implemented for CSR matrices (every
+ * dtype): the first transposed product builds a CSR image of A^T on the
+ * device (counting sort by column, about as much memory again as A); other
+ * formats return RVVLA_ERR_UNSUPPORTED_SPMV. */
+int rvvla_spmv_op(rvvla_mat_t A, int op, double alpha, const void* x, double beta,
+                  void* y, void* stream);
 /* Y = alpha * A * X + beta * Y for k right-hand sides; X is n x k and Y is
  * m x k, row-major with leading dimensions ldx >= k and ldy >= k
  * (CSR, F32/F64).  No reference counterpart: the reference takes one vector. */
@@ -218,6 +227,13 @@ int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host,
 int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host);
 int rvvla_mult_staged(rvvla_mat_t A, double alpha, double beta);
 int rvvla_stage_out(rvvla_mat_t A, void* y_host);
+/* The same with an operation (op as in rvvla_spmv_op; for op 1, b has m
+ * entries and y has n). */
+int rvvla_spmv_host_op(rvvla_mat_t A, int op, double alpha, const void* b_host,
+                       double beta, void* y_host);
+int rvvla_stage_in_op(rvvla_mat_t A, int op, const void* b_host, const void* y_host);
+int rvvla_mult_staged_op(rvvla_mat_t A, int op, double alpha, double beta);
+int rvvla_stage_out_op(rvvla_mat_t A, int op, void* y_host);
 
 /* ---- synthetic matrices (host, deterministic, OpenMP) ----------------
  * Generators for the benchmark matrices.  Every array is allocated by the

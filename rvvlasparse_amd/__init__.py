@@ -74,6 +74,8 @@ _sigs = {
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
     "rvvla_spmv": (_i32, [_vp, _d, _vp, _d, _vp, _vp]),
+    "rvvla_spmv_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp, _vp]),
+    "rvvla_spmv_host_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp]),
     "rvvla_spmm": (_i32, [_vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp]),
     "rvvla_spmv_host": (_i32, [_vp, _d, _vp, _d, _vp]),
     "rvvla_gen_laplacian5": (_i32, [_i64, C.POINTER(GenCSR)]),
@@ -232,6 +234,10 @@ class DeviceMatrix:
         (a torch.cuda.Stream, a raw hipStream_t int, or None = torch's
         current stream when torch tensors are passed)."""
         check(lib.rvvla_spmv(self._h, alpha, _ptr(x), beta, _ptr(y), _stream(stream, y)))
+
+    def spmv_t(self, x, y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        """y = alpha*A^T*x + beta*y (SPARSE_OPERATION_TRANSPOSE; CSR matrices)."""
+        check(lib.rvvla_spmv_op(self._h, 1, alpha, _ptr(x), beta, _ptr(y), _stream(stream, y)))
 
     def spmm(self, X, Y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         k = int(X.shape[1])

@@ -1241,6 +1241,57 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
   return RVVLA_ERR_UNKNOWN_FORMAT;
 }
 
+// CSR image of the transpose: counting sort of the entries by column (a
+// column's entries stay in row order), values moved with them as raw
+// storage elements, adaptive blocks over the n rows of the transpose.
+static int build_transposed_image(rvvla_mat_t A) {
+  if (A->has_tcsr) return RVVLA_OK;
+  return guarded([&]() {
+    const int64_t m = A->m, n = A->n, nnz = A->nnz;
+    const size_t es = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
+    std::vector<int32_t> rst((size_t)m + 1), col((size_t)nnz);
+    std::vector<unsigned char> val((size_t)nnz * es);
+    int rc = rvvla::hip_check(hipMemcpy(rst.data(), A->csr.rst, rst.size() * 4, hipMemcpyDeviceToHost), "copy Rst");
+    if (!rc && nnz) rc = rvvla::hip_check(hipMemcpy(col.data(), A->csr.col, col.size() * 4, hipMemcpyDeviceToHost), "copy Col");
+    if (!rc && nnz) rc = rvvla::hip_check(hipMemcpy(val.data(), A->csr.val, val.size(), hipMemcpyDeviceToHost), "copy Val");
+    if (rc) throw Error(rc, rvvla::last_error());
+    std::vector<int32_t> ptr((size_t)n + 1, 0), row((size_t)nnz);
+    std::vector<unsigned char> tval((size_t)nnz * es);
+    for (int64_t e = 0; e < nnz; ++e) ptr[(size_t)col[(size_t)e] + 1]++;
+    for (int64_t c = 0; c < n; ++c) ptr[(size_t)c + 1] += ptr[(size_t)c];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t r = 0; r < m; ++r)
+      for (int32_t e = rst[(size_t)r]; e < rst[(size_t)r + 1]; ++e) {
+        const int32_t p = fill[(size_t)col[(size_t)e]]++;
+        row[(size_t)p] = (int32_t)r;
+        std::memcpy(&tval[(size_t)p * es], &val[(size_t)e * es], es);
+      }
+    const std::vector<int32_t> blocks = rvvla::csr_row_blocks(ptr.data(), 0, n);
+    rvvla::DevCSR& t = A->tcsr;
+    t.rst = A->put_vec(ptr);
+    t.col = A->put_vec(row);
+    t.val = A->put(tval.data(), tval.size());
+    t.blocks = A->put_vec(blocks);
+    t.nblocks = (int64_t)blocks.size() / 2;
+    A->has_tcsr = true;
+    return RVVLA_OK;
+  });
+}
+
+int rvvla_spmv_op(rvvla_mat_t A, int op, double alpha, const void* x, double beta, void* y,
+                  void* stream) {
+  if (op == 0) return rvvla_spmv(A, alpha, x, beta, y, stream);
+  if (op != 1 || !A) return RVVLA_ERR_INVALID_ARG;
+  if (int rc = need_device(A)) return rc;
+  if (A->format != RVVLA_FMT_CSR) {
+    rvvla::set_last_error("the transposed product is implemented for the CSR format");
+    return RVVLA_ERR_UNSUPPORTED_SPMV;
+  }
+  if ((A->m > 0 && !x) || (A->n > 0 && !y)) return RVVLA_ERR_INVALID_ARG;
+  if (int rc = build_transposed_image(A)) return rc;
+  return rvvla::launch_csr_image(A->tcsr, A->n, A->dtype, alpha, x, beta, y, (hipStream_t)stream);
+}
+
 int rvvla_spmm(rvvla_mat_t A, int k, double alpha, const void* X, int64_t ldx, double beta,
                void* Y, int64_t ldy, void* stream) {
   if (int rc = need_device(A)) return rc;
@@ -1321,47 +1372,65 @@ int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy
   return rvvla::launch_sddmm(*A, k, DY, lddy, X, ldx, alpha, out_vals, (hipStream_t)stream);
 }
 
-int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host) {
+// Staging buffers hold max(m, n) elements each, so that either operation
+// can use them: op 0 reads n entries of b and writes m of y, op 1 the
+// reverse.
+int rvvla_stage_in_op(rvvla_mat_t A, int op, const void* b_host, const void* y_host) {
   if (int rc = need_device(A)) return rc;
-  if (!A || (A->n > 0 && !b_host)) return RVVLA_ERR_INVALID_ARG;
+  if (op != 0 && op != 1) return RVVLA_ERR_INVALID_ARG;
+  const int64_t nb = op ? A->m : A->n, ny = op ? A->n : A->m;
+  if (nb > 0 && !b_host) return RVVLA_ERR_INVALID_ARG;
   const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;
+  const size_t cap = (size_t)std::max(A->m, A->n) * es + 8;
   int rc;
-  if (!A->xstage && (rc = rvvla::hip_check(hipMalloc(&A->xstage, (size_t)A->n * es + 8), "hipMalloc x")))
-    return rc;
-  if (!A->ystage && (rc = rvvla::hip_check(hipMalloc(&A->ystage, (size_t)A->m * es + 8), "hipMalloc y")))
-    return rc;
-  if ((rc = rvvla::hip_check(hipMemcpy(A->xstage, b_host, (size_t)A->n * es, hipMemcpyHostToDevice), "copy b")))
+  if (!A->xstage && (rc = rvvla::hip_check(hipMalloc(&A->xstage, cap), "hipMalloc x"))) return rc;
+  if (!A->ystage && (rc = rvvla::hip_check(hipMalloc(&A->ystage, cap), "hipMalloc y"))) return rc;
+  if ((rc = rvvla::hip_check(hipMemcpy(A->xstage, b_host, (size_t)nb * es, hipMemcpyHostToDevice), "copy b")))
     return rc;
   if (y_host)
-    return rvvla::hip_check(hipMemcpy(A->ystage, y_host, (size_t)A->m * es, hipMemcpyHostToDevice), "copy y");
-  return rvvla::hip_check(hipMemset(A->ystage, 0, (size_t)A->m * es), "clear y");
+    return rvvla::hip_check(hipMemcpy(A->ystage, y_host, (size_t)ny * es, hipMemcpyHostToDevice), "copy y");
+  return rvvla::hip_check(hipMemset(A->ystage, 0, (size_t)ny * es), "clear y");
 }
 
-int rvvla_mult_staged(rvvla_mat_t A, double alpha, double beta) {
+int rvvla_mult_staged_op(rvvla_mat_t A, int op, double alpha, double beta) {
   if (!A) return RVVLA_ERR_INVALID_ARG;
   if (!A->xstage || !A->ystage) {
     rvvla::set_last_error("MULT before PREPARATION");
     return RVVLA_ERR_LOGIC;
   }
-  if (int rc = rvvla_spmv(A, alpha, A->xstage, beta, A->ystage, nullptr)) return rc;
+  if (int rc = rvvla_spmv_op(A, op, alpha, A->xstage, beta, A->ystage, nullptr)) return rc;
   return rvvla::hip_check(hipDeviceSynchronize(), "multiply");
 }
 
-int rvvla_stage_out(rvvla_mat_t A, void* y_host) {
-  if (!A || (A->m > 0 && !y_host)) return RVVLA_ERR_INVALID_ARG;
+int rvvla_stage_out_op(rvvla_mat_t A, int op, void* y_host) {
+  if (!A || (op != 0 && op != 1)) return RVVLA_ERR_INVALID_ARG;
+  const int64_t ny = op ? A->n : A->m;
+  if (ny > 0 && !y_host) return RVVLA_ERR_INVALID_ARG;
   if (!A->ystage) {
     rvvla::set_last_error("GET_RESULTS before PREPARATION");
     return RVVLA_ERR_LOGIC;
   }
   const size_t es = A->dtype == RVVLA_F64 ? 8 : 4;
-  return rvvla::hip_check(hipMemcpy(y_host, A->ystage, (size_t)A->m * es, hipMemcpyDeviceToHost), "copy y back");
+  return rvvla::hip_check(hipMemcpy(y_host, A->ystage, (size_t)ny * es, hipMemcpyDeviceToHost), "copy y back");
 }
 
+int rvvla_spmv_host_op(rvvla_mat_t A, int op, double alpha, const void* b_host, double beta,
+                       void* y_host) {
+  if (int rc = rvvla_stage_in_op(A, op, b_host, y_host)) return rc;
+  if (int rc = rvvla_mult_staged_op(A, op, alpha, beta)) return rc;
+  return rvvla_stage_out_op(A, op, y_host);
+}
+
+int rvvla_stage_in(rvvla_mat_t A, const void* b_host, const void* y_host) {
+  return rvvla_stage_in_op(A, 0, b_host, y_host);
+}
+int rvvla_mult_staged(rvvla_mat_t A, double alpha, double beta) {
+  return rvvla_mult_staged_op(A, 0, alpha, beta);
+}
+int rvvla_stage_out(rvvla_mat_t A, void* y_host) { return rvvla_stage_out_op(A, 0, y_host); }
 int rvvla_spmv_host(rvvla_mat_t A, double alpha, const void* b_host, double beta, void* y_host) {
   if (!A || (A->n > 0 && !b_host) || (A->m > 0 && !y_host)) return RVVLA_ERR_INVALID_ARG;
-  if (int rc = rvvla_stage_in(A, b_host, y_host)) return rc;
-  if (int rc = rvvla_mult_staged(A, alpha, beta)) return rc;
-  return rvvla_stage_out(A, y_host);
+  return rvvla_spmv_host_op(A, 0, alpha, b_host, beta, y_host);
 }
 
 }  // extern "C"

@@ -8,9 +8,10 @@
 // a container was filled some other way.  Stages (reference
 // include/types.h:33-38): PREPARATION copies b and y to the device,
 // MULT runs the kernel, GET_RESULTS copies y back, ALL_STAGES does all
-// three.  Status codes are the RVVLA_* codes (0 = success); the reference
-// ignores type_op, this library answers SPARSE_OPERATION_TRANSPOSE with
-// RVVLA_ERR_UNSUPPORTED_SPMV instead of computing the untransposed product.
+// three.  Status codes are the RVVLA_* codes (0 = success).  The reference
+// ignores type_op; here SPARSE_OPERATION_TRANSPOSE computes A^T b for CRS
+// matrices and is answered with RVVLA_ERR_UNSUPPORTED_SPMV by the other
+// formats instead of computing the untransposed product.
 #include <algorithm>
 #include <type_traits>
 
@@ -92,27 +93,43 @@ rvvla_mat_t image(const spMtxVNEC<T>& a) {
   return a.device.get();
 }
 
+template <class M>
+constexpr bool is_crs() {
+  return false;
+}
+template <>
+constexpr bool is_crs<spMtxCRS<double>>() {
+  return true;
+}
+template <>
+constexpr bool is_crs<spMtxCRS<float>>() {
+  return true;
+}
+
 template <class T, class M>
 sparse_matrix_status run_stage(sparse_matrix_mv_stage st, sparse_operation_t op, T alpha, const M& mat,
                                const std::vector<T>& b, T beta, std::vector<T>& y) {
   sparse_matrix_status status;
-  if (op != SPARSE_OPERATION_NON_TRANSPOSE) {
-    rvvla::set_last_error("only SPARSE_OPERATION_NON_TRANSPOSE is implemented");
+  if (op != SPARSE_OPERATION_NON_TRANSPOSE && !(op == SPARSE_OPERATION_TRANSPOSE && is_crs<M>())) {
+    rvvla::set_last_error("SPARSE_OPERATION_TRANSPOSE is implemented for CRS matrices");
     status.code = RVVLA_ERR_UNSUPPORTED_SPMV;
     return status;
   }
+  const int o = op == SPARSE_OPERATION_TRANSPOSE ? 1 : 0;
   status.code = guard([&]() {
     rvvla_mat_t A = image(mat);
     int64_t m = 0, n = 0;
     rvvla_mat_info(A, &m, &n, nullptr, nullptr, nullptr, nullptr);
-    if ((int64_t)b.size() < n || (int64_t)y.size() < m)
+    if ((int64_t)b.size() < (o ? m : n) || (int64_t)y.size() < (o ? n : m))
       throw Error(RVVLA_ERR_INVALID_ARG, "b or y shorter than the matrix");
     int rc = RVVLA_OK;
     switch (st) {
-      case SPARSE_MATRIX_MV_PREPARATION: rc = rvvla_stage_in(A, b.data(), y.data()); break;
-      case SPARSE_MATRIX_MV_MULT: rc = rvvla_mult_staged(A, alpha, beta); break;
-      case SPARSE_MATRIX_MV_GET_RESULTS: rc = rvvla_stage_out(A, y.data()); break;
-      case SPARSE_MATRIX_MV_ALL_STAGES: rc = rvvla_spmv_host(A, alpha, b.data(), beta, y.data()); break;
+      case SPARSE_MATRIX_MV_PREPARATION: rc = rvvla_stage_in_op(A, o, b.data(), y.data()); break;
+      case SPARSE_MATRIX_MV_MULT: rc = rvvla_mult_staged_op(A, o, alpha, beta); break;
+      case SPARSE_MATRIX_MV_GET_RESULTS: rc = rvvla_stage_out_op(A, o, y.data()); break;
+      case SPARSE_MATRIX_MV_ALL_STAGES:
+        rc = rvvla_spmv_host_op(A, o, alpha, b.data(), beta, y.data());
+        break;
     }
     if (rc) throw Error(rc, rvvla_strerror(rc));
   });

@@ -219,6 +219,8 @@ struct rvvla_mat_s {
   rvvla::DevLAV lav;
   rvvla::DevVNEC vnec;
   rvvla::DevCSC csc;
+  rvvla::DevCSR tcsr;  // CSR of the transpose (built on the first transposed product)
+  bool has_tcsr = false;
   // staging for rvvla_spmv_host
   void* xstage = nullptr;
   void* ystage = nullptr;
@@ -240,6 +242,9 @@ int hip_check(hipError_t e, const char* what);
 
 int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                hipStream_t s);
+// The adaptive CSR product on any CSR image of m rows.
+int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,
+                     double beta, void* y, hipStream_t s);
 int launch_csr_spmm(const rvvla_mat_s& A, int k, double alpha, const void* X, int64_t ldx,
                     double beta, void* Y, int64_t ldy, hipStream_t s);
 int launch_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,

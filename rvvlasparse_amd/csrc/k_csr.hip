@@ -197,16 +197,15 @@ __global__ __launch_bounds__(kWG) void k_csr_spmm(
 }
 
 template <class VT>
-static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                   hipStream_t s) {
+static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, double beta,
+                   void* y, hipStream_t s) {
   const bool xs_nt = nt_enabled();
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
-  const DevCSR& d = A.csr;
   const bool comp = sizeof(VT) == 2;
   if (d.xs.P > 0) {
     const DevXSlice& xs = d.xs;
-    if (A.m == 0) return RVVLA_OK;
+    if (m == 0) return RVVLA_OK;
     if (xs.nblk > 0) {
       if (comp)
         hipLaunchKernelGGL((k_csr_adaptive<VT, true, true>), dim3((unsigned)xs.nblk), dim3(kWG), 0,
@@ -223,9 +222,9 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
       if (int rc = hip_check(hipGetLastError(), "k_csr_adaptive (sliced) launch")) return rc;
     }
     const int64_t per_wg = (int64_t)(kWG / 64) * kCombineGroups * 64;
-    const unsigned g = (unsigned)((A.m + per_wg - 1) / per_wg);
+    const unsigned g = (unsigned)((m + per_wg - 1) / per_wg);
     hipLaunchKernelGGL((k_xs_combine<IO, AC>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base, xs.pref,
-                       (const IO*)xs.part, (IO*)y, A.m, xs.nq, xs.P, (AC)alpha, (AC)beta);
+                       (const IO*)xs.part, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_xs_combine launch");
   }
   if (d.nblocks == 0) return RVVLA_OK;
@@ -244,15 +243,20 @@ static int run_csr(const rvvla_mat_s& A, double alpha, const void* x, double bet
   return hip_check(hipGetLastError(), "k_csr_adaptive launch");
 }
 
-int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s) {
-  switch (A.dtype) {
-    case RVVLA_F64: return run_csr<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_csr<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_csr<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_csr<_Float16>(A, alpha, x, beta, y, s);
+int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,
+                     double beta, void* y, hipStream_t s) {
+  switch (dtype) {
+    case RVVLA_F64: return run_csr<double>(d, m, alpha, x, beta, y, s);
+    case RVVLA_F32: return run_csr<float>(d, m, alpha, x, beta, y, s);
+    case RVVLA_BF16: return run_csr<bf16_t>(d, m, alpha, x, beta, y, s);
+    case RVVLA_F16: return run_csr<_Float16>(d, m, alpha, x, beta, y, s);
   }
   return RVVLA_ERR_VALUE_TYPE;
+}
+
+int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+               hipStream_t s) {
+  return launch_csr_image(A.csr, A.m, A.dtype, alpha, x, beta, y, s);
 }
 
 template <class VT>

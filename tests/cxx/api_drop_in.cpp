@@ -83,12 +83,74 @@ void run(const char* name, const spMtxCRS<T>& a, const convertParams& P) {
   sparse_matrix_status s;
   s.code = s1.code ? s1.code : (s2.code ? s2.code : s3.code);
   check<T>(name, "staged", a, b, y0, alpha, beta, y, s);
-  // the transposed operation is refused, not silently computed untransposed
+  // the transposed operation is refused (these are not CRS), not silently
+  // computed untransposed
   auto st_t = sparse_mv<T, Fmt, true>(SPARSE_OPERATION_TRANSPOSE, alpha, d, sparse_matrix_descr{}, b,
                                       beta, y);
   if (st_t.code != RVVLA_ERR_UNSUPPORTED_SPMV) {
     ++failures;
     std::printf("%-5s transpose not refused (status %d)\n", name, st_t.code);
+  }
+}
+
+// SPARSE_OPERATION_TRANSPOSE on CRS: y = alpha * A^T b + beta * y, for a
+// non-symmetric matrix (row-scaled Laplacian plus one extra column),
+// against a plain transposed loop, in the all-stages and staged forms.
+template <class T>
+void transpose_crs(const spMtxCRS<T>& lap) {
+  const size_t m = lap.m, n = lap.n + 3;
+  spMtxCRS<T> a(m, n, lap.nz + m);
+  size_t k = 0;
+  for (size_t r = 0; r < m; ++r) {
+    a.Rst[r] = (int)k;
+    for (int e = lap.Rst[r]; e < lap.Rst[r + 1]; ++e) {
+      a.Col[k] = lap.Col[e];
+      a.Val[k++] = lap.Val[e] * (T)(1.0 + 0.001 * (double)r);
+    }
+    a.Col[k] = (int)(lap.n + r % 3);
+    a.Val[k++] = (T)(0.5 + 0.25 * (double)(r % 5));
+  }
+  a.Rst[m] = (int)k;
+  std::vector<T> b(m), y0(n);
+  for (size_t i = 0; i < m; ++i) b[i] = (T)std::sin(0.21 * (double)i);
+  for (size_t i = 0; i < n; ++i) y0[i] = (T)std::cos(0.13 * (double)i);
+  const T alpha = (T)0.75, beta = (T)2;
+  std::vector<double> ref(n, 0.0), sa(n, 0.0);
+  std::vector<int> len(n, 0);
+  for (size_t r = 0; r < m; ++r)
+    for (int e = a.Rst[r]; e < a.Rst[r + 1]; ++e) {
+      ref[a.Col[e]] += (double)a.Val[e] * b[r];
+      sa[a.Col[e]] += std::fabs((double)a.Val[e] * b[r]);
+      len[a.Col[e]]++;
+    }
+  const double eps = std::is_same<T, double>::value ? 1.1102230246251565e-16 : 5.960464477539063e-08;
+  for (int form = 0; form < 2; ++form) {
+    std::vector<T> y = y0;
+    sparse_matrix_status st;
+    if (form == 0) {
+      st = sparse_mv<T, spMtxCRS, true>(SPARSE_OPERATION_TRANSPOSE, alpha, a, sparse_matrix_descr{},
+                                        b, beta, y);
+    } else {
+      auto s1 = sparse_mv<T, spMtxCRS, true, SPARSE_MATRIX_MV_PREPARATION>(
+          SPARSE_OPERATION_TRANSPOSE, alpha, a, sparse_matrix_descr{}, b, beta, y);
+      auto s2 = sparse_mv<T, spMtxCRS, true, SPARSE_MATRIX_MV_MULT>(
+          SPARSE_OPERATION_TRANSPOSE, alpha, a, sparse_matrix_descr{}, b, beta, y);
+      auto s3 = sparse_mv<T, spMtxCRS, true, SPARSE_MATRIX_MV_GET_RESULTS>(
+          SPARSE_OPERATION_TRANSPOSE, alpha, a, sparse_matrix_descr{}, b, beta, y);
+      st.code = s1.code ? s1.code : (s2.code ? s2.code : s3.code);
+    }
+    double worst = 0;
+    for (size_t c = 0; c < n; ++c) {
+      const double want = alpha * ref[c] + beta * (double)y0[c];
+      const double scale = std::fabs(alpha) * sa[c] + std::fabs(beta) * std::fabs((double)y0[c]);
+      const double err = std::fabs((double)y[c] - want) / (4 * (len[c] + 2) * eps * (scale > 0 ? scale : 1));
+      if (err > worst) worst = err;
+    }
+    const bool ok = st.code == 0 && worst <= 1.0;
+    if (!ok) ++failures;
+    std::printf("csr^T %-6s %-7s status=%d err/bound=%.3f %s\n",
+                std::is_same<T, double>::value ? "double" : "float", form ? "staged" : "all", st.code,
+                worst, ok ? "ok" : "FAIL");
   }
 }
 
@@ -113,6 +175,7 @@ void all_formats(int g) {
   run<T, spMtxVNEC>("vnec", a, none);
   run<T, spMtxVHCC>("vhcc", a, none);
   run<T, spMtxCVR>("cvr", a, none);
+  transpose_crs<T>(a);
 }
 
 int main(int argc, char** argv) {

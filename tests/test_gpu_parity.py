@@ -228,8 +228,9 @@ def test_hcsr_upload_unsorted_coo_blocks(dtype):
         shuffled += 1
     assert shuffled > 0
     h = C.c_void_p()
+    bp32 = bp.astype(np.int32)  # kept alive across the call
     R.check(R.lib.rvvla_hcsr_upload(dtype, a.m, a.n, a.nnz, 300, 400, int(L.scalar("b_m")),
-                                    int(L.scalar("b_n")), bp.astype(np.int32).ctypes.data,
+                                    int(L.scalar("b_n")), bp32.ctypes.data,
                                     ro.ctypes.data, rp.shape[0], rp.ctypes.data, ci.ctypes.data,
                                     va.ctypes.data, C.byref(h)))
     M = R.DeviceMatrix(h)
@@ -243,6 +244,74 @@ def test_hcsr_upload_unsorted_coo_blocks(dtype):
     y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.0, -0.5, y064.astype(dt))
     assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.0, -0.5,
                         y064.astype(dt), dt, "hcsr unsorted COO")
+
+
+def _transpose_csr(a: R.CSR) -> R.CSR:
+    """CSR of A^T: entries of each column of A in row order."""
+    order = np.argsort(a.col, kind="stable")
+    rows = np.repeat(np.arange(a.m, dtype=np.int32), np.diff(a.rst))
+    cnt = np.bincount(a.col, minlength=a.n)
+    rst = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    return R.CSR(a.n, a.m, rst, rows[order], a.val[order])
+
+
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("name", ["rand_long", "powerlaw", "one_long_row", "empty", "lap64"])
+def test_spmv_transpose_csr(dtype, name):
+    """SPARSE_OPERATION_TRANSPOSE: y = alpha*A^T*x + beta*y on CSR, against the
+    oracle's CSR product with the transpose (also the staged host path)."""
+    a = case(name)
+    t = _transpose_csr(a)
+    dev = _device()
+    rng = np.random.default_rng(12)
+    x64, y064 = rng.uniform(-1, 1, a.m), rng.uniform(-1, 1, a.n)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    M = R.convert("csr", a, dtype).upload()
+    for alpha, beta in ((1.0, 0.0), (0.5, -2.0)):
+        x = torch.from_numpy(x64).to(io).to(dev)
+        y = torch.from_numpy(y064).to(io).to(dev)
+        M.spmv_t(x, y, alpha, beta)
+        torch.cuda.synchronize()
+        v, xx, y0 = t.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        y_ref = O.csr_spmv(t.rst, t.col, v, xx, alpha, beta, y0)
+        assert_within_bound(y.cpu().numpy(), y_ref, t.rst, t.col, v, xx, alpha, beta, y0, dt,
+                            f"A^T {name} a={alpha} b={beta}")
+    yh = y064.astype(dt).copy()
+    xh = x64.astype(dt)  # kept alive across the call
+    R.check(R.lib.rvvla_spmv_host_op(M.handle, 1, 1.0, xh.ctypes.data, 0.0, yh.ctypes.data))
+    y_ref = O.csr_spmv(t.rst, t.col, t.values_for(dtype), x64.astype(dt), 1.0, 0.0,
+                       np.zeros(a.n, dt))
+    assert_within_bound(yh, y_ref, t.rst, t.col, t.values_for(dtype), x64.astype(dt), 1.0, 0.0,
+                        np.zeros(a.n, dt), dt, f"A^T host {name}")
+
+
+@pytest.mark.parametrize("dtype", [R.BF16, R.F16])
+def test_spmv_transpose_16bit(dtype):
+    a = case("rand_long")
+    t = _transpose_csr(a)
+    dev = _device()
+    rng = np.random.default_rng(13)
+    x64 = rng.uniform(-1, 1, a.m).astype(np.float32).astype(np.float64)
+    M = R.convert("csr", a, dtype).upload()
+    x = torch.from_numpy(x64).float().to(dev)
+    y = torch.zeros(a.n, dtype=torch.float32, device=dev)
+    M.spmv_t(x, y)
+    y64 = O.csr_spmv(t.rst, t.col, t.val, x64, 1.0, 0.0, np.zeros(a.n))
+    rel = np.linalg.norm(y.cpu().numpy() - y64) / np.linalg.norm(y64)
+    assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
+
+
+@pytest.mark.parametrize("fmt", [f for f in FORMATS_READY if f != "csr"])
+def test_spmv_transpose_refused_elsewhere(fmt):
+    a = case("lap64")
+    M = R.convert(fmt, a, R.F64).upload()
+    dev = _device()
+    x = torch.zeros(a.m, dtype=torch.float64, device=dev)
+    y = torch.zeros(a.n, dtype=torch.float64, device=dev)
+    with pytest.raises(R.RvvlaError) as e:
+        M.spmv_t(x, y)
+    assert e.value.code == -4
 
 
 def test_spmv_host_path():
