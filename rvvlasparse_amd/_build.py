@@ -71,6 +71,14 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     for demo in sorted((ROOT / "tests" / "cxx").glob("*.cpp")):
         _run(["g++", "-O2", "-std=c++17", f"-I{INCLUDE}", str(demo), f"-L{PKG}", "-lrvvla_hip",
               "-Wl,-rpath,$ORIGIN/../rvvlasparse_amd", "-o", str(BUILD / demo.stem)])
+    # the accuracy and performance programs (C-ABI users), into rvvlasparse_amd/bin
+    bindir = PKG / "bin"
+    bindir.mkdir(exist_ok=True)
+    for app in sorted((CSRC.parent / "apps").glob("*.cpp")):
+        _run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}",
+              f"-I{ROCM / 'include'}", str(app), f"-L{PKG}", "-lrvvla_hip",
+              f"-L{ROCM / 'lib'}", "-lamdhip64", "-Wl,-rpath,$ORIGIN/..",
+              f"-Wl,-rpath,{ROCM / 'lib'}", "-o", str(bindir / app.stem)])
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", flush=True)
     return LIB
