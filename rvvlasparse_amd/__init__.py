@@ -208,7 +208,7 @@ class Layout:
         return DeviceMatrix(h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:  # lib is None at interpreter exit
             lib.rvvla_layout_free(self._h)
             self._h = None
 
@@ -258,7 +258,7 @@ class DeviceMatrix:
         check(lib.rvvla_spmv_host(self._h, alpha, _ptr(b), beta, _ptr(y)))
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:  # lib is None at interpreter exit
             lib.rvvla_mat_free(self._h)
             self._h = None
 
@@ -340,7 +340,7 @@ class Batch:
         check(lib.rvvla_batch_replay(self._h, stream))
 
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and lib is not None:  # lib is None at interpreter exit
             lib.rvvla_batch_free(self._h)
             self._h = None
 
