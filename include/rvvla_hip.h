@@ -126,6 +126,27 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz,
  * (RVVLA_ERR_NO_DEVICE).  For validating layouts where no GPU is present. */
 int rvvla_set_host_validation(int on);
 
+/* Device-side conversion (no reference counterpart: the reference converts
+ * on the host, about 0.3 s for a 4.2 M-row SELL-C-sigma, thousands of
+ * products' worth).  Builds the SELL-C-sigma (C = 64) image straight from a
+ * CSR already on the device (d_val: double for F64, float otherwise; BF16 /
+ * F16 values are rounded on the device) by radix sort, scan and scatter,
+ * ordered on `stream`, synchronously.  The CSR is checked on the device
+ * first (RVVLA_ERR_INVALID_ARG when bad).  sigma <= 0 applies the default
+ * rule (1 when natural 64-row slices pad by at most 1 %, else 16384).  The
+ * arrays equal convert(RVVLA_FMT_SELL) + upload bit for bit (the convert
+ * path's host fields can be read back with rvvla_mat_read). */
+int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                           int sigma, void* stream, rvvla_mat_t* out);
+
+/* Copy of one array of an uploaded matrix into host memory (device or
+ * host-validation matrices): SELL "vCl", "vCs", "vBs", "vSBs", "vCol",
+ * "vPerm", "vVal"; CSR "rst", "col", "val".  dst = NULL asks for *count
+ * and *elem_bytes only; capacity is in bytes (RVVLA_ERR_LENGTH if short). */
+int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity,
+                   int64_t* count, int* elem_bytes);
+
 /* Read-only view of one array of a matrix uploaded in host-validation mode
  * (no reference counterpart; for layout tests).  CSR fields: "rst", "col",
  * "blocks" and, when the column-sliced image was built, "xs.rst", "xs.col",

@@ -70,6 +70,9 @@ _sigs = {
                                  _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "rvvla_set_host_validation": (_i32, [_i32]),
     "rvvla_mat_view": (_i32, [_vp, _cp, C.POINTER(_vp), C.POINTER(_i64), C.POINTER(_i32)]),
+    "rvvla_mat_read": (_i32, [_vp, _cp, _vp, _i64, C.POINTER(_i64), C.POINTER(_i32)]),
+    "rvvla_sell_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp,
+                                      C.POINTER(_vp)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -254,6 +257,15 @@ class DeviceMatrix:
         buf = (C.c_char * (cnt.value * eb.value)).from_address(d.value)
         return np.frombuffer(buf, dtype=raw).copy()
 
+    def read(self, field: str) -> np.ndarray:
+        """Copy of one array of the uploaded image (device or host-validation)."""
+        cnt, eb = C.c_int64(), C.c_int()
+        check(lib.rvvla_mat_read(self._h, field.encode(), None, 0, C.byref(cnt), C.byref(eb)))
+        out = np.empty(cnt.value, dtype=_ELEM[eb.value])
+        check(lib.rvvla_mat_read(self._h, field.encode(), out.ctypes.data if out.size else None,
+                                 out.nbytes, C.byref(cnt), C.byref(eb)))
+        return out
+
     def spmv_host(self, b: np.ndarray, y: np.ndarray, alpha: float = 1.0, beta: float = 0.0):
         check(lib.rvvla_spmv_host(self._h, alpha, _ptr(b), beta, _ptr(y)))
 
@@ -272,6 +284,18 @@ def _stream(stream, like):
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
+
+
+def sell_from_device(dtype: int, m: int, n: int, rst, col, val, sigma: int = 0,
+                     stream=None) -> DeviceMatrix:
+    """SELL-C-sigma image built on the device from device CSR arrays (torch
+    tensors or raw addresses): int32 rst/col, float64 (F64) or float32
+    values.  sigma <= 0: the default rule."""
+    h = C.c_void_p()
+    nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
+    check(lib.rvvla_sell_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), sigma,
+                                     _stream(stream, val), C.byref(h)))
+    return DeviceMatrix(h)
 
 
 def csr_upload(a: CSR, dtype: int = F64) -> DeviceMatrix:

@@ -114,12 +114,26 @@ static std::atomic<int> g_host_validation{0};
 }
 
 rvvla_mat_s::~rvvla_mat_s() {
+  free_temps();
   for (void* p : allocs) {
     if (host_only) std::free(p);
     else (void)hipFree(p);
   }
   if (xstage) (void)hipFree(xstage);
   if (ystage) (void)hipFree(ystage);
+}
+
+void* rvvla_mat_s::temp(size_t bytes) {
+  void* d = nullptr;
+  const int rc = rvvla::hip_check(hipMalloc(&d, bytes ? bytes : 8), "hipMalloc");
+  if (rc) throw Error(rc, rvvla::last_error());
+  temps.push_back(d);
+  return d;
+}
+
+void rvvla_mat_s::free_temps() {
+  for (void* p : temps) (void)hipFree(p);
+  temps.clear();
 }
 
 void* rvvla_mat_s::put(const void* host, size_t bytes) {
@@ -1152,6 +1166,81 @@ int rvvla_sell_upload(int dtype, int64_t m, int64_t n, int64_t nz, int64_t cnt_b
     }
     return RVVLA_OK;
   });
+}
+
+int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                           const int32_t* d_col, const void* d_val, int sigma, void* stream,
+                           rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_SELL, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+                       ? rvvla::build_sell_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, sigma, s)
+                       : rvvla::build_sell_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, sigma, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
+int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity, int64_t* count,
+                   int* elem_bytes) {
+  if (!A || !field || !count || !elem_bytes) return RVVLA_ERR_INVALID_ARG;
+  const std::string f = field;
+  const int vb = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
+  struct V { const void* p; int64_t n; int e; };
+  V v{nullptr, -1, 0};
+  if (A->format == RVVLA_FMT_SELL) {
+    const rvvla::DevSELL& d = A->sell;
+    int64_t slots = 0;
+    if (d.nslices > 0) {
+      int32_t cs = 0, cl = 0, bs = 0;
+      const int64_t l = d.nslices - 1;
+      if (A->host_only) {
+        cs = d.cs[l];
+        cl = d.cl[l];
+        bs = d.bs[l];
+      } else {
+        int rc = rvvla::hip_check(hipMemcpy(&cs, d.cs + l, 4, hipMemcpyDeviceToHost), "read");
+        if (!rc) rc = rvvla::hip_check(hipMemcpy(&cl, d.cl + l, 4, hipMemcpyDeviceToHost), "read");
+        if (!rc) rc = rvvla::hip_check(hipMemcpy(&bs, d.bs + l, 4, hipMemcpyDeviceToHost), "read");
+        if (rc) return rc;
+      }
+      slots = (int64_t)cs + (int64_t)cl * bs;
+    }
+    v = f == "vCl" ? V{d.cl, d.nslices, 4} : f == "vCs" ? V{d.cs, d.nslices, 4}
+      : f == "vBs" ? V{d.bs, d.nslices, 4} : f == "vSBs" ? V{d.sbs, d.nslices, 4}
+      : f == "vCol" ? V{d.col, slots, 4} : f == "vPerm" ? V{d.perm, A->m, 4}
+      : f == "vVal" ? V{d.val, slots, vb} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_CSR) {
+    const rvvla::DevCSR& c = A->csr;
+    v = f == "rst" ? V{c.rst, A->m + 1, 4} : f == "col" ? V{c.col, A->nnz, 4}
+      : f == "val" ? V{c.val, A->nnz, vb} : V{nullptr, -1, 0};
+  }
+  if (v.n < 0) {
+    rvvla::set_last_error("rvvla_mat_read: unknown field " + f + " for this format");
+    return RVVLA_ERR_NOT_FOUND;
+  }
+  *count = v.n;
+  *elem_bytes = v.e;
+  if (!dst) return RVVLA_OK;  // size query
+  if (capacity < v.n * v.e) return RVVLA_ERR_LENGTH;
+  if (v.n == 0) return RVVLA_OK;
+  if (A->host_only) {
+    std::memcpy(dst, v.p, (size_t)(v.n * v.e));
+    return RVVLA_OK;
+  }
+  return rvvla::hip_check(hipMemcpy(dst, v.p, (size_t)(v.n * v.e), hipMemcpyDeviceToHost), "read");
 }
 
 int rvvla_set_host_validation(int on) {

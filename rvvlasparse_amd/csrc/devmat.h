@@ -225,10 +225,15 @@ struct rvvla_mat_s {
   void* xstage = nullptr;
   void* ystage = nullptr;
 
+  std::vector<void*> temps;  // device buffers of a device-side build
+
   ~rvvla_mat_s();
   // Allocate `bytes` of device memory, copy `host` into it (if non-null)
   // and keep it for the lifetime of the matrix.
   void* put(const void* host, size_t bytes);
+  // Device buffer released by free_temps() (or with the matrix).
+  void* temp(size_t bytes);
+  void free_temps();
   template <class T>
   const T* put_vec(const std::vector<T>& v) {
     return (const T*)put(v.data(), v.size() * sizeof(T));
@@ -242,6 +247,15 @@ int hip_check(hipError_t e, const char* what);
 
 int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                hipStream_t s);
+// Device-side CSR checks (RVVLA_ERR_INVALID_ARG on a bad CSR) and the
+// CSR -> SELL-64 build into M.sell (k_convert.hip); sigma <= 0 picks the
+// default rule on the device.
+int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const int32_t* rst,
+                     const int32_t* col, hipStream_t s);
+template <class V>
+int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
+                      const int32_t* col, const V* val, int sigma, hipStream_t s);
+
 // The adaptive CSR product on any CSR image of m rows.
 int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,
                      double beta, void* y, hipStream_t s);

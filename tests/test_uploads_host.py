@@ -120,3 +120,35 @@ def test_products_refused_in_host_mode():
                             None) == NO_DEVICE
     x = np.zeros(a.n, np.float32)
     assert R.lib.rvvla_spmv_host(M.handle, 1.0, x.ctypes.data, 0.0, x.ctypes.data) == NO_DEVICE
+
+
+@pytest.mark.parametrize("name", ["rand_long", "lap", "empty"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_mat_read_matches_layout(name, dtype):
+    """rvvla_mat_read returns the uploaded SELL / CSR arrays unchanged."""
+    a = CASES[name]
+    L = R.convert("sell", a, dtype)
+    M = L.upload()
+    for f in ("vCl", "vCs", "vBs", "vSBs", "vCol", "vPerm"):
+        assert np.array_equal(M.read(f), L.array(f)), f
+    v = M.read("vVal")
+    if dtype == R.F64:
+        assert np.array_equal(v.view(np.float64), L.array("vVal", np.float64))
+    elif dtype == R.F32:
+        assert np.array_equal(v.view(np.float32), L.array("vVal", np.float32))
+    else:
+        assert v.dtype == np.uint16 and v.size == L.array("vCol").size
+    C_ = R.convert("csr", a, dtype).upload()
+    assert np.array_equal(C_.read("rst"), a.rst.astype(np.uint32))
+    assert np.array_equal(C_.read("col"), a.col.astype(np.uint32))
+    with pytest.raises(R.RvvlaError) as e:
+        C_.read("vCl")
+    assert e.value.code == -23
+
+
+def test_device_conversion_needs_a_device():
+    a = CASES["lap"]
+    h = C.c_void_p()
+    rc = R.lib.rvvla_sell_from_device(R.F64, a.m, a.n, a.nnz, a.rst.ctypes.data,
+                                      a.col.ctypes.data, a.val.ctypes.data, 0, None, C.byref(h))
+    assert rc == NO_DEVICE and not h.value
