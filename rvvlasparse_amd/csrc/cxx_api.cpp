@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "devmat.h"
 #include "internal.h"
 
 namespace SparseMatrixLib {
@@ -207,11 +208,85 @@ void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>&
 #define RVVLA_CONVERT_BOTH(FMT, BUILD, UPLOAD)                                                     \
   RVVLA_CONVERT(double, FMT, BUILD, UPLOAD) RVVLA_CONVERT(float, FMT, BUILD, UPLOAD)
 
-// SELL-C-sigma keys "C" (64), "sigma"
-RVVLA_CONVERT_BOTH(spMtxSELL_C_Sigma,
-                   rvvla::build_sell(d, s, iparam(p, "C", 64),
-                                     iparam(p, "sigma", rvvla::default_sell_sigma((int64_t)s.m, (int64_t)s.nz, s.Rst))),
-                   rvvla::upload_sell(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+// SELL-C-sigma keys "C" (64), "sigma", "device" (1: build on the device from
+// the uploaded CSR, then read the fields back; same arrays as the host build)
+namespace {
+
+template <class T>
+void read_field(rvvla_mat_t A, const char* f, void* dst, size_t bytes) {
+  int64_t cnt = 0;
+  int eb = 0;
+  const int rc = rvvla_mat_read(A, f, dst, (int64_t)bytes, &cnt, &eb);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+}
+
+template <class T>
+void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int C = iparam(p, "C", 64);
+  const int sigma = iparam(p, "sigma", rvvla::default_sell_sigma((int64_t)s.m, (int64_t)s.nz, s.Rst));
+  if (!iparam(p, "device", 0) || C != 64) {
+    rvvla::build_sell(d, s, C, sigma);
+    d.device.reset(rvvla::upload_sell(dtype_of<T>(), d));
+    return;
+  }
+  rvvla::check_csr_public((int64_t)s.m, (int64_t)s.n, (int64_t)s.nz, s.Rst, s.Col);
+  struct Buf {
+    void* p = nullptr;
+    ~Buf() {
+      if (p) (void)hipFree(p);
+    }
+  } rst, col, val;
+  const size_t m = s.m, nz = s.nz;
+  int rc = rvvla::hip_check(hipMalloc(&rst.p, (m + 1) * 4), "hipMalloc");
+  if (!rc) rc = rvvla::hip_check(hipMalloc(&col.p, nz * 4 + 8), "hipMalloc");
+  if (!rc) rc = rvvla::hip_check(hipMalloc(&val.p, nz * sizeof(T) + 8), "hipMalloc");
+  if (!rc) rc = rvvla::hip_check(hipMemcpy(rst.p, s.Rst, (m + 1) * 4, hipMemcpyHostToDevice), "copy");
+  if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(col.p, s.Col, nz * 4, hipMemcpyHostToDevice), "copy");
+  if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(val.p, s.Val, nz * sizeof(T), hipMemcpyHostToDevice), "copy");
+  if (rc) throw Error(rc, rvvla::last_error());
+  rvvla_mat_t A = nullptr;
+  rc = rvvla_sell_from_device(dtype_of<T>(), (int64_t)m, (int64_t)s.n, (int64_t)nz,
+                              (const int32_t*)rst.p, (const int32_t*)col.p, val.p, sigma, nullptr, &A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  d.device.reset(A);
+  d.m = s.m;
+  d.n = s.n;
+  d.nz = s.nz;
+  d.C = 64;
+  d.Sigma = sigma;
+  d.cnt_b = (m + 63) / 64;
+  int64_t slots = 0;
+  int eb = 0;
+  rc = rvvla_mat_read(A, "vCol", nullptr, 0, &slots, &eb);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  d.vCl.resize(d.cnt_b);
+  d.vCs.resize(d.cnt_b);
+  d.vBs.resize(d.cnt_b);
+  d.vSBs.resize(d.cnt_b);
+  d.vPerm.resize(m);
+  d.vCol.resize((size_t)slots);
+  d.vVal.resize((size_t)slots);
+  read_field<T>(A, "vCl", d.vCl.data(), d.vCl.size() * sizeof(d.vCl[0]));
+  read_field<T>(A, "vCs", d.vCs.data(), d.vCs.size() * sizeof(d.vCs[0]));
+  read_field<T>(A, "vBs", d.vBs.data(), d.vBs.size() * sizeof(d.vBs[0]));
+  read_field<T>(A, "vSBs", d.vSBs.data(), d.vSBs.size() * sizeof(d.vSBs[0]));
+  read_field<T>(A, "vPerm", d.vPerm.data(), d.vPerm.size() * sizeof(d.vPerm[0]));
+  read_field<T>(A, "vCol", d.vCol.data(), d.vCol.size() * sizeof(d.vCol[0]));
+  read_field<T>(A, "vVal", d.vVal.data(), d.vVal.size() * sizeof(T));
+}
+
+}  // namespace
+
+template <>
+void convert<double, spMtxSELL_C_Sigma, true>(spMtxSELL_C_Sigma<double>& d,
+                                              const spMtxCRS<double>& s, const convertParams& p) {
+  convert_sell<double>(d, s, p);
+}
+template <>
+void convert<float, spMtxSELL_C_Sigma, true>(spMtxSELL_C_Sigma<float>& d,
+                                             const spMtxCRS<float>& s, const convertParams& p) {
+  convert_sell<float>(d, s, p);
+}
 // CSR5 keys "omega" (64), "sigma" (0 = chosen from the row lengths)
 RVVLA_CONVERT_BOTH(spMtxCSR5, rvvla::build_csr5(d, s, iparam(p, "omega", 64), iparam(p, "sigma", 0)),
                    rvvla::upload_csr5(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))

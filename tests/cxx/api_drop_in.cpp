@@ -169,6 +169,23 @@ void all_formats(int g) {
     check<T>("csr", "all", a, b, y0, (T)1, (T)0, y, st);
   }
   run<T, spMtxHCSR>("hcsr", a, hp);
+  // SELL built on the device ("device" = 1): same fields as the host build
+  {
+    convertParams dev;
+    dev.param["device"].i = 1;
+    spMtxSELL_C_Sigma<T> h, d;
+    convert<T, spMtxSELL_C_Sigma, true>(h, a, none);
+    convert<T, spMtxSELL_C_Sigma, true>(d, a, dev);
+    const bool same = h.Sigma == d.Sigma && h.cnt_b == d.cnt_b && h.vCl == d.vCl &&
+                      h.vCs == d.vCs && h.vBs == d.vBs && h.vSBs == d.vSBs && h.vCol == d.vCol &&
+                      h.vPerm == d.vPerm && h.vVal == d.vVal;
+    if (!same) ++failures;
+    std::printf("sell  %-6s device-built fields %s\n", std::is_same<T, double>::value ? "double" : "float",
+                same ? "ok" : "FAIL");
+    convertParams dev16 = dev;
+    dev16.param["sigma"].i = 16;
+    run<T, spMtxSELL_C_Sigma>("sell", a, dev16);
+  }
   run<T, spMtxSELL_C_Sigma>("sell", a, none);
   run<T, spMtxCSR5>("csr5", a, none);
   run<T, spMtxLAV>("lav", a, none);
