@@ -46,15 +46,23 @@ __device__ __forceinline__ uint64_t csr5_flags(const uint32_t* w, int sigma, int
   return fl;
 }
 
-template <class VT>
-__global__ __launch_bounds__(kWG) void k_csr5_tiles(
+// SIG > 0: sigma known at compile time, one tile per 64-thread workgroup,
+// and the x gathers are made in CSR order through LDS: the tile's columns
+// are staged (transposed back to CSR order), each lane gathers x for 64
+// consecutive entries at a time, and the values of x go back to the lane
+// that owns the entry in the tile layout.  SIG = 0: any sigma, four tiles
+// per workgroup, gathers in tile order.
+template <class VT, int SIG>
+__global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
     Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ first_part,
     typename Val<VT>::acc* __restrict__ last_part, int32_t* __restrict__ last_row,
     typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t t = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-  if (t >= a.ntiles) return;  // uniform per wavefront
+  constexpr int TPB = SIG > 0 ? 1 : kWG / 64;  // tiles per workgroup
+  __shared__ A xb[SIG > 0 ? 64 * (SIG + 1) : 1];
+  const int64_t t = (int64_t)blockIdx.x * TPB + (threadIdx.x >> 6);
+  if (t >= a.ntiles) return;  // uniform per wavefront (per workgroup when SIG > 0)
   const int lane = threadIdx.x & 63;
   const int sigma = a.sigma, f0 = 32 - a.bit_y - a.bit_seg;
   const uint32_t tp = a.tile_ptr[t];
@@ -75,35 +83,68 @@ __global__ __launch_bounds__(kWG) void k_csr5_tiles(
   uint32_t nfl = 0;
   const int64_t base = t * 64 * (int64_t)sigma + lane;
   const uint32_t* __restrict__ colp = a.col;
-  // kChunk lane steps at a time: all column and value loads, then all x
-  // gathers, then the flag walk, so that many loads are in flight
-  constexpr int kChunk = 8;
-  for (int i0 = 0; i0 < sigma; i0 += kChunk) {
-    uint32_t c[kChunk];
-    A v[kChunk], p[kChunk];
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const int64_t e = base + (int64_t)(i0 + j) * 64;
-      c[j] = i0 + j < sigma ? colp[e] : 0u;
-      v[j] = i0 + j < sigma ? Val<VT>::get(val, e) : A(0);
-    }
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) p[j] = i0 + j < sigma ? v[j] * (A)x[c[j]] : A(0);
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const int i = i0 + j;
-      if (i >= sigma) break;
-      if ((fl >> i) & 1u) {
-        if (nfl == 0) {
-          head = cur;
-        } else {
-          const int64_t r = row_of(yoff + nfl);
-          y[r] = alpha * cur + scaled_y(beta, y, r);
-        }
-        cur = 0;
-        ++nfl;
+  auto walk = [&](int i, A p) {  // the flag walk over lane step i
+    if ((fl >> i) & 1u) {
+      if (nfl == 0) {
+        head = cur;
+      } else {
+        const int64_t r = row_of(yoff + nfl);
+        y[r] = alpha * cur + scaled_y(beta, y, r);
       }
-      cur += p[j];
+      cur = 0;
+      ++nfl;
+    }
+    cur += p;
+  };
+  if constexpr (SIG > 0) {
+    constexpr int W = sizeof(A) / 4;  // 32-bit words per LDS slot
+    uint32_t* cbits = reinterpret_cast<uint32_t*>(xb);
+    A v[SIG];
+#pragma unroll
+    for (int i = 0; i < SIG; ++i) {
+      const int64_t e = base + (int64_t)i * 64;
+      const uint32_t c = colp[e];
+      v[i] = Val<VT>::get(val, e);
+      cbits[(lane * (SIG + 1) + i) * W] = c;  // entry lane*SIG + i, padded row
+    }
+    __syncthreads();
+    uint32_t cc[SIG];
+#pragma unroll
+    for (int q = 0; q < SIG; ++q) {
+      const int j = q * 64 + lane;  // CSR order within the tile
+      cc[q] = cbits[((j / SIG) * (SIG + 1) + (j % SIG)) * W];
+    }
+    A g[SIG];
+#pragma unroll
+    for (int q = 0; q < SIG; ++q) g[q] = (A)x[cc[q]];
+#pragma unroll
+    for (int q = 0; q < SIG; ++q) {
+      const int j = q * 64 + lane;
+      xb[(j / SIG) * (SIG + 1) + (j % SIG)] = g[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SIG; ++i) walk(i, v[i] * xb[lane * (SIG + 1) + i]);
+  } else {
+    // kChunk lane steps at a time: all column and value loads, then all x
+    // gathers, then the flag walk, so that many loads are in flight
+    constexpr int kChunk = 8;
+    for (int i0 = 0; i0 < sigma; i0 += kChunk) {
+      uint32_t c[kChunk];
+      A v[kChunk], p[kChunk];
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        const int64_t e = base + (int64_t)(i0 + j) * 64;
+        c[j] = i0 + j < sigma ? colp[e] : 0u;
+        v[j] = i0 + j < sigma ? Val<VT>::get(val, e) : A(0);
+      }
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) p[j] = i0 + j < sigma ? v[j] * (A)x[c[j]] : A(0);
+#pragma unroll
+      for (int j = 0; j < kChunk; ++j) {
+        if (i0 + j >= sigma) break;
+        walk(i0 + j, p[j]);
+      }
     }
   }
   const bool flagged = nfl > 0;
@@ -198,9 +239,17 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
   AC* lp = fp + d.ntiles;
   int32_t* lr = (int32_t*)(lp + d.ntiles);
   if (d.ntiles > 0) {
-    const unsigned g1 = (unsigned)((d.ntiles + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_csr5_tiles<VT>), dim3(g1), dim3(kWG), 0, s, a, (const VT*)d.val,
-                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    if (d.sigma == 16) {
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 16>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+                         (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    } else if (d.sigma == 8) {
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 8>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+                         (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    } else {
+      const unsigned g1 = (unsigned)((d.ntiles + kWG / 64 - 1) / (kWG / 64));
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 0>), dim3(g1), dim3(kWG), 0, s, a, (const VT*)d.val,
+                         (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    }
     if (int rc = hip_check(hipGetLastError(), "k_csr5_tiles launch")) return rc;
   }
   const int64_t nfix = d.ntiles + (A.m - d.first_tail_row) + d.nempty;

@@ -168,6 +168,28 @@ def test_csr_column_slices_16bit(dtype):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
+@pytest.mark.parametrize("sigma", [4, 8, 12, 16, 32])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+@pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64", "random16_c1"])
+def test_csr5_sigmas(sigma, dtype, name):
+    """CSR5 tile kernels: the LDS-staged ones (sigma 8, 16) and the general
+    one (other sigmas), against the oracle."""
+    a = case(name)
+    x64, y064 = inputs(a, 31)
+    y = run_gpu("csr5", dtype, a, x64, y064, 0.75, -1.0, f"sigma={sigma}")
+    if dtype == R.BF16:
+        x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+        y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 0.75, -1.0,
+                         y032.astype(np.float64))
+        assert np.linalg.norm(y - y64) / np.linalg.norm(y64) <= 5e-3
+        return
+    dt = np.float64 if dtype == R.F64 else np.float32
+    v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x, 0.75, -1.0, y0)
+    assert_within_bound(y, y_ref, a.rst, a.col, v, x, 0.75, -1.0, y0, dt,
+                        f"csr5 sigma={sigma} {name}")
+
+
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_direct_uploads_match_converted(dtype):
     """rvvla_csr_upload / rvvla_hcsr_upload / rvvla_sell_upload (reference
