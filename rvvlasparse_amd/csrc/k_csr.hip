@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
 // of each per lane; the pair of row r (group q) in slice p is base[p] +
 // pref[p*nq + q] + (lower lanes with bit p set).  All partial loads of all
 // groups are issued before the (in-order) sums.
-constexpr int kCombineGroups = 2;
+constexpr int kCombineGroups = 1;
 template <class IO, class A>
 __global__ __launch_bounds__(kWG) void k_xs_combine(
     const uint16_t* __restrict__ mask, const int32_t* __restrict__ base,
