@@ -881,7 +881,7 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
     case RVVLA_FMT_CVR: {
       auto c = std::make_shared<spMtxCVR<V>>();
       build_cvr<V>(*c, *crs, param_int(P, "SIMD_LEN", 64),
-                   param_int(P, "streams", default_parts((int64_t)crs->m, (int64_t)crs->nz, 2048)));
+                   param_int(P, "streams", default_parts((int64_t)crs->m, (int64_t)crs->nz, kCvrStreamEntries)));
       L.obj = c;
       reg(L, "vPack_vec_vals", c->vPack_vec_vals);
       reg(L, "vPack_vec_cols", c->vPack_vec_cols);
@@ -898,7 +898,7 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
     case RVVLA_FMT_VHCC: {
       auto c = std::make_shared<spMtxVHCC<V>>();
       build_vhcc<V>(*c, *crs, param_int(P, "vec_size", 64),
-                    param_int(P, "threads", default_parts((int64_t)crs->m, (int64_t)crs->nz, 4096)));
+                    param_int(P, "threads", default_parts((int64_t)crs->m, (int64_t)crs->nz, kVhccPartEntries)));
       L.obj = c;
       reg(L, "_veceor_ptr", c->veceor());
       reg(L, "_scan_mask", c->scan_mask());

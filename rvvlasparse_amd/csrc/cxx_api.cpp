@@ -298,12 +298,12 @@ RVVLA_CONVERT_BOTH(spMtxLAV,
 // CVR keys "SIMD_LEN" (64), "streams"
 RVVLA_CONVERT_BOTH(spMtxCVR,
                    rvvla::build_cvr(d, s, iparam(p, "SIMD_LEN", 64),
-                                    iparam(p, "streams", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, 2048))),
+                                    iparam(p, "streams", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kCvrStreamEntries))),
                    rvvla::upload_cvr(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, (int64_t)s.nz))
 // VHCC keys "vec_size" (64), "threads"
 RVVLA_CONVERT_BOTH(spMtxVHCC,
                    rvvla::build_vhcc(d, s, iparam(p, "vec_size", 64),
-                                     iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, 4096))),
+                                     iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kVhccPartEntries))),
                    rvvla::upload_vhcc(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
 // VNEC key "threads" (merge-path partitions; default: 2048 rows+entries each)
 RVVLA_CONVERT_BOTH(spMtxVNEC,

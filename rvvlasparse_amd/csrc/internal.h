@@ -59,6 +59,10 @@ void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCR
 int default_sell_sigma(int64_t m, int64_t nnz, const int* rst);
 int default_csr5_sigma(int64_t m, int64_t nnz);
 int default_parts(int64_t m, int64_t nnz, int64_t items_per_part);
+// Default work per wavefront (measured on the 4.2 M-row Laplacian): CVR
+// streams of about 4096 entries, VHCC partitions of about 1024.
+constexpr int64_t kCvrStreamEntries = 4096;
+constexpr int64_t kVhccPartEntries = 1024;
 // Rows [b[t], b[t+1]) of part t: contiguous, about nnz/parts entries each.
 std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts);
 
