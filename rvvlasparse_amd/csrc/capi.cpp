@@ -357,6 +357,31 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
   return M.release();
 }
 
+// Pieces of the slices wider than kSellPiece steps (see DevSELL).
+void attach_sell_pieces(rvvla_mat_s& M, const std::vector<int>& cl) {
+  std::vector<int32_t> pc_slice, pc_j0, pc_j1, sp_slice, sp_first{0};
+  for (size_t i = 0; i < cl.size(); ++i) {
+    if (cl[i] <= kSellPiece) continue;
+    for (int j = 0; j < cl[i]; j += kSellPiece) {
+      pc_slice.push_back((int32_t)i);
+      pc_j0.push_back(j);
+      pc_j1.push_back(std::min(cl[i], j + kSellPiece));
+    }
+    sp_slice.push_back((int32_t)i);
+    sp_first.push_back((int32_t)pc_slice.size());
+  }
+  DevSELL& d = M.sell;
+  d.npieces = (int64_t)pc_slice.size();
+  d.nsplit = (int64_t)sp_slice.size();
+  if (d.nsplit == 0) return;
+  d.pc_slice = M.put_vec(pc_slice);
+  d.pc_j0 = M.put_vec(pc_j0);
+  d.pc_j1 = M.put_vec(pc_j1);
+  d.sp_slice = M.put_vec(sp_slice);
+  d.sp_first = M.put_vec(sp_first);
+  d.part = M.put(nullptr, (size_t)d.npieces * 64 * (M.dtype == RVVLA_F64 ? 8 : 4));
+}
+
 template <class V>
 rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
@@ -399,6 +424,7 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
   d.perm_identity = true;
   for (int64_t s = 0; s < m && d.perm_identity; ++s) d.perm_identity = a.vPerm[(size_t)s] == (uint32_t)s;
   d.val = put_values(*M, dtype, a.vVal.data(), a.vVal.size());
+  attach_sell_pieces(*M, a.vCl);
   return M.release();
 }
 
@@ -1188,6 +1214,13 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
                        ? rvvla::build_sell_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, sigma, s)
                        : rvvla::build_sell_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, sigma, s);
     if (rc) throw Error(rc, rvvla::last_error());
+    if (M->sell.nslices > 0) {  // pieces of the wide slices, from the slice widths
+      std::vector<int> cl((size_t)M->sell.nslices);
+      const int rc2 = rvvla::hip_check(
+          hipMemcpy(cl.data(), M->sell.cl, cl.size() * 4, hipMemcpyDeviceToHost), "read widths");
+      if (rc2) throw Error(rc2, rvvla::last_error());
+      rvvla::attach_sell_pieces(*M, cl);
+    }
     *out = M.release();
     return RVVLA_OK;
   });

@@ -22,17 +22,31 @@ using namespace SparseMatrixLib;
 
 int default_sell_sigma(int64_t m, int64_t nnz, const int* rst) {
   // sigma = 1 keeps the row order (the permutation is the identity, so the
-  // multiply needs no vPerm read); take it when 64-row slices in natural
-  // order pad by at most 1 % of the entries, else sort in windows of 16384
+  // multiply needs no vPerm read): taken when 64-row slices in natural
+  // order pad by at most 1 % of the entries.  Otherwise sorting all rows by
+  // length (sigma = m) when that at least halves the padded slots (matrices
+  // with very uneven rows), else windows of 16384 rows, which keep the y
+  // writes of a slice close together.
   if (m == 0 || nnz == 0 || !rst) return 16384;
-  int64_t padded = 0;
+  int64_t natural = 0;
   for (int64_t r0 = 0; r0 < m; r0 += 64) {
     const int64_t r1 = std::min(m, r0 + 64);
     int64_t w = 0;
     for (int64_t r = r0; r < r1; ++r) w = std::max<int64_t>(w, rst[r + 1] - rst[r]);
-    padded += w * (r1 - r0);
+    natural += w * (r1 - r0);
   }
-  return (double)(padded - nnz) <= 0.01 * (double)nnz ? 1 : 16384;
+  if ((double)(natural - nnz) <= 0.01 * (double)nnz) return 1;
+  return (double)sorted_sell_slots(m, rst) <= 0.5 * (double)natural ? (int)m : 16384;
+}
+
+int64_t sorted_sell_slots(int64_t m, const int* rst) {
+  // slice k of the rows sorted by descending length starts with its widest
+  std::vector<int32_t> len((size_t)m);
+  for (int64_t r = 0; r < m; ++r) len[(size_t)r] = rst[r + 1] - rst[r];
+  std::sort(len.begin(), len.end(), [](int32_t a, int32_t b) { return a > b; });
+  int64_t slots = 0;
+  for (int64_t r0 = 0; r0 < m; r0 += 64) slots += (int64_t)len[(size_t)r0] * std::min<int64_t>(64, m - r0);
+  return slots;
 }
 
 template <class V>

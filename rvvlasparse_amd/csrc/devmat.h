@@ -65,7 +65,17 @@ struct DevSELL {
   const uint32_t* perm = nullptr;
   const void* val = nullptr;
   bool perm_identity = false;  // vPerm[s] == s: rows written in place
+  // slices wider than kSellPiece steps run as pieces of at most kSellPiece
+  // steps, one wavefront each, whose lane partials are added in order
+  int64_t npieces = 0, nsplit = 0;
+  const int32_t* pc_slice = nullptr;
+  const int32_t* pc_j0 = nullptr;   // step range of each piece
+  const int32_t* pc_j1 = nullptr;
+  const int32_t* sp_slice = nullptr;  // the split slices
+  const int32_t* sp_first = nullptr;  // nsplit + 1: their pieces
+  void* part = nullptr;               // npieces * 64 partial sums
 };
+constexpr int kSellPiece = 64;
 
 // CSR5 (omega = 64): tiles as the host container holds them, plus the
 // empty-row list and per-tile scratch (leading / trailing partial sums and
@@ -255,6 +265,9 @@ int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 template <class V>
 int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int sigma, hipStream_t s);
+
+// Pieces of the SELL slices wider than kSellPiece steps, from the widths.
+void attach_sell_pieces(rvvla_mat_s& M, const std::vector<int>& cl);
 
 // The adaptive CSR product on any CSR image of m rows.
 int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,

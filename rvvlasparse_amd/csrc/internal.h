@@ -57,6 +57,8 @@ void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCR
 
 // Default partition counts and parameters chosen for gfx950 (256 CUs).
 int default_sell_sigma(int64_t m, int64_t nnz, const int* rst);
+// SELL-64 slots when all rows are sorted by length (sigma = m).
+int64_t sorted_sell_slots(int64_t m, const int* rst);
 int default_csr5_sigma(int64_t m, int64_t nnz);
 int default_parts(int64_t m, int64_t nnz, int64_t items_per_part);
 // Default work per wavefront (measured on the 4.2 M-row Laplacian): CVR

@@ -16,9 +16,10 @@
 //     Cs + j*Bs + k holds entry j of sorted row k, or value 0 and the row's
 //     last column (column 0 for an empty row) past the row's end.
 // The arrays are bit-identical to the host builder's for the same sigma;
-// with sigma <= 0 the default rule is evaluated on the device (sigma = 1
+// with sigma <= 0 the default rule is evaluated on the device: sigma = 1
 // when 64-row slices in natural order pad by at most 1 % of the entries,
-// else 16384).
+// else sigma = m when sorting all rows by length at least halves the padded
+// slots, else 16384 (the host rule, same figures).
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -87,6 +88,23 @@ __global__ void k_conv_padding(const int32_t* __restrict__ rst, int64_t m,
   for (int off = 32; off > 0; off >>= 1) w = max(w, __shfl_xor(w, off, 64));
   const int height = (int)min<int64_t>(64, m - slice * 64);
   if (lane == 0) atomicAdd(padded, (unsigned long long)w * (unsigned long long)height);
+}
+
+// lengths of all rows, as keys sorting to descending order
+__global__ void k_conv_len_keys(const int32_t* __restrict__ rst, int64_t m,
+                                uint32_t* __restrict__ key) {
+  const int64_t r = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (r < m) key[r] = 0xFFFFFFFFu - (uint32_t)(rst[r + 1] - rst[r]);
+}
+
+// slots of SELL-64 over rows sorted by descending length
+__global__ void k_conv_sorted_slots(const uint32_t* __restrict__ key, int64_t m,
+                                    unsigned long long* __restrict__ slots) {
+  const int64_t k = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  const int64_t r0 = k * 64;
+  if (r0 >= m) return;
+  const unsigned long long w = 0xFFFFFFFFu - key[r0];
+  atomicAdd(slots, w * (unsigned long long)min<int64_t>(64, m - r0));
 }
 
 __global__ void k_conv_slices(const int32_t* __restrict__ rst, const uint32_t* __restrict__ perm,
@@ -178,16 +196,38 @@ int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst
     d.perm_identity = true;
     return RVVLA_OK;
   }
-  // default sigma from the natural-order padding
+  // default sigma: natural-order padding, then the padding of a full sort
   if (sigma <= 0) {
-    unsigned long long* dpad = (unsigned long long*)M.temp(8);
-    if (int rc = hip_check(hipMemsetAsync(dpad, 0, 8, s), "clear")) return rc;
+    unsigned long long* dpad = (unsigned long long*)M.temp(16);
+    if (int rc = hip_check(hipMemsetAsync(dpad, 0, 16, s), "clear")) return rc;
     hipLaunchKernelGGL(k_conv_padding, dim3(g_slices), dim3(kWG), 0, s, rst, m, dpad);
-    unsigned long long padded = 0;
-    if (int rc = hip_check(hipMemcpyAsync(&padded, dpad, 8, hipMemcpyDeviceToHost, s), "padding"))
+    unsigned long long padded[2] = {0, 0};
+    if (int rc = hip_check(hipMemcpyAsync(padded, dpad, 8, hipMemcpyDeviceToHost, s), "padding"))
       return rc;
     if (int rc = hip_check(hipStreamSynchronize(s), "padding")) return rc;
-    sigma = nnz > 0 && (double)((int64_t)padded - nnz) <= 0.01 * (double)nnz ? 1 : 16384;
+    if (nnz > 0 && (double)((int64_t)padded[0] - nnz) <= 0.01 * (double)nnz) {
+      sigma = 1;
+    } else if (nnz == 0) {
+      sigma = 16384;
+    } else {
+      uint32_t* key = (uint32_t*)M.temp(2 * (size_t)m * 4);
+      uint32_t* key_out = key + m;
+      hipLaunchKernelGGL(k_conv_len_keys, dim3(g_rows), dim3(kWG), 0, s, rst, m, key);
+      size_t tmp = 0;
+      if (int rc = hip_check(rocprim::radix_sort_keys(nullptr, tmp, key, key_out, (size_t)m, 0, 32, s),
+                             "radix sort size"))
+        return rc;
+      void* tbuf = M.temp(tmp);
+      if (int rc = hip_check(rocprim::radix_sort_keys(tbuf, tmp, key, key_out, (size_t)m, 0, 32, s),
+                             "radix sort"))
+        return rc;
+      hipLaunchKernelGGL(k_conv_sorted_slots, dim3((unsigned)((nslices + kWG - 1) / kWG)), dim3(kWG), 0, s,
+                         key_out, m, dpad + 1);
+      if (int rc = hip_check(hipMemcpyAsync(&padded[1], dpad + 1, 8, hipMemcpyDeviceToHost, s), "padding"))
+        return rc;
+      if (int rc = hip_check(hipStreamSynchronize(s), "padding")) return rc;
+      sigma = (double)padded[1] <= 0.5 * (double)padded[0] ? (int)m : 16384;
+    }
   }
   uint32_t* perm = (uint32_t*)M.put(nullptr, (size_t)m * 4);
   if (sigma == 1) {

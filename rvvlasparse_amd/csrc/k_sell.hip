@@ -8,7 +8,10 @@
 // path does (reference src/rvv/spmv_mtx_sell_c_sigma.cpp:60-85):
 // y[perm[s]] = alpha * acc + beta * y[perm[s]].  When the permutation is
 // the identity (sigma = 1) it is not read at all.  FP64 values and their
-// columns are read with nontemporal loads.
+// columns are read with nontemporal loads.  Slices wider than kSellPiece
+// steps (very uneven rows) are cut into pieces of kSellPiece steps run by
+// their own wavefronts (k_sell_pieces); k_sell_combine adds a slice's
+// pieces in order and writes its rows.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -29,6 +32,7 @@ __global__ __launch_bounds__(kWG) void k_sell64(
   const int lane = threadIdx.x & 63;
   if (slice >= nslices) return;
   const int height = bs[slice], width = cl[slice];
+  if (width > kSellPiece) return;  // a wide slice: run as pieces
   const int64_t base = cs[slice];
   const int srow = sbs[slice];
   if (lane >= height) return;
@@ -54,6 +58,62 @@ __global__ __launch_bounds__(kWG) void k_sell64(
   y[r] = alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r);
 }
 
+// One piece (steps [j0, j1) of a wide slice) per wavefront: lane partials
+// to part[piece * 64 + lane].
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_sell_pieces(
+    const int32_t* __restrict__ pc_slice, const int32_t* __restrict__ pc_j0,
+    const int32_t* __restrict__ pc_j1, int64_t npieces, const int32_t* __restrict__ cs,
+    const int32_t* __restrict__ bs, const uint32_t* __restrict__ col, const VT* __restrict__ val,
+    const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc* __restrict__ part) {
+  using A = typename Val<VT>::acc;
+  const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (q >= npieces) return;
+  const int lane = threadIdx.x & 63;
+  const int sl = pc_slice[q], height = bs[sl];
+  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  if (lane < height) {
+    int j = pc_j0[q];
+    const int j1 = pc_j1[q];
+    int64_t idx = cs[sl] + (int64_t)j * height + lane;
+    for (; j + 3 < j1; j += 4) {
+      const uint32_t c0 = col[idx], c1 = col[idx + height];
+      const uint32_t c2 = col[idx + 2 * (int64_t)height], c3 = col[idx + 3 * (int64_t)height];
+      const A v0 = Val<VT>::get(val, idx), v1 = Val<VT>::get(val, idx + height);
+      const A v2 = Val<VT>::get(val, idx + 2 * (int64_t)height);
+      const A v3 = Val<VT>::get(val, idx + 3 * (int64_t)height);
+      a0 += v0 * (A)x[c0];
+      a1 += v1 * (A)x[c1];
+      a2 += v2 * (A)x[c2];
+      a3 += v3 * (A)x[c3];
+      idx += 4 * (int64_t)height;
+    }
+    for (; j < j1; ++j, idx += height) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
+  }
+  part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
+}
+
+// One wide slice per wavefront: its pieces' partials in piece order.
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_sell_combine(
+    const int32_t* __restrict__ sp_slice, const int32_t* __restrict__ sp_first, int64_t nsplit,
+    const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
+    const uint32_t* __restrict__ perm, bool ident, const typename Val<VT>::acc* __restrict__ part,
+    typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc alpha,
+    typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (w >= nsplit) return;
+  const int lane = threadIdx.x & 63;
+  const int sl = sp_slice[w];
+  if (lane >= bs[sl]) return;
+  A s = 0;
+  for (int32_t q = sp_first[w]; q < sp_first[w + 1]; ++q) s += part[(int64_t)q * 64 + lane];
+  const int k = sbs[sl] + lane;
+  const uint32_t r = ident ? (uint32_t)k : perm[k];
+  y[r] = alpha * s + scaled_y(beta, y, r);
+}
+
 template <class VT>
 static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                     hipStream_t s) {
@@ -77,6 +137,17 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
     else RVVLA_SELL_LAUNCH(false, false);
   }
 #undef RVVLA_SELL_LAUNCH
+  if (d.nsplit > 0) {
+    if (int rc = hip_check(hipGetLastError(), "k_sell64 launch")) return rc;
+    constexpr int W = kWG / 64;
+    hipLaunchKernelGGL((k_sell_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
+                       0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs, d.col,
+                       (const VT*)d.val, (const IO*)x, (AC*)d.part);
+    if (int rc = hip_check(hipGetLastError(), "k_sell_pieces launch")) return rc;
+    hipLaunchKernelGGL((k_sell_combine<VT>), dim3((unsigned)((d.nsplit + W - 1) / W)), dim3(kWG),
+                       0, s, d.sp_slice, d.sp_first, d.nsplit, d.bs, d.sbs, d.perm,
+                       d.perm_identity, (const AC*)d.part, (IO*)y, (AC)alpha, (AC)beta);
+  }
   return hip_check(hipGetLastError(), "k_sell64 launch");
 }
 

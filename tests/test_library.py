@@ -108,8 +108,22 @@ def test_sell_default_sigma():
     L = R.convert("sell", lap, R.F64)
     assert L.scalar("Sigma") == 1
     assert np.array_equal(L.array("vPerm"), np.arange(lap.m, dtype=np.uint32))
+    # very uneven rows: a full sort by length (sigma = m) at least halves the
+    # padded slots, so it is taken
     pl = R.gen_powerlaw(20000, 1.8, 2000, 200000, 256, seed=1)
-    assert R.convert("sell", pl, R.F64).scalar("Sigma") == 16384
+    L = np.diff(pl.rst)
+    natural = sum(L[r:r + 64].max() * len(L[r:r + 64]) for r in range(0, pl.m, 64))
+    srt = np.sort(L)[::-1]
+    full = sum(srt[r] * min(64, pl.m - r) for r in range(0, pl.m, 64))
+    assert full <= 0.5 * natural
+    assert R.convert("sell", pl, R.F64).scalar("Sigma") == pl.m
+    # moderately uneven rows (lengths 10..13 at random): windows of 16384
+    rng = np.random.default_rng(5)
+    lens = rng.integers(10, 14, 5000)
+    rst = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.arange(l) for l in lens]).astype(np.int32)
+    mod = R.CSR(5000, 100, rst, col, np.ones(rst[-1]))
+    assert R.convert("sell", mod, R.F64).scalar("Sigma") == 16384
     # padding just over / under 1 %: rows of 100 entries, one row per slice longer
     m, n = 64 * 50, 4000
     for extra, want in ((1, 1), (2, 16384)):
