@@ -340,8 +340,35 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
     brow_ptr[(size_t)br + 1] = (int32_t)nb_col.size();
   }
   nb_ptr.push_back((int32_t)nz);
+  // block-rows with many blocks: per-entry local rows and block columns
+  std::vector<uint8_t> flat((size_t)bm, 0);
+  std::vector<uint16_t> ebc16;
+  bool any_flat = false;
+  if (bn <= 65536) {
+    for (int64_t br = 0; br < bm; ++br) {
+      const int32_t k0 = brow_ptr[(size_t)br], k1 = brow_ptr[(size_t)br + 1];
+      if (k1 - k0 <= kHcsrFlat) continue;
+      flat[(size_t)br] = 1;
+      if (!any_flat) ebc16.assign((size_t)nz, 0);
+      any_flat = true;
+      for (int32_t k = k0; k < k1; ++k) {
+        const int32_t e0 = nb_ptr[(size_t)k], e1 = nb_ptr[(size_t)k + 1];
+        for (int32_t e = e0; e < e1; ++e) ebc16[(size_t)e] = (uint16_t)nb_col[(size_t)k];
+        if ((nb_roff[(size_t)k] >> 30) == 1) {  // CSR kind: expand the local row pointers
+          const uint32_t* rp = rp32.data() + (nb_roff[(size_t)k] & 0x3FFFFFFFu);
+          const int64_t rows = std::min<int64_t>(R, m - br * R);
+          for (int64_t r = 0; r < rows; ++r)
+            for (uint32_t e = rp[r]; e < rp[r + 1]; ++e) coo16[(size_t)(e0 + e)] = (uint16_t)r;
+        }
+      }
+    }
+  }
   auto M = new_mat(RVVLA_FMT_HCSR, dtype, m, n, nz);
   DevHCSR& d = M->hcsr;
+  if (any_flat) {
+    d.flat = M->put_vec(flat);
+    d.ebc16 = M->put_vec(ebc16);
+  }
   d.R = R;
   d.C = C;
   d.b_m = bm;

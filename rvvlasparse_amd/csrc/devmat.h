@@ -52,7 +52,13 @@ struct DevHCSR {
   const uint16_t* coo16 = nullptr;    // local row of each COO-kind entry (nnz-sized)
   const uint16_t* col16 = nullptr;
   const void* val = nullptr;
+  // block-rows with more than kHcsrFlat non-empty blocks are walked entry
+  // by entry: coo16 then also holds the local row of their CSR-kind
+  // entries and ebc16 the block column of every entry
+  const uint8_t* flat = nullptr;      // b_m flags (nullptr: none)
+  const uint16_t* ebc16 = nullptr;
 };
+constexpr int kHcsrFlat = 32;
 
 // SELL-C-sigma with C = 64: one wavefront per slice.
 struct DevSELL {

@@ -4,7 +4,9 @@
 // block of block-row br adds alpha * (block . x[bc*C ...]) to y[br*R ...],
 // after y has been scaled by beta.  Here one workgroup owns a block-row:
 // its R partial sums live in LDS, only the non-empty blocks are visited
-// (the reference walks all b_n of them), and y is read and written once:
+// (the reference walks all b_n of them), and y is read and written once
+// (a block-row with more than kHcsrFlat blocks is walked entry by entry,
+// each entry's local row and block column precomputed at upload):
 // y = alpha * acc + beta * y.
 //  * CSR-kind block: thread per local row, local row_ptr from rp32,
 //    16-bit columns relative to the block's first column.
@@ -20,6 +22,7 @@ __global__ __launch_bounds__(kWG) void k_hcsr(
     const int32_t* __restrict__ nb_ptr, const uint32_t* __restrict__ nb_roff,
     const uint32_t* __restrict__ rp32, const uint16_t* __restrict__ coo16,
     const uint16_t* __restrict__ col16, const VT* __restrict__ val,
+    const uint8_t* __restrict__ flat, const uint16_t* __restrict__ ebc16,
     const typename Val<VT>::io* __restrict__ x, typename Val<VT>::io* __restrict__ y, int64_t m,
     int R, int C, typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
@@ -31,6 +34,16 @@ __global__ __launch_bounds__(kWG) void k_hcsr(
   for (int i = tid; i < rows; i += kWG) acc[i] = 0;
   __syncthreads();
   const int k0 = brow_ptr[blockIdx.x], k1 = brow_ptr[blockIdx.x + 1];
+  if (flat && flat[blockIdx.x]) {
+    // many blocks: one pass over the block-row's entries, thread per entry
+    const int E0 = nb_ptr[k0], E1 = nb_ptr[k1];
+    for (int e = E0 + tid; e < E1; e += kWG)
+      atomicAdd(&acc[coo16[e]],
+                Val<VT>::get(val, e) * (A)x[(int64_t)ebc16[e] * C + col16[e]]);
+    __syncthreads();
+    for (int i = tid; i < rows; i += kWG) y[r0 + i] = alpha * acc[i] + scaled_y(beta, y, r0 + i);
+    return;
+  }
   for (int k = k0; k < k1; ++k) {
     const auto* xb = x + (int64_t)nb_col[k] * C;
     const int e0 = nb_ptr[k], e1 = nb_ptr[k + 1];
@@ -61,7 +74,8 @@ static int run_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double be
   if (d.b_m == 0) return RVVLA_OK;
   const size_t lds = (size_t)d.R * sizeof(AC);
   hipLaunchKernelGGL((k_hcsr<VT>), dim3((unsigned)d.b_m), dim3(kWG), lds, s, d.brow_ptr, d.nb_col,
-                     d.nb_ptr, d.nb_roff, d.rp32, d.coo16, d.col16, (const VT*)d.val, (const IO*)x,
+                     d.nb_ptr, d.nb_roff, d.rp32, d.coo16, d.col16, (const VT*)d.val, d.flat,
+                     d.ebc16, (const IO*)x,
                      (IO*)y, A.m, (int)d.R, (int)d.C, (AC)alpha, (AC)beta);
   return hip_check(hipGetLastError(), "k_hcsr launch");
 }

@@ -168,6 +168,21 @@ def test_csr_column_slices_16bit(dtype):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
+@pytest.mark.parametrize("params", ["R=16,C=16,p=2", "R=64,C=32,p=1", "R=300,C=400,p=2"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64"])
+def test_hcsr_block_sizes(params, dtype, name):
+    """HCSR with small blocks (block-rows of more than 32 non-empty blocks
+    take the entry-by-entry path) and larger ones, against the oracle."""
+    a = case(name)
+    x64, y064 = inputs(a, 41)
+    y = run_gpu("hcsr", dtype, a, x64, y064, 1.0, 0.5, params)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x, 1.0, 0.5, y0)
+    assert_within_bound(y, y_ref, a.rst, a.col, v, x, 1.0, 0.5, y0, dt, f"hcsr {params} {name}")
+
+
 @pytest.mark.parametrize("sigma", [4, 8, 12, 16, 32])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
 @pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64", "random16_c1"])

@@ -152,3 +152,12 @@ def test_device_conversion_needs_a_device():
     rc = R.lib.rvvla_sell_from_device(R.F64, a.m, a.n, a.nnz, a.rst.ctypes.data,
                                       a.col.ctypes.data, a.val.ctypes.data, 0, None, C.byref(h))
     assert rc == NO_DEVICE and not h.value
+
+
+@pytest.mark.parametrize("params", ["R=16,C=16,p=2", "R=8,C=24,p=1"])
+def test_hcsr_small_blocks_upload(params):
+    """Block-rows with many blocks get per-entry rows and block columns."""
+    for name in ("rand_long", "one_long_row", "lap", "empty"):
+        a = CASES[name]
+        M = R.convert("hcsr", a, R.F64, params).upload()
+        assert (M.m, M.n, M.nnz) == (a.m, a.n, a.nnz)
