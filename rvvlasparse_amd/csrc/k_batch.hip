@@ -59,27 +59,42 @@ __device__ void batch_csr_rows(const BatchMat& M, int first, int count,
   const T* xp = X + M.x_off + c;
   T* yp = Y + M.y_off + c;
   const int last = min(first + count, M.m);
+  // row bounds of the next step are fetched while this step runs
+  int nb0 = 0, nb1 = 0;
+  if (first + g < last && act) {
+    nb0 = rp[first + g];
+    nb1 = rp[first + g + 1];
+  }
   for (int row0 = first; row0 < last; row0 += 64 / G) {
     const int row = row0 + g;
+    const int b0 = nb0, b1 = nb1;
+    const int nrow = row + 64 / G;
+    if (nrow < last && act) {
+      nb0 = rp[nrow];
+      nb1 = rp[nrow + 1];
+    }
     T a[V], b[V];
 #pragma unroll
     for (int q = 0; q < V; ++q) a[q] = b[q] = T(0);
     if (row < last && act) {
-      const int e1 = rp[row + 1];
-      int e = rp[row] + slot;
-      for (; e + S < e1; e += 2 * S) {
-        const int c0 = cp[e], c1 = cp[e + S];
-        const T v0 = vp[e], v1 = vp[e + S];
-        T x0[V], x1[V];
+      const int e1 = b1;
+      int e = b0 + slot;
+      // four entries in flight per lane: columns and values, then X
+      for (; e + 3 * S < e1; e += 4 * S) {
+        const int c0 = cp[e], c1 = cp[e + S], c2 = cp[e + 2 * S], c3 = cp[e + 3 * S];
+        const T v0 = vp[e], v1 = vp[e + S], v2 = vp[e + 2 * S], v3 = vp[e + 3 * S];
+        T x0[V], x1[V], x2[V], x3[V];
         load_v<T, V>(xp + (int64_t)c0 * k, x0);
         load_v<T, V>(xp + (int64_t)c1 * k, x1);
+        load_v<T, V>(xp + (int64_t)c2 * k, x2);
+        load_v<T, V>(xp + (int64_t)c3 * k, x3);
 #pragma unroll
         for (int q = 0; q < V; ++q) {
-          a[q] += v0 * x0[q];
-          b[q] += v1 * x1[q];
+          a[q] += v0 * x0[q] + v2 * x2[q];
+          b[q] += v1 * x1[q] + v3 * x3[q];
         }
       }
-      if (e < e1) {
+      for (; e < e1; e += S) {
         const T v0 = vp[e];
         T x0[V];
         load_v<T, V>(xp + (int64_t)cp[e] * k, x0);
