@@ -28,6 +28,10 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# thread placement of the host code (the CPU baseline and the converters),
+# read by the OpenMP runtime when the libraries load
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "SpMV GB/s & % HBM peak per format; BF16-val vs FP64 speedup @ stated rel err"
 PEAK_GBS = 8000.0  # MI355X HBM3E, AMD specification
@@ -79,9 +83,12 @@ def copy_probe(torch, nbytes=1 << 30, steps=20):
     return {"bytes_moved": 2 * nbytes, "ms": round(ms, 4), "GBs": round(2 * nbytes / ms / 1e6, 1)}
 
 
-def cpu_baseline(a, x64, byts):
+def cpu_baseline(a, x64, byts, seconds=10.0):
     """Scalar OpenMP CSR restatement of the reference (reference
-    src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores."""
+    src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores: the median
+    of as many full Laplacian SpMVs as fit in `seconds` (at least 20) after 5
+    warm-ups, with every allowed thread; then the same on one thread for
+    about a third of that time."""
     from oracle import oracle as O
     if O.orc is None:
         O.build(ref=False)
@@ -90,22 +97,28 @@ def cpu_baseline(a, x64, byts):
     cores = len(os.sched_getaffinity(0))
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+
+    def sample(threads, budget):
+        O.orc.orc_set_threads(threads)
+        y = np.zeros(a.m)
+        for _ in range(5):
+            O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
+        ts = []
+        t_end = time.perf_counter() + budget
+        while len(ts) < 20 or time.perf_counter() < t_end:
+            t0 = time.perf_counter()
+            O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts), len(ts)
+
+    med, cnt = sample(cores, seconds)
+    med1, cnt1 = sample(1, seconds / 3)
     O.orc.orc_set_threads(cores)
-    y = np.zeros(a.m)
-    for _ in range(5):
-        O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
-    ts = []
-    t_end = time.perf_counter() + 15.0
-    while len(ts) < 20 or (time.perf_counter() < t_end and len(ts) < 200):
-        t0 = time.perf_counter()
-        O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
-        ts.append(time.perf_counter() - t0)
-        if len(ts) >= 20 and time.perf_counter() > t_end:
-            break
-    med = statistics.median(ts)
     return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"full Laplacian FP64 SpMV, median of {len(ts)} runs after 5 warm-ups "
-                      f"({med * 1e3:.3f} ms each)"}
+            "sample": f"full Laplacian FP64 SpMV, median of {cnt} runs after 5 warm-ups "
+                      f"({med * 1e3:.3f} ms each, {cores} threads)",
+            "single_thread": {"value": round(byts / med1 / 1e9, 2), "unit": "GB/s", "cores": 1,
+                              "sample": f"median of {cnt1} runs ({med1 * 1e3:.3f} ms each)"}}
 
 
 def powerlaw_leg(R, torch, steps, warmup, fmts):
