@@ -28,10 +28,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# thread placement of the host code (the CPU baseline and the converters),
-# read by the OpenMP runtime when the libraries load
-os.environ.setdefault("OMP_PROC_BIND", "close")
-os.environ.setdefault("OMP_PLACES", "cores")
+# host threads allowed to this process (no OpenMP thread binding: it would
+# pin the main thread, and with it the GPU runtime's threads, to one core)
+HOST_CPUS = len(os.sched_getaffinity(0))
 
 METRIC = "SpMV GB/s & % HBM peak per format; BF16-val vs FP64 speedup @ stated rel err"
 PEAK_GBS = 8000.0  # MI355X HBM3E, AMD specification
@@ -94,7 +93,7 @@ def cpu_baseline(a, x64, byts, seconds=10.0):
         O.build(ref=False)
         import importlib
         importlib.reload(O)
-    cores = len(os.sched_getaffinity(0))
+    cores = HOST_CPUS
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
 
