@@ -469,6 +469,9 @@ rvvla_mat_t upload_csr5(int dtype, const spMtxCSR5<V>& a) {
   if (a._omega != 64) throw Error(RVVLA_ERR_UNSUPPORTED_OMEGA, "CSR5: omega must be 64");
   if (a._sigma < 1 || a._sigma > 64 || a._tile_desc_len < 1 || a._tile_desc_len > 3)
     throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "CSR5: sigma must be in [1, 64]");
+  if (a._bit_y_offset < 1 || a._bit_seg_offset < 0 || a._bit_y_offset + a._bit_seg_offset > 31 ||
+      a._bit_y_offset + a._bit_seg_offset + a._sigma > 32 * a._tile_desc_len)
+    throw Error(RVVLA_ERR_INVALID_ARG, "CSR5: descriptor bit fields do not fit the descriptor");
   const int64_t m = a._m, n = a._n, nnz = a._nnz, W = 64LL * a._sigma, ntiles = a._p - 1;
   check_csr(m, n, nnz, a._csr_row_pointer.data(), (const int*)a._csr_column_index.data());
   if ((int64_t)a._csr5_tile_pointer.size() != a._p + 1 ||
@@ -500,6 +503,48 @@ rvvla_mat_t upload_csr5(int dtype, const spMtxCSR5<V>& a) {
   d.col = M->put_vec(a._csr_column_index);
   d.val = put_values(*M, dtype, a._csr_value.data(), a._csr_value.size());
   d.empty_rows = M->put_vec(empty);
+  // Where each open segment ends, from the descriptor flags (the same bits
+  // the tile kernel reads): bit 0 of kind = a row starts at the tile's first
+  // entry, bit 1 = a row starts later in the tile.  A segment left open by
+  // tile t runs through the following tiles up to the first one with a row
+  // start, including that tile's leading segment unless the start is its
+  // first entry; with no such tile it runs into the tail.
+  {
+    const int f0 = 32 - a._bit_y_offset - a._bit_seg_offset, dl = a._tile_desc_len, sg = a._sigma;
+    const uint32_t* desc = a._csr5_tile_descriptor.data();
+    std::vector<uint8_t> kind((size_t)d.ntiles);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < d.ntiles; ++t) {
+      bool first = false, later = false;
+      for (int l = 0; l < 64; ++l) {
+        const uint32_t* w = desc + t * 64 * dl + l;
+        for (int i = 0; i < sg; ++i) {
+          const uint32_t bit = i < f0 ? (w[0] >> (f0 - 1 - i)) & 1u
+                                      : (w[64 * (1 + (i - f0) / 32)] >> (31 - (i - f0) % 32)) & 1u;
+          if (!bit) continue;
+          if (l == 0 && i == 0) first = true;
+          else later = true;
+        }
+      }
+      kind[(size_t)t] = (uint8_t)((first ? 1 : 0) | (later ? 2 : 0));
+    }
+    std::vector<int32_t> chain((size_t)d.ntiles), longs;
+    int64_t next = d.ntiles;  // the first tile after t with a row start
+    for (int64_t t = d.ntiles - 1; t >= 0; --t) {
+      const uint32_t end = next == d.ntiles ? ((uint32_t)d.ntiles | 0x80000000u)
+                           : (kind[(size_t)next] & 1) ? (uint32_t)next : (uint32_t)next + 1;
+      chain[(size_t)t] = (int32_t)end;
+      if (kind[(size_t)t] != 0 &&
+          ((end >> 31) || (int64_t)(end & 0x7FFFFFFFu) - t - 1 > kCsr5Serial))
+        longs.push_back((int32_t)t);
+      if (kind[(size_t)t] != 0) next = t;
+    }
+    for (int64_t r = d.first_tail_row; r < m; ++r)
+      if (rst[r + 1] - rst[r] > kCsr5Serial) longs.push_back((int32_t)(-(r + 1)));
+    d.chain = M->put_vec(chain);
+    d.nlong = (int64_t)longs.size();
+    d.long_fix = M->put_vec(longs);
+  }
   const size_t acc = dtype == RVVLA_F64 ? 8 : 4;
   d.scratch = M->put(nullptr, (size_t)d.ntiles * (2 * acc + 4) + 16);
   M->device_bytes -= (int64_t)((size_t)d.ntiles * (2 * acc + 4) + 16);

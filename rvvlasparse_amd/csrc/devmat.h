@@ -72,7 +72,7 @@ struct DevSELL {
   const void* val = nullptr;
   bool perm_identity = false;  // vPerm[s] == s: rows written in place
   // slices wider than kSellPiece steps run as pieces of at most kSellPiece
-  // steps, one wavefront each, whose lane partials are added in order
+  // steps, one wavefront each, whose lane partials are added in a fixed order
   int64_t npieces = 0, nsplit = 0;
   const int32_t* pc_slice = nullptr;
   const int32_t* pc_j0 = nullptr;   // step range of each piece
@@ -98,7 +98,16 @@ struct DevCSR5 {
   const void* val = nullptr;
   const int32_t* empty_rows = nullptr;
   void* scratch = nullptr;
+  // chain[i]: the segment left open at the end of tile i continues through
+  // tiles i+1 .. (chain[i] & 0x7FFFFFFF) - 1; bit 31 set: on into the tail.
+  const int32_t* chain = nullptr;
+  // open segments with more than kCsr5Serial tiles to add, or that run into
+  // the tail (tile index t >= 0), and tail rows of more than kCsr5Serial
+  // entries (-(row + 1)): finished by a wavefront each instead of a thread
+  const int32_t* long_fix = nullptr;
+  int64_t nlong = 0;
 };
+constexpr int kCsr5Serial = 8;
 
 // CVR (SIMD_LEN = 64): one stream per wavefront; records re-laid as a
 // flush mask per pack plus the flushed rows in record order.
@@ -135,7 +144,7 @@ struct DevVHCC {
 // Chunks with more than kLavPiece steps over all segments are cut into
 // pieces of at most kLavPiece steps (within one segment), each run by its
 // own wavefront into a 64-lane partial; the chunk's partials are then added
-// in piece order.
+// in an order fixed by the piece count.
 constexpr int kLavPiece = 64;
 struct DevLAV {
   int64_t nseg = 0, nchunks = 0;

@@ -13,10 +13,12 @@
 //    (before its first counted flag) goes to first_part[t], and the segment
 //    opened by its last flag to last_part[t] / last_row[t].
 //  * k_csr5_fixup (one thread per full tile, then one per tail row, then
-//    one per empty row): the owner of each open segment walks the following
-//    tiles' leading segments until the row ends (adding the tail entries if
-//    it runs into the tail), tail rows are summed as CSR, empty rows get
-//    alpha*0 + beta*y.
+//    one per empty row): the owner of each open segment adds the following
+//    tiles' leading segments up to the tile where the row ends (that range
+//    is fixed by the structure and found at upload), tail rows are summed
+//    as CSR, empty rows get alpha*0 + beta*y.  Segments spanning more than
+//    kCsr5Serial tiles or running into the tail, and tail rows longer than
+//    kCsr5Serial entries, are left to k_csr5_long (one wavefront each).
 #include "kcommon.h"
 
 namespace rvvla {
@@ -29,7 +31,9 @@ struct Csr5Args {
   const int32_t* rst;
   const uint32_t* col;
   const int32_t* empty_rows;
-  int64_t ntiles, nempty, m;
+  const int32_t* chain;
+  const int32_t* long_fix;
+  int64_t ntiles, nempty, m, nlong;
   int sigma, dl, bit_y, bit_seg, tail_start, first_tail_row;
   int64_t tail_base;  // (p-1)*omega*sigma
 };
@@ -196,28 +200,21 @@ __global__ __launch_bounds__(kWG) void k_csr5_fixup(
     typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  const int f0 = 32 - a.bit_y - a.bit_seg;
   const int64_t ntail = a.m - a.first_tail_row;
   if (i < a.ntiles) {
     const int32_t r = last_row[i];
     if (r < 0) return;
+    const uint32_t ch = (uint32_t)a.chain[i];
+    const int64_t q1 = ch & 0x7FFFFFFFu;
+    if ((ch >> 31) || q1 - i - 1 > kCsr5Serial) return;  // k_csr5_long's
     A s = last_part[i];
-    int64_t q = i + 1;
-    for (; q < a.ntiles; ++q) {
-      const uint32_t w0 = a.desc[q * 64 * a.dl];  // lane 0, word 0
-      if ((w0 >> (31 - (32 - f0))) & 1u) break;    // tile q starts a new row
-      s += first_part[q];
-      if (last_row[q] >= 0) break;                 // the row ends inside tile q
-    }
-    if (q == a.ntiles) {  // ran past the full tiles: the row may go on into the tail
-      const int64_t e1 = a.rst[r + 1];
-      for (int64_t e = a.tail_base; e < e1; ++e) s += Val<VT>::get(val, e) * (A)x[a.col[e]];
-    }
+    for (int64_t q = i + 1; q < q1; ++q) s += first_part[q];
     y[r] = alpha * s + scaled_y(beta, y, r);
   } else if (i < a.ntiles + ntail) {
     const int64_t r = a.first_tail_row + (i - a.ntiles);
     const int64_t e0 = a.rst[r], e1 = a.rst[r + 1];
     if (e0 == e1) return;  // empty rows are in the empty list
+    if (e1 - e0 > kCsr5Serial) return;  // k_csr5_long's
     A s = 0;
     for (int64_t e = e0; e < e1; ++e) s += Val<VT>::get(val, e) * (A)x[a.col[e]];
     y[r] = alpha * s + scaled_y(beta, y, r);
@@ -227,14 +224,51 @@ __global__ __launch_bounds__(kWG) void k_csr5_fixup(
   }
 }
 
+// One long segment or long tail row per wavefront: an open segment adds
+// the last part of its tile, the leading parts of the tiles it spans and
+// (if it runs into the tail) its tail entries, lanes striding and then a
+// butterfly; a tail row is summed the same way.
+template <class VT>
+__global__ __launch_bounds__(kWG) void k_csr5_long(
+    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
+    const typename Val<VT>::acc* __restrict__ last_part, const int32_t* __restrict__ last_row,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (w >= a.nlong) return;
+  const int lane = threadIdx.x & 63;
+  const int32_t code = a.long_fix[w];
+  Acc<A, false> s;
+  int64_t r, e0, e1;
+  if (code >= 0) {
+    const int64_t i = code;
+    r = last_row[i];
+    const uint32_t ch = (uint32_t)a.chain[i];
+    const int64_t q1 = ch & 0x7FFFFFFFu;
+    if (lane == 0) s.add(last_part[i]);
+    for (int64_t q = i + 1 + lane; q < q1; q += 64) s.add(first_part[q]);
+    e0 = (ch >> 31) ? a.tail_base : 0;
+    e1 = (ch >> 31) ? (int64_t)a.rst[r + 1] : 0;
+  } else {
+    r = -(int64_t)code - 1;
+    e0 = a.rst[r];
+    e1 = a.rst[r + 1];
+  }
+  for (int64_t e = e0 + lane; e < e1; e += 64) s.add(Val<VT>::get(val, e) * (A)x[a.col[e]]);
+  s = group_reduce(s, 64);
+  if (lane == 0) y[r] = alpha * s.value() + scaled_y(beta, y, r);
+}
+
 template <class VT>
 static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                     hipStream_t s) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevCSR5& d = A.csr5;
-  Csr5Args a{d.tile_ptr, d.desc, d.eptr, d.eoff, d.rst, d.col, d.empty_rows, d.ntiles, d.nempty,
-             A.m, d.sigma, d.dl, d.bit_y, d.bit_seg, d.tail_start, d.first_tail_row, d.tail_base};
+  Csr5Args a{d.tile_ptr, d.desc,   d.eptr,   d.eoff,    d.rst,        d.col,
+             d.empty_rows, d.chain, d.long_fix, d.ntiles, d.nempty, A.m, d.nlong,
+             d.sigma, d.dl, d.bit_y, d.bit_seg, d.tail_start, d.first_tail_row, d.tail_base};
   AC* fp = (AC*)d.scratch;
   AC* lp = fp + d.ntiles;
   int32_t* lr = (int32_t*)(lp + d.ntiles);
@@ -257,7 +291,13 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
     const unsigned g2 = (unsigned)((nfix + kWG - 1) / kWG);
     hipLaunchKernelGGL((k_csr5_fixup<VT>), dim3(g2), dim3(kWG), 0, s, a, (const VT*)d.val,
                        (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
-    return hip_check(hipGetLastError(), "k_csr5_fixup launch");
+    if (int rc = hip_check(hipGetLastError(), "k_csr5_fixup launch")) return rc;
+  }
+  if (d.nlong > 0) {
+    const unsigned g3 = (unsigned)((d.nlong + kWG / 64 - 1) / (kWG / 64));
+    hipLaunchKernelGGL((k_csr5_long<VT>), dim3(g3), dim3(kWG), 0, s, a, (const VT*)d.val,
+                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
+    return hip_check(hipGetLastError(), "k_csr5_long launch");
   }
   return RVVLA_OK;
 }

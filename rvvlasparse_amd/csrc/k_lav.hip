@@ -10,8 +10,8 @@
 // past a step's mask were zeroed at upload, so all 64 lanes load every step
 // (one contiguous access per step) without per-lane predicates.  Chunks
 // longer than kLavPiece steps are cut into pieces (k_lav_pieces) whose
-// partials k_lav_combine adds in order, so no wavefront walks a chunk of
-// thousands of steps alone.
+// partials k_lav_combine adds (a workgroup per chunk), so no wavefront
+// walks a chunk of thousands of steps alone.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -86,20 +86,20 @@ __global__ __launch_bounds__(kWG) void k_lav_pieces(
   part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
 }
 
-// One split chunk per wavefront: its pieces' partials in piece order, then
-// y += alpha * sum for the chunk's rows.
+// One split chunk per workgroup: the sum of its pieces' partials (all four
+// wavefronts load pieces), then the first wavefront adds alpha * sum into
+// the chunk's rows.
 template <class VT>
 __global__ __launch_bounds__(kWG) void k_lav_combine(
-    const int32_t* __restrict__ sp_chunk, const int32_t* __restrict__ sp_first, int64_t nsplit,
+    const int32_t* __restrict__ sp_chunk, const int32_t* __restrict__ sp_first,
     const typename Val<VT>::acc* __restrict__ part, const uint32_t* __restrict__ out_order,
     typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha) {
   using A = typename Val<VT>::acc;
-  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-  if (w >= nsplit) return;
-  const int lane = threadIdx.x & 63;
-  A s = 0;
-  for (int32_t q = sp_first[w]; q < sp_first[w + 1]; ++q) s += part[(int64_t)q * 64 + lane];
-  const int64_t rp = (int64_t)sp_chunk[w] * 64 + lane;
+  __shared__ A lds[kWG];
+  const int64_t w = blockIdx.x;
+  const A s = wg_piece_sum(part, sp_first[w], sp_first[w + 1], lds);
+  if (threadIdx.x >= 64) return;
+  const int64_t rp = (int64_t)sp_chunk[w] * 64 + threadIdx.x;
   if (rp < m) {
     const uint32_t r = out_order[rp];
     y[r] = (A)y[r] + alpha * s;
@@ -126,8 +126,8 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
                        0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
                        (AC*)d.part);
     if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
-    hipLaunchKernelGGL((k_lav_combine<VT>), dim3((unsigned)((d.nsplit + W - 1) / W)), dim3(kWG),
-                       0, s, d.sp_chunk, d.sp_first, d.nsplit, (const AC*)d.part, d.out_order,
+    hipLaunchKernelGGL((k_lav_combine<VT>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+                       d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order,
                        (IO*)y, A.m, (AC)alpha);
     if (int rc = hip_check(hipGetLastError(), "k_lav_combine launch")) return rc;
   }

@@ -11,7 +11,8 @@
 // columns are read with nontemporal loads.  Slices wider than kSellPiece
 // steps (very uneven rows) are cut into pieces of kSellPiece steps run by
 // their own wavefronts (k_sell_pieces); k_sell_combine adds a slice's
-// pieces in order and writes its rows.
+// pieces (a workgroup per slice, in an order fixed by the piece count)
+// and writes its rows.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -93,22 +94,23 @@ __global__ __launch_bounds__(kWG) void k_sell_pieces(
   part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
 }
 
-// One wide slice per wavefront: its pieces' partials in piece order.
+// One wide slice per workgroup: the sum of its pieces' partials (all four
+// wavefronts load pieces; a slice of the longest rows has hundreds), then
+// the first wavefront writes the slice's rows.
 template <class VT>
 __global__ __launch_bounds__(kWG) void k_sell_combine(
-    const int32_t* __restrict__ sp_slice, const int32_t* __restrict__ sp_first, int64_t nsplit,
+    const int32_t* __restrict__ sp_slice, const int32_t* __restrict__ sp_first,
     const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
     const uint32_t* __restrict__ perm, bool ident, const typename Val<VT>::acc* __restrict__ part,
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-  if (w >= nsplit) return;
-  const int lane = threadIdx.x & 63;
+  __shared__ A lds[kWG];
+  const int64_t w = blockIdx.x;
   const int sl = sp_slice[w];
-  if (lane >= bs[sl]) return;
-  A s = 0;
-  for (int32_t q = sp_first[w]; q < sp_first[w + 1]; ++q) s += part[(int64_t)q * 64 + lane];
+  const A s = wg_piece_sum(part, sp_first[w], sp_first[w + 1], lds);
+  const int lane = threadIdx.x;
+  if (lane >= bs[sl]) return;  // rows past the slice's height, and wavefronts 1..3
   const int k = sbs[sl] + lane;
   const uint32_t r = ident ? (uint32_t)k : perm[k];
   y[r] = alpha * s + scaled_y(beta, y, r);
@@ -144,8 +146,8 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
                        0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs, d.col,
                        (const VT*)d.val, (const IO*)x, (AC*)d.part);
     if (int rc = hip_check(hipGetLastError(), "k_sell_pieces launch")) return rc;
-    hipLaunchKernelGGL((k_sell_combine<VT>), dim3((unsigned)((d.nsplit + W - 1) / W)), dim3(kWG),
-                       0, s, d.sp_slice, d.sp_first, d.nsplit, d.bs, d.sbs, d.perm,
+    hipLaunchKernelGGL((k_sell_combine<VT>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+                       d.sp_slice, d.sp_first, d.bs, d.sbs, d.perm,
                        d.perm_identity, (const AC*)d.part, (IO*)y, (AC)alpha, (AC)beta);
   }
   return hip_check(hipGetLastError(), "k_sell64 launch");

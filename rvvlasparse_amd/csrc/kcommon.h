@@ -133,6 +133,34 @@ __device__ __forceinline__ T shfl_xor(T v, int m) {
   return __shfl_xor(v, m, 64);
 }
 
+// Sum of part[q * 64 + lane] over the pieces q in [q0, q1), computed by a
+// whole workgroup: wavefront w adds pieces q0 + w, q0 + w + 4, ... with
+// eight loads in flight, and the four wavefront sums are added in
+// wavefront order through LDS, so the order of the additions is fixed by
+// the piece count alone.  Every thread of the workgroup must call it; each
+// gets the sum for its lane.  lds holds kWG values.
+template <class A>
+__device__ __forceinline__ A wg_piece_sum(const A* __restrict__ part, int64_t q0, int64_t q1,
+                                          A* lds) {
+  constexpr int W = kWG / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  A a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = A(0);
+  int64_t q = q0 + wv;
+  for (; q + 7 * W < q1; q += 8 * W) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] += part[(q + i * W) * 64 + lane];
+  }
+  for (; q < q1; q += W) a[0] += part[q * 64 + lane];
+  lds[wv * 64 + lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  __syncthreads();
+  A s = lds[lane];
+#pragma unroll
+  for (int w = 1; w < W; ++w) s += lds[w * 64 + lane];
+  return s;
+}
+
 // Sum over aligned groups of `width` lanes (power of two <= 64); every lane
 // of the wavefront must call it.
 template <class T, bool COMP>

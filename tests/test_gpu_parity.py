@@ -205,6 +205,36 @@ def test_csr5_sigmas(sigma, dtype, name):
                         f"csr5 sigma={sigma} {name}")
 
 
+@pytest.mark.parametrize("sigma", [4, 8, 16])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_csr5_long_segments(sigma, dtype):
+    """Rows spanning many CSR5 tiles: one ending exactly at a tile boundary,
+    ones ending inside a later tile, one running from the full tiles into
+    the tail, and tail rows on either side of the per-thread limit, with
+    empty rows between them (the segment ranges are found at upload and the
+    long ones are summed by a wavefront each)."""
+    T = 64 * sigma
+    lens = [3, 10 * T - 3, 5, 9 * T + 7, 0, 0, 12, 1, 9, 0, 3 * T + 1, 17 * T + 11, 0]
+    # the last row starts in the full tiles and ends in the tail, then tail
+    # rows of 1, 8, 9 and 40 entries (the tail holds fewer than T entries)
+    head = sum(lens)
+    lens.append(T - head % T + T // 2)
+    lens += [1, 0, 8, 9, 40]
+    assert sum(lens) % T != 0
+    rng = np.random.default_rng(sigma)
+    n = 40000
+    rst = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(n, l, replace=False)) for l in lens]).astype(np.int32)
+    a = R.CSR(len(lens), n, rst, col, rng.normal(size=int(rst[-1])))
+    x64, y064 = inputs(a, 7)
+    y = run_gpu("csr5", dtype, a, x64, y064, 1.25, 0.5, f"sigma={sigma}")
+    dt = np.float64 if dtype == R.F64 else np.float32
+    v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x, 1.25, 0.5, y0)
+    assert_within_bound(y, y_ref, a.rst, a.col, v, x, 1.25, 0.5, y0, dt,
+                        f"csr5 long segments sigma={sigma}")
+
+
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_direct_uploads_match_converted(dtype):
     """rvvla_csr_upload / rvvla_hcsr_upload / rvvla_sell_upload (reference
