@@ -216,23 +216,28 @@ int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy
  * No reference counterpart.  `count` CSR matrices (host arrays; values
  * double for F64, float for F32), matrix i with k[i] in 1..16 right-hand
  * sides, become one device work queue.  The batch owns row-major X_i
- * (n_i x k_i) and Y_i (m_i x k_i) blocks in HBM (rvvla_batch_block gives
- * their device addresses).  rvvla_batch_run computes every
- * Y_i = alpha A_i X_i + beta Y_i with one persistent kernel (plus a one-word
- * queue reset); rvvla_batch_capture records that pair into a hipGraph once
- * and rvvla_batch_replay launches it.  params: "tune=0|1" (default 1: the
+ * (n_i x k_i) and Y_i (m_i x k_i) blocks in HBM with row stride ld_i
+ * (rvvla_batch_block gives their device addresses and ld_i: k_i rounded up
+ * to a multiple of 4 (FP64: 2) when k_i > 2, so that each lane reads 16
+ * bytes of X_i at once; columns k_i .. ld_i-1 are padding, zero after
+ * creation, never written, and do not affect Y_i).  rvvla_batch_run
+ * computes every Y_i = alpha A_i X_i + beta Y_i with one persistent kernel
+ * (plus a one-word queue reset); rvvla_batch_capture records that pair into
+ * a hipGraph once and rvvla_batch_replay launches it.  params: "tune=0|1" (default 1: the
  * autotuner times the candidate item kernels on the first matrix of each
  * row-length-histogram key), "format=0|1,param=P" to force CSR rows (P rows
  * per item) or SELL-64 (sigma P), "cache=<path>" to persist the tuner's
- * choices. */
+ * choices, "wave_items=W" (default 128) work items per wavefront of the
+ * persistent grid. */
 typedef struct rvvla_batch_s* rvvla_batch_t;
 int rvvla_batch_create(int dtype, int count, const int64_t* m, const int64_t* n,
                        const int32_t* const* rst, const int32_t* const* col,
                        const void* const* val, const int* k, const char* params,
                        rvvla_batch_t* out);
-/* Device addresses of X_i and Y_i, and the item kernel chosen for matrix i
- * (RVVLA_FMT_CSR or RVVLA_FMT_SELL). */
-int rvvla_batch_block(rvvla_batch_t B, int i, void** x, void** y, int* format);
+/* Device addresses of X_i and Y_i, their row stride ld_i (elements), and
+ * the item kernel chosen for matrix i (RVVLA_FMT_CSR or RVVLA_FMT_SELL).
+ * Any output pointer may be NULL. */
+int rvvla_batch_block(rvvla_batch_t B, int i, void** x, void** y, int64_t* ld, int* format);
 int rvvla_batch_tune_key(rvvla_batch_t B, int i, char* buf, int len);
 int rvvla_batch_run(rvvla_batch_t B, double alpha, double beta, void* stream);
 int rvvla_batch_capture(rvvla_batch_t B, double alpha, double beta);

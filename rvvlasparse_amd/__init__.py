@@ -312,7 +312,8 @@ def convert(fmt: str | int, a: CSR, dtype: int = F64, params: str = "") -> Layou
 
 for _name, (_res, _args) in {
     "rvvla_batch_create": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _cp, C.POINTER(_vp)]),
-    "rvvla_batch_block": (_i32, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i32)]),
+    "rvvla_batch_block": (_i32, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i64),
+                                 C.POINTER(_i32)]),
     "rvvla_batch_tune_key": (_i32, [_vp, _i32, _cp, _i32]),
     "rvvla_batch_run": (_i32, [_vp, _d, _d, _vp]),
     "rvvla_batch_capture": (_i32, [_vp, _d, _d]),
@@ -326,7 +327,8 @@ for _name, (_res, _args) in {
 class Batch:
     """Many CSR matrices (each with its own 1..16 right-hand sides) multiplied
     in one persistent launch (rvvla_batch_*).  X_i and Y_i live in the
-    batch's device buffers; ``x(i)`` / ``y(i)`` return device addresses."""
+    batch's device buffers, row-major with row stride ld_i >= k_i;
+    ``block(i)`` returns their device addresses and ld_i."""
 
     def __init__(self, mats: list, ks: list, dtype: int = F32, params: str = ""):
         cnt = len(mats)
@@ -345,9 +347,10 @@ class Batch:
         self._h = h
 
     def block(self, i: int):
-        x, y, f = C.c_void_p(), C.c_void_p(), C.c_int()
-        check(lib.rvvla_batch_block(self._h, i, C.byref(x), C.byref(y), C.byref(f)))
-        return x.value, y.value, f.value
+        """(X_i address, Y_i address, item kernel format, row stride ld_i)."""
+        x, y, f, ld = C.c_void_p(), C.c_void_p(), C.c_int(), C.c_int64()
+        check(lib.rvvla_batch_block(self._h, i, C.byref(x), C.byref(y), C.byref(ld), C.byref(f)))
+        return x.value, y.value, f.value, ld.value
 
     def tune_key(self, i: int) -> str:
         buf = C.create_string_buffer(512)

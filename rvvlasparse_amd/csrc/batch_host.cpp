@@ -12,9 +12,11 @@
 // 1 or 4096.
 //
 // Work items: after all matrices are laid out, each matrix is cut into runs
-// of rows (CSR) or slices (SELL) of about equal cost, about 8 items per
-// wavefront of the persistent grid, so the queue costs few atomics and the
-// last items finish together.
+// of rows (CSR) or slices (SELL) of about equal cost, about 128 items per
+// wavefront of the persistent grid (parameter wave_items): few enough that
+// the queue costs little in atomics, enough that the waves of an XCD work
+// on about one matrix at a time (whose X then stays in its L2 when it fits)
+// and that the last items finish together.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -137,6 +139,7 @@ struct Builder {
   std::vector<BatchMat> mats;
   std::vector<BatchItem> items;
   int64_t x_elems = 0, y_elems = 0;
+  int wave_items = 128;  // work items per wavefront of the persistent grid
 
   void add(const HostMat<V>& h, const Choice& c) {
     BatchMat M;
@@ -144,15 +147,20 @@ struct Builder {
     M.n = (int32_t)h.n;
     M.k = h.k;
     M.fmt = c.fmt;
-    M.vw = vec_width(h.k, sizeof(V) == 8 ? 2 : 4);
-    M.lpe = pow2_ceil((h.k + M.vw - 1) / M.vw);
+    // rows of X_i and Y_i padded to whole vector loads (16 bytes) when k is
+    // not a multiple of them: one lane then reads 4 (FP64: 2) right-hand
+    // sides at once instead of one
+    const int vmax = sizeof(V) == 8 ? 2 : 4;
+    M.ld = h.k <= 2 ? h.k : (h.k + vmax - 1) / vmax * vmax;
+    M.vw = vec_width(M.ld, vmax);
+    M.lpe = pow2_ceil(M.ld / M.vw);
     // X and Y blocks start on 4-element boundaries (vector loads)
     x_elems = (x_elems + 3) & ~int64_t(3);
     y_elems = (y_elems + 3) & ~int64_t(3);
     M.x_off = x_elems;
     M.y_off = y_elems;
-    x_elems += h.n * h.k;
-    y_elems += h.m * h.k;
+    x_elems += h.n * M.ld;
+    y_elems += h.m * M.ld;
     if (c.fmt == 0) {
       const int64_t nnz = h.rst[h.m];
       int slots = c.param;
@@ -207,7 +215,7 @@ struct Builder {
       for (int64_t u = 0; u < units; ++u) cost[i] += unit_cost(M, u);
       total += cost[i];
     }
-    const double target = std::max(16.0, total / (8.0 * std::max(1, waves)));
+    const double target = std::max(16.0, total / ((double)wave_items * std::max(1, waves)));
     std::vector<size_t> order(mats.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
@@ -331,6 +339,9 @@ rvvla_batch_t create(int dtype, int count, const int64_t* m, const int64_t* n,
   if (!cache_path.empty()) load_cache(cache_path);
   auto B = std::make_unique<rvvla_batch_s>();
   Builder<V> b;
+  b.wave_items = rvvla::param_int(P, "wave_items", 128);
+  if (b.wave_items < 1 || b.wave_items > 1024)
+    throw Error(RVVLA_ERR_INVALID_ARG, "batch: wave_items must be in [1, 1024]");
   for (int i = 0; i < count; ++i) {
     const HostMat<V>& h = hm[(size_t)i];
     const std::string key = tune_key(h.m, h.rst, h.k);
@@ -401,12 +412,13 @@ int rvvla_batch_create(int dtype, int count, const int64_t* m, const int64_t* n,
   }
 }
 
-int rvvla_batch_block(rvvla_batch_t B, int i, void** x, void** y, int* format) {
+int rvvla_batch_block(rvvla_batch_t B, int i, void** x, void** y, int64_t* ld, int* format) {
   if (!B || i < 0 || i >= B->count) return RVVLA_ERR_INVALID_ARG;
   const size_t es = B->dtype == RVVLA_F64 ? 8 : 4;
   const BatchMat& M = B->mats[(size_t)i];
   if (x) *x = (char*)B->X + M.x_off * es;
   if (y) *y = (char*)B->Y + M.y_off * es;
+  if (ld) *ld = M.ld;
   if (format) *format = M.fmt == 0 ? RVVLA_FMT_CSR : RVVLA_FMT_SELL;
   return RVVLA_OK;
 }

@@ -196,10 +196,10 @@ struct BatchMat {
   int64_t row_off = 0;   // into the concatenated SELL permutations
   int64_t x_off = 0, y_off = 0;  // multiples of 4 elements
   int32_t m = 0, n = 0, k = 0, fmt = 0;
-  int32_t vw = 1;   // right-hand sides per lane: a power of two (<= 4) dividing k
-  int32_t lpe = 1;  // lanes per entry: power of two >= k / vw
+  int32_t vw = 1;   // right-hand sides per lane: a power of two (<= 4) dividing ld
+  int32_t lpe = 1;  // lanes per entry: power of two >= ld / vw
   int32_t gw = 1;   // lanes per row in the CSR item kernel: lpe * slots, <= 64
-  int32_t pad = 0;
+  int32_t ld = 1;   // row stride of X_i and Y_i: k rounded up to a multiple of vw
 };
 struct BatchItem {
   int32_t mat, first, count, pad;

@@ -18,7 +18,10 @@
 //    or narrower load of X per entry), and the gw/lpe slots stride over the
 //    row's entries; the slots are combined by a butterfly.
 //  * SELL item: lane per row of a 64-row slice, k accumulators per lane.
-// Y = alpha * A X + beta * Y, X n x k and Y m x k row-major per matrix.
+// Y = alpha * A X + beta * Y, X n x k and Y m x k row-major per matrix with
+// row stride ld (k rounded up to whole vector loads).  The padding columns
+// of X are read (they hold zeros unless the caller wrote there) but only
+// feed lanes whose results are dropped; those of Y are never written.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -50,7 +53,7 @@ __device__ void batch_csr_rows(const BatchMat& M, int first, int count,
                                const T* __restrict__ val, const T* __restrict__ X,
                                T* __restrict__ Y, T alpha, T beta) {
   const int lane = threadIdx.x & 63;
-  const int k = M.k, lpe = M.lpe, G = M.gw, S = G / lpe;
+  const int k = M.k, ld = M.ld, lpe = M.lpe, G = M.gw, S = G / lpe;
   const int g = lane / G, t = lane & (G - 1), slot = t / lpe, c = (t & (lpe - 1)) * V;
   const bool act = c < k;
   const int32_t* rp = rst + M.rst_off;
@@ -84,10 +87,10 @@ __device__ void batch_csr_rows(const BatchMat& M, int first, int count,
         const int c0 = cp[e], c1 = cp[e + S], c2 = cp[e + 2 * S], c3 = cp[e + 3 * S];
         const T v0 = vp[e], v1 = vp[e + S], v2 = vp[e + 2 * S], v3 = vp[e + 3 * S];
         T x0[V], x1[V], x2[V], x3[V];
-        load_v<T, V>(xp + (int64_t)c0 * k, x0);
-        load_v<T, V>(xp + (int64_t)c1 * k, x1);
-        load_v<T, V>(xp + (int64_t)c2 * k, x2);
-        load_v<T, V>(xp + (int64_t)c3 * k, x3);
+        load_v<T, V>(xp + (int64_t)c0 * ld, x0);
+        load_v<T, V>(xp + (int64_t)c1 * ld, x1);
+        load_v<T, V>(xp + (int64_t)c2 * ld, x2);
+        load_v<T, V>(xp + (int64_t)c3 * ld, x3);
 #pragma unroll
         for (int q = 0; q < V; ++q) {
           a[q] += v0 * x0[q] + v2 * x2[q];
@@ -97,7 +100,7 @@ __device__ void batch_csr_rows(const BatchMat& M, int first, int count,
       for (; e < e1; e += S) {
         const T v0 = vp[e];
         T x0[V];
-        load_v<T, V>(xp + (int64_t)cp[e] * k, x0);
+        load_v<T, V>(xp + (int64_t)cp[e] * ld, x0);
 #pragma unroll
         for (int q = 0; q < V; ++q) a[q] += v0 * x0[q];
       }
@@ -108,9 +111,10 @@ __device__ void batch_csr_rows(const BatchMat& M, int first, int count,
 #pragma unroll
       for (int q = 0; q < V; ++q) a[q] += __shfl_xor(a[q], off, 64);
     if (row < last && act && slot == 0) {
-      T* yo = yp + (int64_t)row * k;
+      T* yo = yp + (int64_t)row * ld;
 #pragma unroll
-      for (int q = 0; q < V; ++q) yo[q] = alpha * a[q] + scaled_y(beta, yo, q);
+      for (int q = 0; q < V; ++q)
+        if (c + q < k) yo[q] = alpha * a[q] + scaled_y(beta, yo, q);  // not the padding
     }
   }
 }
@@ -122,7 +126,7 @@ __device__ void batch_sell_slices(const BatchMat& M, int first, int count,
                                   const int32_t* __restrict__ perm, const T* __restrict__ X,
                                   T* __restrict__ Y, T alpha, T beta) {
   const int lane = threadIdx.x & 63;
-  const int k = M.k;
+  const int k = M.k, ld = M.ld;
   const T* xp = X + M.x_off;
   T* yp = Y + M.y_off;
   const int nsl = (M.m + 63) / 64;
@@ -137,7 +141,7 @@ __device__ void batch_sell_slices(const BatchMat& M, int first, int count,
     for (int q = 0; q < 16; ++q) acc[q] = 0;
     for (int j = 0; j < width; ++j, idx += height) {
       const T v = val[idx];
-      const T* xr = xp + (int64_t)col[idx] * k;
+      const T* xr = xp + (int64_t)col[idx] * ld;
 #pragma unroll
       for (int q = 0; q < 16; ++q)
         if (q < k) acc[q] += v * xr[q];
@@ -146,7 +150,7 @@ __device__ void batch_sell_slices(const BatchMat& M, int first, int count,
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (q < k) {
-        T* yo = yp + (int64_t)row * k + q;
+        T* yo = yp + (int64_t)row * ld + q;
         *yo = alpha * acc[q] + scaled_y(beta, yo, 0);
       }
   }

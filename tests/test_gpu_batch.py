@@ -23,6 +23,12 @@ def _batch_inputs(count=24, seed=0):
     return mats, ks
 
 
+def _expected_ld(k, dtype):
+    """Row stride of X_i / Y_i: k, rounded up to whole 16-byte loads when k > 2."""
+    v = 2 if dtype == R.F64 else 4
+    return k if k <= 2 else -(-k // v) * v
+
+
 def _run_and_check(B, mats, ks, dtype, alpha, beta, seed=1):
     dev = torch.device("cuda:0")
     tdt = torch.float64 if dtype == R.F64 else torch.float32
@@ -30,20 +36,21 @@ def _run_and_check(B, mats, ks, dtype, alpha, beta, seed=1):
     rng = np.random.default_rng(seed)
     Xs, Y0s = [], []
     for i, (a, k) in enumerate(zip(mats, ks)):
-        xp, yp, _ = B.block(i)
+        xp, yp, _, ld = B.block(i)
+        assert ld == _expected_ld(k, dtype)
         X = rng.uniform(-1, 1, (a.n, k)).astype(dt)
         Y0 = rng.uniform(-1, 1, (a.m, k)).astype(dt)
         torch.cuda.synchronize()
-        _copy_to(xp, X, tdt, dev)
-        _copy_to(yp, Y0, tdt, dev)
+        _copy_to(xp, X, ld, tdt, dev)
+        _copy_to(yp, Y0, ld, tdt, dev)
         Xs.append(X)
         Y0s.append(Y0)
     torch.cuda.synchronize()
     B.run(alpha, beta, torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     for i, (a, k) in enumerate(zip(mats, ks)):
-        _, yp, _ = B.block(i)
-        Y = _copy_from(yp, (a.m, k), tdt)
+        _, yp, _, ld = B.block(i)
+        Y = _copy_from(yp, (a.m, k), ld, tdt)
         v = a.val.astype(dt)
         for j in range(k):
             y_ref = O.csr_spmv(a.rst, a.col, v, Xs[i][:, j], alpha, beta, Y0s[i][:, j])
@@ -52,33 +59,32 @@ def _run_and_check(B, mats, ks, dtype, alpha, beta, seed=1):
     return Xs
 
 
-def _copy_to(ptr, arr, tdt, dev):
+def _copy_to(ptr, arr, ld, tdt, dev):
     t = torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
-    import ctypes
-    n = t.numel() * t.element_size()
     torch.cuda.synchronize()
     # device-to-device copy into the batch's buffer through a torch view of it
-    dst = _view(ptr, arr.shape, tdt)
-    dst.copy_(t)
+    _view(ptr, arr.shape, ld, tdt).copy_(t)
 
 
-def _view(ptr, shape, tdt):
-    """A torch tensor aliasing device memory at ptr (no ownership)."""
-    import ctypes
-    numel = int(np.prod(shape))
+def _view(ptr, shape, ld, tdt):
+    """A torch tensor aliasing the rows x cols block at ptr whose rows are ld
+    elements apart (no ownership)."""
+    rows, cols = shape
 
     class _Holder:
         pass
 
     h = _Holder()
+    es = 8 if tdt == torch.float64 else 4
     h.__cuda_array_interface__ = {
-        "shape": tuple(shape), "typestr": "<f8" if tdt == torch.float64 else "<f4",
+        "shape": (rows, cols), "strides": (ld * es, es),
+        "typestr": "<f8" if tdt == torch.float64 else "<f4",
         "data": (int(ptr), False), "version": 2}
     return torch.as_tensor(h, device="cuda")
 
 
-def _copy_from(ptr, shape, tdt):
-    return _view(ptr, shape, tdt).cpu().numpy()
+def _copy_from(ptr, shape, ld, tdt):
+    return _view(ptr, shape, ld, tdt).cpu().numpy()
 
 
 @pytest.mark.parametrize("dtype", [R.F32, R.F64])
@@ -112,7 +118,8 @@ def test_batch_graph_replays():
     mats, ks = _batch_inputs(10, seed=9)
     B = R.Batch(mats, ks, R.F32)
     Xs = _run_and_check(B, mats, ks, R.F32, 1.0, 0.0)
-    eager = [_copy_from(B.block(i)[1], (a.m, k), torch.float32) for i, (a, k) in enumerate(zip(mats, ks))]
+    eager = [_copy_from(B.block(i)[1], (a.m, k), B.block(i)[3], torch.float32)
+             for i, (a, k) in enumerate(zip(mats, ks))]
     B.capture(1.0, 0.0)
     s = torch.cuda.current_stream().cuda_stream
     for rep in range(4):
@@ -120,11 +127,34 @@ def test_batch_graph_replays():
         junk = torch.randn(1 << 20, device="cuda").sum()
         torch.cuda.synchronize()
         for i, (a, k) in enumerate(zip(mats, ks)):
-            _view(B.block(i)[1], (a.m, k), torch.float32).fill_(float(rep + 7))
+            _view(B.block(i)[1], (a.m, k), B.block(i)[3], torch.float32).fill_(float(rep + 7))
         torch.cuda.synchronize()
         B.replay(s)
         torch.cuda.synchronize()
         for i, (a, k) in enumerate(zip(mats, ks)):
-            got = _copy_from(B.block(i)[1], (a.m, k), torch.float32)
+            got = _copy_from(B.block(i)[1], (a.m, k), B.block(i)[3], torch.float32)
             np.testing.assert_array_equal(got, eager[i])
         del junk
+
+
+@pytest.mark.parametrize("dtype", [R.F32, R.F64])
+def test_batch_padding_columns(dtype):
+    """NaN in the padding columns of X_i and Y_i: Y_i's real columns match
+    the oracle (the padding feeds only dropped lanes, and beta never reads
+    Y's padding) and Y's padding is left as it was."""
+    mats, ks = _batch_inputs(8, seed=12)
+    ks = [3, 5, 7, 9, 11, 13, 15, 6]
+    B = R.Batch(mats, ks, dtype, "format=0,param=0")
+    tdt = torch.float64 if dtype == R.F64 else torch.float32
+    for i, (a, k) in enumerate(zip(mats, ks)):
+        xp, yp, _, ld = B.block(i)
+        if ld > k:
+            _view(xp, (a.n, ld), ld, tdt)[:, k:].fill_(float("nan"))
+            _view(yp, (a.m, ld), ld, tdt)[:, k:].fill_(float("nan"))
+    torch.cuda.synchronize()
+    _run_and_check(B, mats, ks, dtype, 1.0, 0.5, seed=4)
+    for i, (a, k) in enumerate(zip(mats, ks)):
+        _, yp, _, ld = B.block(i)
+        if ld > k:
+            pad = _copy_from(yp, (a.m, ld), ld, tdt)[:, k:]
+            assert np.isnan(pad).all(), f"batch mat {i}: Y padding written"
