@@ -758,6 +758,11 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   DevLAV& d = M->lav;
   d.nseg = S;
   d.nchunks = cc;
+  {
+    int64_t longest = 0;
+    for (int64_t r = 0; r < m; ++r) longest = std::max<int64_t>(longest, sp.Rst[r + 1] - sp.Rst[r]);
+    d.fused = S > 0 && longest <= kLavFuseRow;
+  }
   d.coff = M->put_vec(a.chunk_offsets);
   d.sptr = M->put_vec(sptr);
   d.out_order = (const uint32_t*)M->put(S > 0 ? a.out_order.data() : nullptr,
@@ -812,6 +817,7 @@ rvvla_mat_t upload_vnec(int dtype, const spMtxVNEC<V>& a, const spMtxCRS<V>& src
       (int64_t)a.nec_ptr.size() != P + 1 || (int64_t)a.ecr_indices.size() != nnz ||
       (int64_t)a.use_x_indices.size() != a.nec_ptr[(size_t)P] || a.m != m || a.nnz != nnz)
     throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: array sizes do not match");
+  const int64_t items = vnec_items(dtype == RVVLA_F64 ? 8 : 4);  // LDS staging per partition
   std::vector<int32_t> cx((size_t)P + 1), cy((size_t)P + 1);
   for (int64_t t = 0; t < P; ++t) {
     cx[(size_t)t] = a.thread_coord_start[(size_t)t].x;
@@ -828,7 +834,7 @@ rvvla_mat_t upload_vnec(int dtype, const spMtxVNEC<V>& a, const spMtxCRS<V>& src
   for (int64_t t = 0; t < P; ++t) {
     const int64_t x0 = cx[(size_t)t], x1 = cx[(size_t)t + 1], y0 = cy[(size_t)t], y1 = cy[(size_t)t + 1];
     const int64_t nec = a.nec_ptr[(size_t)t + 1] - a.nec_ptr[(size_t)t];
-    if (x1 < x0 || y1 < y0 || y1 - y0 > kVnecItems || nec < 0 || nec > kVnecItems ||
+    if (x1 < x0 || y1 < y0 || y1 - y0 > items || nec < 0 || nec > items ||
         x1 - x0 + 1 > (int64_t)1 << 30)
       throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "VNEC: a partition exceeds the LDS staging size");
     // the merge path: rows before x0 end at or before y0, row x0 has not ended
@@ -1045,7 +1051,8 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
     case RVVLA_FMT_VNEC: {
       auto c = std::make_shared<spMtxVNEC<V>>();
       const int64_t items = (int64_t)crs->m + (int64_t)crs->nz;
-      const int dflt = (int)std::max<int64_t>(1, (items + kVnecItems - 1) / kVnecItems);
+      const int64_t cap = vnec_items((int)sizeof(V));
+      const int dflt = (int)std::max<int64_t>(1, (items + cap - 1) / cap);
       build_vnec<V>(*c, *crs, param_int(P, "threads", dflt));
       L.obj = c;
       reg(L, "ecr_indices", c->ecr_indices);

@@ -205,6 +205,12 @@ void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>&
     d.device.reset(UPLOAD);                                                                        \
   }
 
+// merge-path partitions of at most one workgroup's LDS staging each
+static int vnec_parts(int64_t items, size_t acc_bytes) {
+  const int64_t cap = rvvla::vnec_items((int)acc_bytes);
+  return (int)std::max<int64_t>(1, (items + cap - 1) / cap);
+}
+
 #define RVVLA_CONVERT_BOTH(FMT, BUILD, UPLOAD)                                                     \
   RVVLA_CONVERT(double, FMT, BUILD, UPLOAD) RVVLA_CONVERT(float, FMT, BUILD, UPLOAD)
 
@@ -305,9 +311,10 @@ RVVLA_CONVERT_BOTH(spMtxVHCC,
                    rvvla::build_vhcc(d, s, iparam(p, "vec_size", 64),
                                      iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kVhccPartEntries))),
                    rvvla::upload_vhcc(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
-// VNEC key "threads" (merge-path partitions; default: 2048 rows+entries each)
+// VNEC key "threads" (merge-path partitions; default: as many rows+entries
+// each as one workgroup stages, 1024 for double, 2048 for float)
 RVVLA_CONVERT_BOTH(spMtxVNEC,
-                   rvvla::build_vnec(d, s, iparam(p, "threads", (int)std::max<int64_t>(1, ((int64_t)s.m + (int64_t)s.nz + 2047) / 2048))),
+                   rvvla::build_vnec(d, s, iparam(p, "threads", vnec_parts((int64_t)s.m + (int64_t)s.nz, sizeof(*s.Val)))),
                    rvvla::upload_vnec(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, s))
 
 }  // namespace SparseMatrixLib

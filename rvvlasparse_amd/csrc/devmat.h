@@ -162,11 +162,19 @@ struct DevLAV {
   const int32_t* sp_chunk = nullptr;   // the split chunks
   const int32_t* sp_first = nullptr;   // nsplit + 1: their pieces
   void* part = nullptr;                // npieces * 64 partial sums
+  // sparse_part rows of at most kLavFuseRow entries: each row's sparse
+  // entries are added by the lane that finishes its dense sum (one pass,
+  // y written once) instead of by a CSR multiply before the dense kernels
+  bool fused = false;
 };
+constexpr int kLavFuseRow = 32;
 
-// VNEC: merge-path partitions of at most kVnecItems rows + entries, each
-// with its compact list of touched columns (staged in LDS).
-constexpr int kVnecItems = 2048;
+// VNEC: merge-path partitions of at most vnec_items(accumulator bytes)
+// rows + entries, each with its compact list of touched columns (staged in
+// LDS).  A workgroup stages at most that many x values and products: 16 KiB
+// of LDS, so that ten workgroups fit a CU (1024 items for FP64, 2048 for the
+// FP32-accumulating forms).
+constexpr int vnec_items(int acc_bytes) { return 8192 / acc_bytes; }
 struct DevVNEC {
   int64_t nparts = 0;
   const int32_t* cx = nullptr;       // nparts + 1 merge-path row coordinates

@@ -4,7 +4,11 @@
 // Semantics follow the reference scalar kernel (reference
 // src/scalar/spmv_mtx_lav.cpp:23-79): first the CSR multiply of sparse_part
 // with beta (here the adaptive CSR kernel), then every dense entry adds
-// alpha * val * x[col] into its chunk row.  Because the library sorts rows
+// alpha * val * x[col] into its chunk row.  When no sparse_part row has more
+// than kLavFuseRow entries the two are fused: the lane that finishes a
+// row's dense sum adds the row's sparse entries and writes
+// y = alpha * (dense + sparse) + beta * y once (no CSR pass, no second
+// pass over y).  Because the library sorts rows
 // once for all segments, a wavefront walks chunk c through every segment,
 // keeps the lane sums in registers and adds alpha * sum into y once.  Slots
 // past a step's mask were zeroed at upload, so all 64 lanes load every step
@@ -16,18 +20,48 @@
 
 namespace rvvla {
 
+// The sparse_part CSR arrays, for the fused form.
 template <class VT>
+struct LavSparse {
+  const int32_t* rst;
+  const int32_t* col;
+  const VT* val;
+};
+
+// Fused form: the sparse_part sum of row r (at most kLavFuseRow entries).
+template <class VT>
+__device__ __forceinline__ typename Val<VT>::acc lav_sparse_row(
+    const LavSparse<VT>& sp, int e0, int e1, const typename Val<VT>::io* __restrict__ x) {
+  using A = typename Val<VT>::acc;
+  A s = 0;
+  for (int e = e0; e < e1; ++e) s += Val<VT>::get(sp.val, e) * (A)x[sp.col[e]];
+  return s;
+}
+
+template <class VT, bool FUSED>
 __global__ __launch_bounds__(kWG) void k_lav_dense(
     const int32_t* __restrict__ coff, const int64_t* __restrict__ sptr, int64_t nseg,
     int64_t nchunks, const int32_t* __restrict__ whole, int64_t nwhole,
     const uint32_t* __restrict__ out_order, const uint32_t* __restrict__ col,
     const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
-    typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha) {
+    typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha,
+    LavSparse<VT> sp, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
   if (w >= nwhole) return;
   const int64_t c = whole ? (int64_t)whole[w] : w;
   const int lane = threadIdx.x & 63;
+  const int64_t rp = c * 64 + lane;
+  // the lane's row (and, fused, its sparse_part bounds) before the dense walk
+  uint32_t r = 0;
+  int s0 = 0, s1 = 0;
+  if (rp < m) {
+    r = out_order[rp];
+    if constexpr (FUSED) {
+      s0 = sp.rst[r];
+      s1 = sp.rst[r + 1];
+    }
+  }
   A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int64_t g = 0; g < nseg; ++g) {
     const int64_t base = sptr[g];
@@ -48,10 +82,12 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
       a0 += Val<VT>::get(val, e0) * (A)x[col[e0]];
     }
   }
-  const int64_t rp = c * 64 + lane;
   if (rp < m) {
-    const uint32_t r = out_order[rp];
-    y[r] = (A)y[r] + alpha * ((a0 + a1) + (a2 + a3));
+    const A dense = (a0 + a1) + (a2 + a3);
+    if constexpr (FUSED)
+      y[r] = alpha * (lav_sparse_row(sp, s0, s1, x) + dense) + scaled_y(beta, y, r);
+    else
+      y[r] = (A)y[r] + alpha * dense;
   }
 }
 
@@ -89,11 +125,12 @@ __global__ __launch_bounds__(kWG) void k_lav_pieces(
 // One split chunk per workgroup: the sum of its pieces' partials (all four
 // wavefronts load pieces), then the first wavefront adds alpha * sum into
 // the chunk's rows.
-template <class VT>
+template <class VT, bool FUSED>
 __global__ __launch_bounds__(kWG) void k_lav_combine(
     const int32_t* __restrict__ sp_chunk, const int32_t* __restrict__ sp_first,
     const typename Val<VT>::acc* __restrict__ part, const uint32_t* __restrict__ out_order,
-    typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha) {
+    typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha,
+    LavSparse<VT> sp, const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   __shared__ A lds[kWG];
   const int64_t w = blockIdx.x;
@@ -102,7 +139,10 @@ __global__ __launch_bounds__(kWG) void k_lav_combine(
   const int64_t rp = (int64_t)sp_chunk[w] * 64 + threadIdx.x;
   if (rp < m) {
     const uint32_t r = out_order[rp];
-    y[r] = (A)y[r] + alpha * s;
+    if constexpr (FUSED)
+      y[r] = alpha * (lav_sparse_row(sp, sp.rst[r], sp.rst[r + 1], x) + s) + scaled_y(beta, y, r);
+    else
+      y[r] = (A)y[r] + alpha * s;
   }
 }
 
@@ -111,14 +151,23 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
                    hipStream_t s) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
-  if (int rc = launch_csr(A, alpha, x, beta, y, s)) return rc;  // sparse_part, with beta
   const DevLAV& d = A.lav;
+  // sparse_part with beta first, unless each row's sparse entries are added
+  // by the dense kernels (fused)
+  if (!d.fused)
+    if (int rc = launch_csr(A, alpha, x, beta, y, s)) return rc;
   if (d.nseg == 0 || d.nchunks == 0) return RVVLA_OK;
+  const LavSparse<VT> sp{A.csr.rst, A.csr.col, (const VT*)A.csr.val};
   constexpr int W = kWG / 64;
   if (d.nwhole > 0) {
-    hipLaunchKernelGGL((k_lav_dense<VT>), dim3((unsigned)((d.nwhole + W - 1) / W)), dim3(kWG), 0,
-                       s, d.coff, d.sptr, d.nseg, d.nchunks, d.whole, d.nwhole, d.out_order, d.col,
-                       (const VT*)d.val, (const IO*)x, (IO*)y, A.m, (AC)alpha);
+    const unsigned g = (unsigned)((d.nwhole + W - 1) / W);
+#define RVVLA_LAV_DENSE(F)                                                                      \
+  hipLaunchKernelGGL((k_lav_dense<VT, F>), dim3(g), dim3(kWG), 0, s, d.coff, d.sptr, d.nseg,    \
+                     d.nchunks, d.whole, d.nwhole, d.out_order, d.col, (const VT*)d.val,        \
+                     (const IO*)x, (IO*)y, A.m, (AC)alpha, sp, (AC)beta)
+    if (d.fused) RVVLA_LAV_DENSE(true);
+    else RVVLA_LAV_DENSE(false);
+#undef RVVLA_LAV_DENSE
     if (int rc = hip_check(hipGetLastError(), "k_lav_dense launch")) return rc;
   }
   if (d.nsplit > 0) {
@@ -126,9 +175,14 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
                        0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
                        (AC*)d.part);
     if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
-    hipLaunchKernelGGL((k_lav_combine<VT>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
-                       d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order,
-                       (IO*)y, A.m, (AC)alpha);
+    if (d.fused)
+      hipLaunchKernelGGL((k_lav_combine<VT, true>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+                         d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order, (IO*)y, A.m,
+                         (AC)alpha, sp, (const IO*)x, (AC)beta);
+    else
+      hipLaunchKernelGGL((k_lav_combine<VT, false>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+                         d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order, (IO*)y, A.m,
+                         (AC)alpha, sp, (const IO*)x, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_lav_combine launch")) return rc;
   }
   return RVVLA_OK;

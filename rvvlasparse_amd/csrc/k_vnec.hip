@@ -16,7 +16,8 @@
 
 namespace rvvla {
 
-static_assert(kVnecItems <= 8 * kWG, "k_vnec stages at most 8 entries per thread");
+static_assert(vnec_items(4) <= 8 * kWG && vnec_items(8) <= 8 * kWG,
+              "k_vnec stages at most 8 entries per thread");
 
 template <class VT>
 __global__ __launch_bounds__(kWG) void k_vnec(
@@ -28,8 +29,9 @@ __global__ __launch_bounds__(kWG) void k_vnec(
     typename Val<VT>::acc* __restrict__ fwd_part, int64_t m, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  __shared__ A xs[kVnecItems];
-  __shared__ A prod[kVnecItems];
+  constexpr int kItems = vnec_items((int)sizeof(A));
+  __shared__ A xs[kItems];
+  __shared__ A prod[kItems];
   const int t = blockIdx.x, tid = threadIdx.x;
   const int x0 = cx[t], x1 = cx[t + 1], y0 = cy[t], y1 = cy[t + 1];
   const int k0 = nec_ptr[t], nec = nec_ptr[t + 1] - k0;
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kWG) void k_vnec(
     lo = max(rst[r], y0) - y0;
     hi = min(rst[r + 1], y1) - y0;
   }
-  const int n = y1 - y0;  // <= kVnecItems = 8 * kWG
+  const int n = y1 - y0;  // <= kItems <= 8 * kWG
   A v[8];
   uint16_t c[8];
 #pragma unroll
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(kWG) void k_vnec(
     }
   };
   if (nrows > kWG / 2) {
-    // a partition may hold up to kVnecItems (empty) rows: stride over them
+    // a partition may hold up to kItems (empty) rows: stride over them
     for (int i = tid; i < nrows; i += kWG) {
       const int r = x0 + i;
       const int l = i == tid ? lo : max(rst[r], y0) - y0;

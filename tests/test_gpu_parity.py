@@ -205,6 +205,37 @@ def test_csr5_sigmas(sigma, dtype, name):
                         f"csr5 sigma={sigma} {name}")
 
 
+def _band(m, half):
+    """m x m band matrix, entries |i - j| <= half, values from a fixed seed."""
+    rows, cols = [], []
+    for i in range(m):
+        j = np.arange(max(0, i - half), min(m, i + half + 1))
+        rows.append(np.full(len(j), i))
+        cols.append(j)
+    col = np.concatenate(cols).astype(np.int32)
+    rst = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int32)
+    return R.CSR(m, m, rst, col, np.random.default_rng(m).normal(size=len(col)))
+
+
+@pytest.mark.parametrize("params", ["T=0", "", "T=0.5"])
+@pytest.mark.parametrize("name", ["lap64", "band", "random16_c1"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_lav_sparse_part_forms(params, name, dtype):
+    """LAV with the sparse_part rows added by the dense kernels (no
+    sparse_part row longer than 32 entries; with T=0 the sparse_part is
+    empty and the band's long chunks run as pieces) and with a separate
+    sparse_part multiply (long sparse_part rows), against the oracle."""
+    a = _band(3000, 70) if name == "band" else case(name)
+    x64, y064 = inputs(a, 17)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    for alpha, beta in ((1.0, 0.0), (0.75, -0.5)):
+        y = run_gpu("lav", dtype, a, x64, y064, alpha, beta, params)
+        v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        y_ref = O.csr_spmv(a.rst, a.col, v, x, alpha, beta, y0)
+        assert_within_bound(y, y_ref, a.rst, a.col, v, x, alpha, beta, y0, dt,
+                            f"lav {params} {name}/a={alpha},b={beta}")
+
+
 @pytest.mark.parametrize("sigma", [4, 8, 16])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_csr5_long_segments(sigma, dtype):
