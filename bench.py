@@ -20,7 +20,9 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -28,9 +30,24 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-# host threads allowed to this process (no OpenMP thread binding: it would
-# pin the main thread, and with it the GPU runtime's threads, to one core)
-HOST_CPUS = len(os.sched_getaffinity(0))
+# host CPUs allowed to this process; the CPU baseline runs in a child
+# process whose OpenMP threads are bound one per physical core (binding them
+# here would pin this process's main thread, and the GPU runtime's threads,
+# to one core)
+HOST_CPUS = sorted(os.sched_getaffinity(0))
+
+
+def physical_cores(cpus) -> int:
+    """Distinct (package, core) pairs among `cpus` (sysfs topology)."""
+    seen = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            seen.add((Path(base + "physical_package_id").read_text().strip(),
+                      Path(base + "core_id").read_text().strip()))
+        except OSError:
+            return len(cpus)
+    return max(1, len(seen))
 
 METRIC = "SpMV GB/s & % HBM peak per format; BF16-val vs FP64 speedup @ stated rel err"
 PEAK_GBS = 8000.0  # MI355X HBM3E, AMD specification
@@ -84,40 +101,59 @@ def copy_probe(torch, nbytes=1 << 30, steps=20):
 
 def cpu_baseline(a, x64, byts, seconds=10.0):
     """Scalar OpenMP CSR restatement of the reference (reference
-    src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores: the median
-    of as many full Laplacian SpMVs as fit in `seconds` (at least 20) after 5
-    warm-ups, with every allowed thread; then the same on one thread for
-    about a third of that time."""
+    src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores, in a child
+    process with OMP_NUM_THREADS = the physical cores allowed to this
+    process, OMP_PROC_BIND=close, OMP_PLACES=cores: the median of as many
+    full Laplacian SpMVs as fit in `seconds` (at least 20) after 5 warm-ups;
+    then the same on one thread for about a third of that time."""
+    cores = physical_cores(HOST_CPUS)
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    tmpdir = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    with tempfile.TemporaryDirectory(dir=tmpdir) as td:
+        path = os.path.join(td, "csr.npz")
+        np.savez(path, rst=a.rst, col=a.col, val=a.val, x=x64)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        res = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-child", path,
+                              "--cpu-seconds", str(seconds)],
+                             env=env, stdout=subprocess.PIPE, text=True, timeout=600, check=True)
+    med, cnt, med1, cnt1 = json.loads(res.stdout.strip().splitlines()[-1])
+    return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+            "sample": f"full Laplacian FP64 SpMV, median of {cnt} runs after 5 warm-ups "
+                      f"({med * 1e3:.3f} ms each, {cores} threads, one per physical core, "
+                      f"OMP_PROC_BIND=close)",
+            "single_thread": {"value": round(byts / med1 / 1e9, 2), "unit": "GB/s", "cores": 1,
+                              "sample": f"median of {cnt1} runs ({med1 * 1e3:.3f} ms each)"}}
+
+
+def cpu_child(path, seconds):
+    """The timed part of cpu_baseline (runs in the child process; no GPU)."""
     from oracle import oracle as O
     if O.orc is None:
         O.build(ref=False)
         import importlib
         importlib.reload(O)
-    cores = HOST_CPUS
-    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
-        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    d = np.load(path)
+    rst, col, val, x = d["rst"], d["col"], d["val"], d["x"]
+    m = len(rst) - 1
 
     def sample(threads, budget):
         O.orc.orc_set_threads(threads)
-        y = np.zeros(a.m)
+        y = np.zeros(m)
         for _ in range(5):
-            O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
+            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
         ts = []
         t_end = time.perf_counter() + budget
         while len(ts) < 20 or time.perf_counter() < t_end:
             t0 = time.perf_counter()
-            O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.0, y)
+            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
             ts.append(time.perf_counter() - t0)
         return statistics.median(ts), len(ts)
 
-    med, cnt = sample(cores, seconds)
+    med, cnt = sample(int(os.environ.get("OMP_NUM_THREADS", "1")), seconds)
     med1, cnt1 = sample(1, seconds / 3)
-    O.orc.orc_set_threads(cores)
-    return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"full Laplacian FP64 SpMV, median of {cnt} runs after 5 warm-ups "
-                      f"({med * 1e3:.3f} ms each, {cores} threads)",
-            "single_thread": {"value": round(byts / med1 / 1e9, 2), "unit": "GB/s", "cores": 1,
-                              "sample": f"median of {cnt1} runs ({med1 * 1e3:.3f} ms each)"}}
+    print(json.dumps([med, cnt, med1, cnt1]), flush=True)
 
 
 def powerlaw_leg(R, torch, steps, warmup, fmts):
@@ -271,7 +307,12 @@ def main():
                     help="skip the autograd-SpMM and batched legs")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy probe (counter calibration)")
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args.cpu_child, args.cpu_seconds)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "row-block sharding is a design note, not a measured path", file=sys.stderr)
