@@ -151,8 +151,12 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
  * (no reference counterpart; for layout tests).  CSR fields: "rst", "col",
  * "blocks" and, when the column-sliced image was built, "xs.rst", "xs.col",
  * "xs.val", "xs.blk", "xs.mask", "xs.base", "xs.pref" ("xs.val" in the
- * storage type of the dtype).  *count is 0 for an absent array.  Matrices
- * on a device give RVVLA_ERR_LOGIC. */
+ * storage type of the dtype).  HCSR fields: "brow_ptr", "nb_col", "nb_ptr",
+ * "nb_roff", "rp32", "coo16", "col16", "val", "ebc16", "flat_rows",
+ * "u_br", "u_lr0", "csplit" (the non-empty block list, the per-entry
+ * arrays, the entry-by-entry block-rows and the row units of the others).
+ * *count is 0 for an absent array.  Matrices on a device give
+ * RVVLA_ERR_LOGIC. */
 int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t* count,
                    int* elem_bytes);
 

@@ -363,12 +363,71 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
       }
     }
   }
+  // the other block-rows: units of kWG local rows; COO-kind blocks there
+  // get their entries ordered by local row (a stable reorder of the block,
+  // only where the caller's order differs) and the entry offset of every
+  // unit boundary
+  const int64_t S = (R + kWG - 1) / kWG;
+  std::vector<int32_t> u_br, u_lr0, csplit, flat_rows;
+  std::vector<uint16_t> col16;  // reordered copies, made on first need
+  std::vector<V> vals;
+  for (int64_t br = 0; br < bm; ++br) {
+    if (flat[(size_t)br]) {
+      flat_rows.push_back((int32_t)br);
+      continue;
+    }
+    const int64_t rows = std::min<int64_t>(R, m - br * R);
+    for (int64_t lr = 0; lr < rows; lr += kWG) {
+      u_br.push_back((int32_t)br);
+      u_lr0.push_back((int32_t)lr);
+    }
+    for (int32_t k = brow_ptr[(size_t)br]; k < brow_ptr[(size_t)br + 1]; ++k) {
+      if ((nb_roff[(size_t)k] >> 30) != 2) continue;
+      const int32_t e0 = nb_ptr[(size_t)k], e1 = nb_ptr[(size_t)k + 1];
+      if (!std::is_sorted(coo16.begin() + e0, coo16.begin() + e1)) {
+        if (col16.empty()) {
+          col16.assign(a.col_idx.begin(), a.col_idx.end());
+          vals.assign(a.values.begin(), a.values.end());
+        }
+        std::vector<int32_t> ord((size_t)(e1 - e0));
+        for (int32_t i = 0; i < e1 - e0; ++i) ord[(size_t)i] = e0 + i;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](int32_t p, int32_t q) { return coo16[(size_t)p] < coo16[(size_t)q]; });
+        std::vector<uint16_t> r2, c2;
+        std::vector<V> v2;
+        for (int32_t e : ord) {
+          r2.push_back(coo16[(size_t)e]);
+          c2.push_back(col16[(size_t)e]);
+          v2.push_back(vals[(size_t)e]);
+        }
+        std::copy(r2.begin(), r2.end(), coo16.begin() + e0);
+        std::copy(c2.begin(), c2.end(), col16.begin() + e0);
+        std::copy(v2.begin(), v2.end(), vals.begin() + e0);
+      }
+      if (csplit.size() + (size_t)S + 1 > 0x3FFFFFFFu) throw Error(RVVLA_ERR_LENGTH, "HCSR: csplit overflow");
+      nb_roff[(size_t)k] = (2u << 30) | (uint32_t)csplit.size();
+      int32_t e = e0;
+      for (int64_t s = 0; s <= S; ++s) {
+        while (e < e1 && (int64_t)coo16[(size_t)e] < s * kWG) ++e;
+        csplit.push_back(e - e0);
+      }
+    }
+  }
   auto M = new_mat(RVVLA_FMT_HCSR, dtype, m, n, nz);
   DevHCSR& d = M->hcsr;
   if (any_flat) {
     d.flat = M->put_vec(flat);
     d.ebc16 = M->put_vec(ebc16);
+    d.nflat = (int64_t)flat_rows.size();
+    d.flat_rows = M->put_vec(flat_rows);
   }
+  d.nunits = (int64_t)u_br.size();
+  d.nblk = (int64_t)nb_col.size();
+  d.nrp32 = (int64_t)rp32.size();
+  d.ncsplit = (int64_t)csplit.size();
+  d.u_br = M->put_vec(u_br);
+  d.u_lr0 = M->put_vec(u_lr0);
+  d.csplit = M->put_vec(csplit);
   d.R = R;
   d.C = C;
   d.b_m = bm;
@@ -379,8 +438,9 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
   d.nb_roff = M->put_vec(nb_roff);
   d.rp32 = M->put_vec(rp32);
   d.coo16 = M->put_vec(coo16);
-  d.col16 = M->put_vec(a.col_idx);
-  d.val = put_values(*M, dtype, a.values.data(), a.values.size());
+  d.col16 = M->put_vec(col16.empty() ? a.col_idx : col16);
+  d.val = vals.empty() ? put_values(*M, dtype, a.values.data(), a.values.size())
+                       : put_values(*M, dtype, vals.data(), vals.size());
   return M.release();
 }
 
@@ -1367,18 +1427,34 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
     rvvla::set_last_error("rvvla_mat_view needs a matrix uploaded in host-validation mode");
     return RVVLA_ERR_LOGIC;
   }
-  if (A->format != RVVLA_FMT_CSR) {
-    rvvla::set_last_error("rvvla_mat_view: CSR matrices only");
+  if (A->format != RVVLA_FMT_CSR && A->format != RVVLA_FMT_HCSR) {
+    rvvla::set_last_error("rvvla_mat_view: CSR and HCSR matrices only");
     return RVVLA_ERR_UNKNOWN_FORMAT;
   }
   const rvvla::DevCSR& c = A->csr;
   const rvvla::DevXSlice& x = c.xs;
+  const rvvla::DevHCSR& h = A->hcsr;
   const int vb = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
   const int64_t nblk = 2 * c.nblocks;
   const int64_t xnnz = x.P ? A->nnz : 0, xpairs = x.P ? x.pairs + 1 : 0;
   const std::string f = field;
   struct V { const void* p; int64_t n; int e; };
-  const V v = f == "rst"     ? V{c.rst, A->m + 1, 4}
+  const bool hc = A->format == RVVLA_FMT_HCSR;
+  const V v = hc ? (f == "brow_ptr"    ? V{h.brow_ptr, h.b_m + 1, 4}
+                  : f == "nb_col"    ? V{h.nb_col, h.nblk, 4}
+                  : f == "nb_ptr"    ? V{h.nb_ptr, h.nblk + 1, 4}
+                  : f == "nb_roff"   ? V{h.nb_roff, h.nblk, 4}
+                  : f == "rp32"      ? V{h.rp32, h.nrp32, 4}
+                  : f == "coo16"     ? V{h.coo16, A->nnz, 2}
+                  : f == "col16"     ? V{h.col16, A->nnz, 2}
+                  : f == "val"       ? V{h.val, A->nnz, vb}
+                  : f == "ebc16"     ? V{h.ebc16, h.nflat ? A->nnz : 0, 2}
+                  : f == "flat_rows" ? V{h.flat_rows, h.nflat, 4}
+                  : f == "u_br"      ? V{h.u_br, h.nunits, 4}
+                  : f == "u_lr0"     ? V{h.u_lr0, h.nunits, 4}
+                  : f == "csplit"    ? V{h.csplit, h.ncsplit, 4}
+                                     : V{nullptr, -1, 0})
+            : f == "rst"     ? V{c.rst, A->m + 1, 4}
             : f == "col"     ? V{c.col, A->nnz, 4}
             : f == "blocks"  ? V{c.blocks, nblk, 4}
             : f == "xs.rst"  ? V{x.rst, xpairs, 4}

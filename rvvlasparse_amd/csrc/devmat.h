@@ -57,6 +57,17 @@ struct DevHCSR {
   // entries and ebc16 the block column of every entry
   const uint8_t* flat = nullptr;      // b_m flags (nullptr: none)
   const uint16_t* ebc16 = nullptr;
+  int64_t nflat = 0;
+  const int32_t* flat_rows = nullptr;  // the flat block-rows
+  // the other block-rows run as units of kWG local rows (one per thread):
+  // unit u is local rows u_lr0[u] .. +kWG of block-row u_br[u]; the entries
+  // of a COO-kind block there are ordered by local row, and csplit (at the
+  // block's offset field) holds the entry offset of each unit boundary
+  int64_t nunits = 0;
+  int64_t nblk = 0, nrp32 = 0, ncsplit = 0;  // sizes of nb_*, rp32, csplit
+  const int32_t* u_br = nullptr;
+  const int32_t* u_lr0 = nullptr;
+  const int32_t* csplit = nullptr;
 };
 constexpr int kHcsrFlat = 32;
 

@@ -306,7 +306,7 @@ def test_direct_uploads_match_converted(dtype):
 def test_hcsr_upload_unsorted_coo_blocks(dtype):
     """A directly uploaded HCSR whose COO-kind blocks list their entries out
     of row order (allowed by the format) gives the same product: the upload
-    re-sorts those blocks for the band kernel."""
+    orders those blocks by local row for the row-unit kernel."""
     import ctypes as C
     a = case("rand_long")
     x64, y064 = inputs(a, 8)
