@@ -133,16 +133,32 @@ int rvvla_set_host_validation(int on);
  * F16 values are rounded on the device) by radix sort, scan and scatter,
  * ordered on `stream`, synchronously.  The CSR is checked on the device
  * first (RVVLA_ERR_INVALID_ARG when bad).  sigma <= 0 applies the default
- * rule (1 when natural 64-row slices pad by at most 1 %, else 16384).  The
- * arrays equal convert(RVVLA_FMT_SELL) + upload bit for bit (the convert
- * path's host fields can be read back with rvvla_mat_read). */
+ * rule (1 when natural 64-row slices pad by at most 1 %; m when sorting all
+ * rows at least halves the padded slots; else 16384).  The arrays equal
+ * convert(RVVLA_FMT_SELL) + upload bit for bit (the convert path's host
+ * fields can be read back with rvvla_mat_read). */
 int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int sigma, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> HCSR (replaces the host converter crs_to_HCSR,
+ * reference src/scalar/crs_to_hcsr.cpp:30-169, behind
+ * convert<T, spMtxHCSR>, include/convert.h:30-31, for a CSR already on the
+ * device).  Radix sort by block, binary searches and scans build the
+ * reference layout -- block_ptr, row_offset, row_ptr, col_idx, values, bit
+ * for bit what the host converter gives for the same R, C, p (readable with
+ * rvvla_mat_read under those names) -- and the kernels' image from it, on
+ * `stream`, synchronously.  R, C <= 0 and p <= 0 take the reference
+ * defaults (R = C = 2000 for F64, 4000 otherwise; p = 2).  The CSR is
+ * checked on the device first (RVVLA_ERR_INVALID_ARG when bad). */
+int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                           int64_t R, int64_t C, double p, void* stream, rvvla_mat_t* out);
+
 /* Copy of one array of an uploaded matrix into host memory (device or
  * host-validation matrices): SELL "vCl", "vCs", "vBs", "vSBs", "vCol",
- * "vPerm", "vVal"; CSR "rst", "col", "val".  dst = NULL asks for *count
+ * "vPerm", "vVal"; CSR "rst", "col", "val"; HCSR built on the device
+ * "block_ptr", "row_offset", "row_ptr", "col_idx", "values".  dst = NULL asks for *count
  * and *elem_bytes only; capacity is in bytes (RVVLA_ERR_LENGTH if short). */
 int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity,
                    int64_t* count, int* elem_bytes);

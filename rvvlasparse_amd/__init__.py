@@ -73,6 +73,8 @@ _sigs = {
     "rvvla_mat_read": (_i32, [_vp, _cp, _vp, _i64, C.POINTER(_i64), C.POINTER(_i32)]),
     "rvvla_sell_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp,
                                       C.POINTER(_vp)]),
+    "rvvla_hcsr_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _d,
+                                      _vp, C.POINTER(_vp)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -294,6 +296,18 @@ def sell_from_device(dtype: int, m: int, n: int, rst, col, val, sigma: int = 0,
     h = C.c_void_p()
     nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
     check(lib.rvvla_sell_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), sigma,
+                                     _stream(stream, val), C.byref(h)))
+    return DeviceMatrix(h)
+
+
+def hcsr_from_device(dtype: int, m: int, n: int, rst, col, val, R: int = 0, C_: int = 0,
+                     p: float = 0.0, stream=None) -> DeviceMatrix:
+    """HCSR built on the device from device CSR arrays (torch tensors or raw
+    addresses): int32 rst/col, float64 (F64) or float32 values.  R, C_, p <= 0:
+    the reference defaults."""
+    h = C.c_void_p()
+    nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
+    check(lib.rvvla_hcsr_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), R, C_, p,
                                      _stream(stream, val), C.byref(h)))
     return DeviceMatrix(h)
 

@@ -1365,6 +1365,43 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                           const int32_t* d_col, const void* d_val, int64_t R, int64_t C,
+                           double p, void* stream, rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    // reference defaults: R = C = 2000 (FP64), 4000 (FP32), p = 2
+    // (reference src/scalar/crs_to_hcsr.cpp:141-143, :158-160)
+    const int64_t dflt = dtype == RVVLA_F64 ? 2000 : 4000;
+    if (R <= 0) R = dflt;
+    if (C <= 0) C = dflt;
+    if (!(p > 0.0)) p = 2.0;
+    if (C > 65536) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: C must not exceed 65536 (16-bit col_idx)");
+    if (R * (dtype == RVVLA_F64 ? 8 : 4) > 65536)
+      throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "HCSR: R * accumulator size must fit 64 KiB of LDS");
+    const int64_t bm = (m + R - 1) / R, bn = (n + C - 1) / C;
+    if (bn != 0 && bm > (int64_t(2147483646) / bn))
+      throw Error(RVVLA_ERR_LENGTH, "HCSR: more than 2^31 - 2 blocks");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_HCSR, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_hcsr_device<double>(*M, m, n, nnz, d_rst, d_col, (const double*)d_val, R, C, p, s)
+        : rvvla::build_hcsr_device<float>(*M, m, n, nnz, d_rst, d_col, (const float*)d_val, R, C, p, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity, int64_t* count,
                    int* elem_bytes) {
   if (!A || !field || !count || !elem_bytes) return RVVLA_ERR_INVALID_ARG;
@@ -1398,6 +1435,13 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     const rvvla::DevCSR& c = A->csr;
     v = f == "rst" ? V{c.rst, A->m + 1, 4} : f == "col" ? V{c.col, A->nnz, 4}
       : f == "val" ? V{c.val, A->nnz, vb} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_HCSR && A->hcsr.ref_block_ptr) {
+    // built on the device: the reference layout
+    const rvvla::DevHCSR& h = A->hcsr;
+    const int64_t nb = h.b_m * h.b_n + 1;
+    v = f == "block_ptr" ? V{h.ref_block_ptr, nb, 4} : f == "row_offset" ? V{h.ref_row_offset, nb, 4}
+      : f == "row_ptr" ? V{h.rp32, h.nrp32, 4} : f == "col_idx" ? V{h.col16, A->nnz, 2}
+      : f == "values" ? V{h.val, A->nnz, vb} : V{nullptr, -1, 0};
   }
   if (v.n < 0) {
     rvvla::set_last_error("rvvla_mat_read: unknown field " + f + " for this format");

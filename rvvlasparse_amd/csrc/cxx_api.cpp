@@ -177,27 +177,6 @@ double dparam(const convertParams& p, const char* k, double dflt) {
 
 }  // namespace
 
-// convert returns void as in the reference; failures are thrown as
-// rvvla::Error (a std::runtime_error carrying the RVVLA_* code), where the
-// reference converter throws std::invalid_argument / length_error /
-// logic_error (reference src/scalar/crs_to_hcsr.cpp:38,55,87-91).
-// HCSR keys "R", "C" (int) and "p" (double), defaults as the reference's
-// (reference src/scalar/crs_to_hcsr.cpp:141-149, :158-166).
-template <>
-void convert<double, spMtxHCSR, true>(spMtxHCSR<double>& d, const spMtxCRS<double>& s,
-                                      const convertParams& p) {
-  rvvla::build_hcsr<double>(d, s, (size_t)iparam(p, "R", 2000), (size_t)iparam(p, "C", 2000),
-                            dparam(p, "p", 2.0));
-  d.device.reset(rvvla::upload_hcsr<double>(RVVLA_F64, d));
-}
-template <>
-void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>& s,
-                                     const convertParams& p) {
-  rvvla::build_hcsr<float>(d, s, (size_t)iparam(p, "R", 4000), (size_t)iparam(p, "C", 4000),
-                           dparam(p, "p", 2.0));
-  d.device.reset(rvvla::upload_hcsr<float>(RVVLA_F32, d));
-}
-
 #define RVVLA_CONVERT(T, FMT, BUILD, UPLOAD)                                                       \
   template <>                                                                                      \
   void convert<T, FMT, true>(FMT<T> & d, const spMtxCRS<T>& s, const convertParams& p) {           \
@@ -226,6 +205,33 @@ void read_field(rvvla_mat_t A, const char* f, void* dst, size_t bytes) {
   if (rc) throw Error(rc, rvvla_strerror(rc));
 }
 
+// The CSR's arrays copied to device memory, for the device-side builders.
+template <class T>
+struct DeviceCsr {
+  void* rst = nullptr;
+  void* col = nullptr;
+  void* val = nullptr;
+  explicit DeviceCsr(const spMtxCRS<T>& s) {
+    rvvla::check_csr_public((int64_t)s.m, (int64_t)s.n, (int64_t)s.nz, s.Rst, s.Col);
+    const size_t m = s.m, nz = s.nz;
+    int rc = rvvla::hip_check(hipMalloc(&rst, (m + 1) * 4), "hipMalloc");
+    if (!rc) rc = rvvla::hip_check(hipMalloc(&col, nz * 4 + 8), "hipMalloc");
+    if (!rc) rc = rvvla::hip_check(hipMalloc(&val, nz * sizeof(T) + 8), "hipMalloc");
+    if (!rc) rc = rvvla::hip_check(hipMemcpy(rst, s.Rst, (m + 1) * 4, hipMemcpyHostToDevice), "copy");
+    if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(col, s.Col, nz * 4, hipMemcpyHostToDevice), "copy");
+    if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(val, s.Val, nz * sizeof(T), hipMemcpyHostToDevice), "copy");
+    if (rc) {
+      release();
+      throw Error(rc, rvvla::last_error());
+    }
+  }
+  ~DeviceCsr() { release(); }
+  void release() {
+    for (void** p : {&rst, &col, &val})
+      if (*p) (void)hipFree(*p), *p = nullptr;
+  }
+};
+
 template <class T>
 void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
   const int C = iparam(p, "C", 64);
@@ -235,24 +241,12 @@ void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertPa
     d.device.reset(rvvla::upload_sell(dtype_of<T>(), d));
     return;
   }
-  rvvla::check_csr_public((int64_t)s.m, (int64_t)s.n, (int64_t)s.nz, s.Rst, s.Col);
-  struct Buf {
-    void* p = nullptr;
-    ~Buf() {
-      if (p) (void)hipFree(p);
-    }
-  } rst, col, val;
   const size_t m = s.m, nz = s.nz;
-  int rc = rvvla::hip_check(hipMalloc(&rst.p, (m + 1) * 4), "hipMalloc");
-  if (!rc) rc = rvvla::hip_check(hipMalloc(&col.p, nz * 4 + 8), "hipMalloc");
-  if (!rc) rc = rvvla::hip_check(hipMalloc(&val.p, nz * sizeof(T) + 8), "hipMalloc");
-  if (!rc) rc = rvvla::hip_check(hipMemcpy(rst.p, s.Rst, (m + 1) * 4, hipMemcpyHostToDevice), "copy");
-  if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(col.p, s.Col, nz * 4, hipMemcpyHostToDevice), "copy");
-  if (!rc && nz) rc = rvvla::hip_check(hipMemcpy(val.p, s.Val, nz * sizeof(T), hipMemcpyHostToDevice), "copy");
-  if (rc) throw Error(rc, rvvla::last_error());
+  DeviceCsr<T> dc(s);
   rvvla_mat_t A = nullptr;
-  rc = rvvla_sell_from_device(dtype_of<T>(), (int64_t)m, (int64_t)s.n, (int64_t)nz,
-                              (const int32_t*)rst.p, (const int32_t*)col.p, val.p, sigma, nullptr, &A);
+  int rc = rvvla_sell_from_device(dtype_of<T>(), (int64_t)m, (int64_t)s.n, (int64_t)nz,
+                                  (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val, sigma,
+                                  nullptr, &A);
   if (rc) throw Error(rc, rvvla_strerror(rc));
   d.device.reset(A);
   d.m = s.m;
@@ -282,6 +276,72 @@ void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertPa
 }
 
 }  // namespace
+
+// convert returns void as in the reference; failures are thrown as
+// rvvla::Error (a std::runtime_error carrying the RVVLA_* code), where the
+// reference converter throws std::invalid_argument / length_error /
+// logic_error (reference src/scalar/crs_to_hcsr.cpp:38,55,87-91).
+// HCSR keys "R", "C" (int) and "p" (double), defaults as the reference's
+// (reference src/scalar/crs_to_hcsr.cpp:141-149, :158-166), and "device"
+// (1: build on the device from the uploaded CSR, then read the reference
+// fields back; the same arrays as the host build).
+namespace {
+
+template <class T>
+void convert_hcsr(spMtxHCSR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int dflt = std::is_same<T, double>::value ? 2000 : 4000;
+  const int64_t R = iparam(p, "R", dflt), C = iparam(p, "C", dflt);
+  const double pp = dparam(p, "p", 2.0);
+  if (!iparam(p, "device", 0)) {
+    rvvla::build_hcsr<T>(d, s, (size_t)R, (size_t)C, pp);
+    d.device.reset(rvvla::upload_hcsr<T>(dtype_of<T>(), d));
+    return;
+  }
+  if (R <= 0 || C <= 0) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: R and C must be positive");
+  DeviceCsr<T> dc(s);
+  rvvla_mat_t A = nullptr;
+  int rc = rvvla_hcsr_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                  (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val, R, C, pp,
+                                  nullptr, &A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  d = spMtxHCSR<T>();
+  d.device.reset(A);
+  d.R = (size_t)R;
+  d.C = (size_t)C;
+  d.m = s.m;
+  d.n = s.n;
+  d.nz = s.nz;
+  d.b_m = (s.m + (size_t)R - 1) / (size_t)R;
+  d.b_n = (s.n + (size_t)C - 1) / (size_t)C;
+  int64_t nrp = 0;
+  int eb = 0;
+  rc = rvvla_mat_read(A, "row_ptr", nullptr, 0, &nrp, &eb);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  const size_t nb = d.b_m * d.b_n + 1;
+  d.block_ptr.resize(nb);
+  d.row_offset.resize(nb);
+  d.row_ptr.resize((size_t)nrp);
+  d.col_idx.resize(s.nz);
+  d.values.resize(s.nz);
+  read_field<T>(A, "block_ptr", d.block_ptr.data(), nb * 4);
+  read_field<T>(A, "row_offset", d.row_offset.data(), nb * 4);
+  read_field<T>(A, "row_ptr", d.row_ptr.data(), (size_t)nrp * 4);
+  read_field<T>(A, "col_idx", d.col_idx.data(), s.nz * 2);
+  read_field<T>(A, "values", d.values.data(), s.nz * sizeof(T));
+}
+
+}  // namespace
+
+template <>
+void convert<double, spMtxHCSR, true>(spMtxHCSR<double>& d, const spMtxCRS<double>& s,
+                                      const convertParams& p) {
+  convert_hcsr<double>(d, s, p);
+}
+template <>
+void convert<float, spMtxHCSR, true>(spMtxHCSR<float>& d, const spMtxCRS<float>& s,
+                                     const convertParams& p) {
+  convert_hcsr<float>(d, s, p);
+}
 
 template <>
 void convert<double, spMtxSELL_C_Sigma, true>(spMtxSELL_C_Sigma<double>& d,

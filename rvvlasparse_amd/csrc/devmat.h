@@ -65,6 +65,11 @@ struct DevHCSR {
   // block's offset field) holds the entry offset of each unit boundary
   int64_t nunits = 0;
   int64_t nblk = 0, nrp32 = 0, ncsplit = 0;  // sizes of nb_*, rp32, csplit
+  // built on the device (rvvla_hcsr_from_device): the reference layout's
+  // block_ptr and row_offset over all b_m * b_n blocks (rp32 is then the
+  // reference row_ptr, col16 its col_idx, val its values)
+  const int32_t* ref_block_ptr = nullptr;
+  const uint32_t* ref_row_offset = nullptr;
   const int32_t* u_br = nullptr;
   const int32_t* u_lr0 = nullptr;
   const int32_t* csplit = nullptr;
@@ -299,6 +304,13 @@ int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 template <class V>
 int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int sigma, hipStream_t s);
+// CSR -> HCSR on the device into M.hcsr (k_convert_hcsr.hip): the
+// reference layout (bit-identical to the host converter) and the kernels'
+// image built from it.
+template <class V>
+int build_hcsr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const int32_t* rst,
+                      const int32_t* col, const V* val, int64_t R, int64_t C, double p,
+                      hipStream_t s);
 
 // Pieces of the SELL slices wider than kSellPiece steps, from the widths.
 void attach_sell_pieces(rvvla_mat_s& M, const std::vector<int>& cl);

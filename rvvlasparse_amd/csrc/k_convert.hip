@@ -143,17 +143,6 @@ __global__ void k_conv_not_identity(const uint32_t* __restrict__ perm, int64_t m
   if (r < m && perm[r] != (uint32_t)r) atomicOr(flag, 1);
 }
 
-__device__ __forceinline__ uint16_t dev_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ uint16_t dev_f16(float f) {
-  const _Float16 h = (_Float16)f;
-  return __builtin_bit_cast(uint16_t, h);
-}
-
 // V: input value type (double for F64, float otherwise); S: stored type
 // (V, or uint16_t bits for BF16 / F16 with NARROW = 1 / 2)
 template <class V, class S, int NARROW>

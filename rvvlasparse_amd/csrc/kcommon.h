@@ -57,6 +57,19 @@ struct Val<_Float16> {
   }
 };
 
+// 16-bit value storage on the device, rounded to nearest even (the host
+// conversions f32_to_bf16 / f32_to_f16 give the same bits).
+__device__ __forceinline__ uint16_t dev_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ uint16_t dev_f16(float f) {
+  const _Float16 h = (_Float16)f;
+  return __builtin_bit_cast(uint16_t, h);
+}
+
 // Streamed loads: the nontemporal form marks the line as not worth keeping
 // in the caches (used where x must stay resident in L2).
 template <bool NT, class VT>

@@ -169,6 +169,21 @@ void all_formats(int g) {
     check<T>("csr", "all", a, b, y0, (T)1, (T)0, y, st);
   }
   run<T, spMtxHCSR>("hcsr", a, hp);
+  // HCSR built on the device ("device" = 1): same fields as the host build
+  {
+    convertParams dev = hp;
+    dev.param["device"].i = 1;
+    spMtxHCSR<T> h, d;
+    convert<T, spMtxHCSR, true>(h, a, hp);
+    convert<T, spMtxHCSR, true>(d, a, dev);
+    const bool same = h.R == d.R && h.C == d.C && h.b_m == d.b_m && h.b_n == d.b_n &&
+                      h.block_ptr == d.block_ptr && h.row_offset == d.row_offset &&
+                      h.row_ptr == d.row_ptr && h.col_idx == d.col_idx && h.values == d.values;
+    if (!same) ++failures;
+    std::printf("hcsr  %-6s device-built fields %s\n", std::is_same<T, double>::value ? "double" : "float",
+                same ? "ok" : "FAIL");
+    run<T, spMtxHCSR>("hcsr", a, dev);
+  }
   // SELL built on the device ("device" = 1): same fields as the host build
   {
     convertParams dev;

@@ -1,7 +1,9 @@
-"""Device-side CSR -> SELL-C-sigma conversion (rvvla_sell_from_device): the
-arrays read back from the device equal the host builder's (convert + upload)
-bit for bit, for the default sigma rule and explicit sigmas, in every
-precision; the product matches the oracle; a bad device CSR is refused."""
+"""Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device) and
+CSR -> HCSR (rvvla_hcsr_from_device).  The arrays read back from the device
+equal the host builders' bit for bit (SELL: for the default sigma rule and
+explicit sigmas; HCSR: the reference layout for several block sizes and
+p), in every precision; the products match the oracle; a bad device CSR is
+refused."""
 from __future__ import annotations
 
 import ctypes as C
@@ -77,3 +79,82 @@ def test_device_sell_refuses_bad_csr():
     rc = R.lib.rvvla_sell_from_device(R.F64, a.m, a.n, a.nnz, badr.data_ptr(), col.data_ptr(),
                                       val.data_ptr(), 0, None, C.byref(h))
     assert rc == -10 and not h.value
+
+
+# ---------------------------------------------------------------- HCSR
+HCSR_FIELDS = ("block_ptr", "row_offset", "row_ptr", "col_idx")
+
+
+def _round16(v32, dtype):
+    """Host rounding of FP32 values to the 16-bit storage (nearest even)."""
+    if dtype == R.F16:
+        return v32.astype(np.float16).view(np.uint16)
+    u = v32.view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def _hcsr_params(params):
+    kv = dict(p.split("=") for p in params.split(",")) if params else {}
+    return int(kv.get("R", 0)), int(kv.get("C", 0)), float(kv.get("p", 0.0))
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1"])
+@pytest.mark.parametrize("params", ["", "R=300,C=400,p=2", "R=16,C=16,p=2", "R=64,C=32,p=1",
+                                    "R=7,C=5,p=0.5"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_device_hcsr_equals_host(name, params, dtype):
+    """The reference layout built on the device equals the host converter's
+    (which is pinned against the reference's crs_to_HCSR) bit for bit."""
+    a = case(name)
+    print("device hcsr", name, params, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    Rb, Cb, p = _hcsr_params(params)
+    D = R.hcsr_from_device(dtype, a.m, a.n, rst, col, val, Rb, Cb, p)
+    torch.cuda.synchronize()
+    L = R.convert("hcsr", a, dtype, params)
+    for f in HCSR_FIELDS:
+        h = L.array(f)
+        d = D.read(f).view(h.dtype) if h.size else D.read(f)
+        assert d.shape == h.shape and np.array_equal(d, h), f
+    hv = L.array("values", np.float64 if dtype == R.F64 else np.float32)
+    dv = D.read("values")
+    if dtype in (R.F64, R.F32):
+        assert np.array_equal(dv.view(hv.dtype), hv)
+    else:
+        assert np.array_equal(dv, _round16(hv, dtype))
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty"])
+@pytest.mark.parametrize("params", ["", "R=300,C=400,p=2", "R=16,C=16,p=2"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_hcsr_product(name, params, dtype):
+    a = case(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.hcsr_from_device(dtype, a.m, a.n, rst, col, val, *_hcsr_params(params))
+    dev = _device()
+    x64, y064 = inputs(a, 23)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, 0.75, -1.5)
+    torch.cuda.synchronize()
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 0.75, -1.5, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 0.75, -1.5,
+                        y064.astype(dt), dt, f"device-built HCSR {name} {params}")
+
+
+def test_device_hcsr_refuses_bad_input():
+    a = case("rand_long")
+    rst, col, val = _device_csr(a, R.F64)
+    bad = col.clone()
+    bad[10] = a.n
+    h = C.c_void_p()
+    rc = R.lib.rvvla_hcsr_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), bad.data_ptr(),
+                                      val.data_ptr(), 300, 400, 2.0, None, C.byref(h))
+    assert rc == -10 and not h.value
+    rc = R.lib.rvvla_hcsr_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), col.data_ptr(),
+                                      val.data_ptr(), 300, 70000, 2.0, None, C.byref(h))
+    assert rc == -10 and not h.value  # C above 65536 (16-bit col_idx)
