@@ -2,8 +2,10 @@
 // C-ABI, on a matrix file or a generated matrix.
 //
 //   rvvla_perf <matrix> [--formats all|csr,sell,...] [--dtypes f64,f32,bf16,f16]
-//              [--iters N] [--warmup W]
+//              [--iters N] [--warmup W] [--params "key=value,..."]
 //
+// --params is handed to every conversion (convertParams, e.g. "xslices=0"
+// or "R=256,C=8192").
 // <matrix> as for rvvla_accuracy.  For each (format, dtype): convert and
 // upload once (timed on the host), W untimed products, then N back-to-back
 // products y = A*x timed with HIP events on one stream.  Reports the time
@@ -19,10 +21,10 @@ using namespace apps;
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s <matrix> [--formats all] [--dtypes f64,f32,bf16,f16] "
-                         "[--iters N] [--warmup W]\n", argv[0]);
+                         "[--iters N] [--warmup W] [--params key=value,...]\n", argv[0]);
     return 2;
   }
-  std::string fmts = "all", dts = "f64,f32";
+  std::string fmts = "all", dts = "f64,f32", params;
   int iters = 100, warmup = 10;
   for (int i = 2; i + 1 < argc; i += 2) {
     const std::string k = argv[i];
@@ -30,6 +32,7 @@ int main(int argc, char** argv) {
     else if (k == "--dtypes") dts = argv[i + 1];
     else if (k == "--iters") iters = std::max(1, std::atoi(argv[i + 1]));
     else if (k == "--warmup") warmup = std::max(0, std::atoi(argv[i + 1]));
+    else if (k == "--params") params = argv[i + 1];
     else {
       std::fprintf(stderr, "unknown option %s\n", k.c_str());
       return 2;
@@ -64,7 +67,7 @@ int main(int argc, char** argv) {
         rvvla_layout_t L = nullptr;
         rvvla_mat_t A = nullptr;
         const auto t0 = std::chrono::steady_clock::now();
-        int rc = rvvla_convert(f, dtype, g.m, g.n, g.nnz, g.rst, g.col, V.ptr(dtype), "", &L);
+        int rc = rvvla_convert(f, dtype, g.m, g.n, g.nnz, g.rst, g.col, V.ptr(dtype), params.c_str(), &L);
         if (!rc) rc = rvvla_upload(L, &A);
         const double up = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (rc) {
