@@ -141,6 +141,18 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int sigma, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> CSR5 (omega = 64; replaces spMtxCSR5::inputCSR +
+ * asCSR5, declared without a body in reference include/CSR5.h:61-65, for a
+ * CSR already on the device).  Binary searches, scans and one wavefront per
+ * tile build the tile pointers, descriptors, empty offsets and transposed
+ * entries -- bit for bit what convert(RVVLA_FMT_CSR5) gives for the same
+ * sigma (readable with rvvla_mat_read under the spMtxCSR5 field names) --
+ * and the kernels' extras, on `stream`, synchronously.  sigma <= 0: the
+ * default (16).  The CSR is checked on the device first. */
+int rvvla_csr5_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                           int sigma, void* stream, rvvla_mat_t* out);
+
 /* Device-side CSR -> HCSR (replaces the host converter crs_to_HCSR,
  * reference src/scalar/crs_to_hcsr.cpp:30-169, behind
  * convert<T, spMtxHCSR>, include/convert.h:30-31, for a CSR already on the
@@ -157,8 +169,11 @@ int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
 
 /* Copy of one array of an uploaded matrix into host memory (device or
  * host-validation matrices): SELL "vCl", "vCs", "vBs", "vSBs", "vCol",
- * "vPerm", "vVal"; CSR "rst", "col", "val"; HCSR built on the device
- * "block_ptr", "row_offset", "row_ptr", "col_idx", "values".  dst = NULL asks for *count
+ * "vPerm", "vVal"; CSR "rst", "col", "val"; CSR5 "_csr5_tile_pointer",
+ * "_csr5_tile_descriptor", "_csr5_tile_empty_offset_pointer",
+ * "_csr5_tile_empty_offset", "_csr_column_index", "_csr_value"; HCSR built
+ * on the device "block_ptr", "row_offset", "row_ptr", "col_idx", "values".
+ * dst = NULL asks for *count
  * and *elem_bytes only; capacity is in bytes (RVVLA_ERR_LENGTH if short). */
 int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity,
                    int64_t* count, int* elem_bytes);

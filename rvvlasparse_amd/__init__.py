@@ -75,6 +75,8 @@ _sigs = {
                                       C.POINTER(_vp)]),
     "rvvla_hcsr_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _i64, _d,
                                       _vp, C.POINTER(_vp)]),
+    "rvvla_csr5_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp,
+                                      C.POINTER(_vp)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -308,6 +310,17 @@ def hcsr_from_device(dtype: int, m: int, n: int, rst, col, val, R: int = 0, C_: 
     h = C.c_void_p()
     nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
     check(lib.rvvla_hcsr_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), R, C_, p,
+                                     _stream(stream, val), C.byref(h)))
+    return DeviceMatrix(h)
+
+
+def csr5_from_device(dtype: int, m: int, n: int, rst, col, val, sigma: int = 0,
+                     stream=None) -> DeviceMatrix:
+    """CSR5 (omega = 64) built on the device from device CSR arrays (torch
+    tensors or raw addresses).  sigma <= 0: the default."""
+    h = C.c_void_p()
+    nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
+    check(lib.rvvla_csr5_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), sigma,
                                      _stream(stream, val), C.byref(h)))
     return DeviceMatrix(h)
 

@@ -563,6 +563,8 @@ rvvla_mat_t upload_csr5(int dtype, const spMtxCSR5<V>& a) {
   d.col = M->put_vec(a._csr_column_index);
   d.val = put_values(*M, dtype, a._csr_value.data(), a._csr_value.size());
   d.empty_rows = M->put_vec(empty);
+  d.np = a._p;
+  d.noffsets = (int64_t)a._csr5_tile_empty_offset.size();
   // Where each open segment ends, from the descriptor flags (the same bits
   // the tile kernel reads): bit 0 of kind = a row starts at the tile's first
   // entry, bit 1 = a row starts later in the tile.  A segment left open by
@@ -1365,6 +1367,33 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_csr5_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                           const int32_t* d_col, const void* d_val, int sigma, void* stream,
+                           rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (sigma <= 0) sigma = rvvla::default_csr5_sigma(m, nnz);
+    if (sigma > 64) throw Error(RVVLA_ERR_CONVERSION, "CSR5: sigma must be in [1, 64]");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_CSR5, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_csr5_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, sigma, s)
+        : rvvla::build_csr5_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, sigma, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
                            const int32_t* d_col, const void* d_val, int64_t R, int64_t C,
                            double p, void* stream, rvvla_mat_t* out) {
@@ -1435,6 +1464,14 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     const rvvla::DevCSR& c = A->csr;
     v = f == "rst" ? V{c.rst, A->m + 1, 4} : f == "col" ? V{c.col, A->nnz, 4}
       : f == "val" ? V{c.val, A->nnz, vb} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_CSR5) {
+    const rvvla::DevCSR5& c = A->csr5;
+    v = f == "_csr5_tile_pointer" ? V{c.tile_ptr, c.np + 1, 4}
+      : f == "_csr5_tile_descriptor" ? V{c.desc, c.ntiles * 64 * c.dl, 4}
+      : f == "_csr5_tile_empty_offset_pointer" ? V{c.eptr, c.np + 1, 4}
+      : f == "_csr5_tile_empty_offset" ? V{c.eoff, c.noffsets, 4}
+      : f == "_csr_column_index" ? V{c.col, A->nnz, 4} : f == "_csr_value" ? V{c.val, A->nnz, vb}
+      : V{nullptr, -1, 0};
   } else if (A->format == RVVLA_FMT_HCSR && A->hcsr.ref_block_ptr) {
     // built on the device: the reference layout
     const rvvla::DevHCSR& h = A->hcsr;

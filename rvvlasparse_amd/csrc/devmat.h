@@ -104,6 +104,7 @@ constexpr int kSellPiece = 64;
 // the trailing segment's row).
 struct DevCSR5 {
   int64_t ntiles = 0, nempty = 0, tail_base = 0;
+  int64_t np = 0, noffsets = 0;  // tile_ptr / empty-offset pointer: np + 1; offsets
   int sigma = 0, dl = 0, bit_y = 0, bit_seg = 0, tail_start = 0, first_tail_row = 0;
   const uint32_t* tile_ptr = nullptr;
   const uint32_t* desc = nullptr;
@@ -303,6 +304,11 @@ int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
                      const int32_t* col, hipStream_t s);
 template <class V>
 int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
+                      const int32_t* col, const V* val, int sigma, hipStream_t s);
+// CSR -> CSR5 (omega = 64) on the device into M.csr5 (k_convert_csr5.hip):
+// the host builder's layout bit for bit, plus the kernels' extras.
+template <class V>
+int build_csr5_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int sigma, hipStream_t s);
 // CSR -> HCSR on the device into M.hcsr (k_convert_hcsr.hip): the
 // reference layout (bit-identical to the host converter) and the kernels'

@@ -1,9 +1,11 @@
-"""Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device) and
-CSR -> HCSR (rvvla_hcsr_from_device).  The arrays read back from the device
-equal the host builders' bit for bit (SELL: for the default sigma rule and
-explicit sigmas; HCSR: the reference layout for several block sizes and
-p), in every precision; the products match the oracle; a bad device CSR is
-refused."""
+"""Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device),
+CSR -> HCSR (rvvla_hcsr_from_device) and CSR -> CSR5
+(rvvla_csr5_from_device).  The arrays read back from the device equal the
+host builders' bit for bit (SELL: for the default sigma rule and explicit
+sigmas; HCSR: the reference layout for several block sizes and p; CSR5:
+tile pointers, descriptors, empty offsets and transposed entries for
+several sigmas), in every precision; the products match the oracle; a bad
+device CSR is refused."""
 from __future__ import annotations
 
 import ctypes as C
@@ -158,3 +160,89 @@ def test_device_hcsr_refuses_bad_input():
     rc = R.lib.rvvla_hcsr_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), col.data_ptr(),
                                       val.data_ptr(), 300, 70000, 2.0, None, C.byref(h))
     assert rc == -10 and not h.value  # C above 65536 (16-bit col_idx)
+
+
+# ---------------------------------------------------------------- CSR5
+CSR5_FIELDS = ("_csr5_tile_pointer", "_csr5_tile_descriptor", "_csr5_tile_empty_offset_pointer",
+               "_csr5_tile_empty_offset", "_csr_column_index")
+
+
+def _csr5_long_case():
+    """Rows spanning many tiles, a row from the full tiles into the tail and
+    long and short tail rows, with empty rows between (sigma 4: 256-entry
+    tiles)."""
+    T = 256
+    lens = [3, 10 * T - 3, 5, 9 * T + 7, 0, 0, 12, 1, 9, 0, 3 * T + 1, 17 * T + 11, 0]
+    lens.append(T - sum(lens) % T + T // 2)
+    lens += [1, 0, 8, 9, 40]
+    rng = np.random.default_rng(3)
+    rst = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.sort(rng.choice(40000, l, replace=False)) for l in lens]).astype(np.int32)
+    return R.CSR(len(lens), 40000, rst, col, rng.normal(size=int(rst[-1])))
+
+
+def _case5(name):
+    return _csr5_long_case() if name == "long_segments" else case(name)
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1", "long_segments"])
+@pytest.mark.parametrize("sigma", [0, 4, 12, 16, 64])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_device_csr5_equals_host(name, sigma, dtype):
+    """The CSR5 layout built on the device equals the host builder's bit for
+    bit: tile pointers (with the empty-row flag), descriptors, empty
+    offsets, transposed columns and values."""
+    a = _case5(name)
+    print("device csr5", name, sigma, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.csr5_from_device(dtype, a.m, a.n, rst, col, val, sigma)
+    torch.cuda.synchronize()
+    L = R.convert("csr5", a, dtype, f"sigma={sigma}" if sigma > 0 else "")
+    for f in CSR5_FIELDS:
+        h = L.array(f)
+        d = D.read(f)
+        d = d.view(h.dtype) if d.size else d
+        assert d.shape == h.shape and np.array_equal(d, h), f
+    hv = L.array("_csr_value", np.float64 if dtype == R.F64 else np.float32)
+    dv = D.read("_csr_value")
+    if dtype in (R.F64, R.F32):
+        assert np.array_equal(dv.view(hv.dtype), hv)
+    else:
+        assert np.array_equal(dv, _round16(hv, dtype))
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "long_segments"])
+@pytest.mark.parametrize("sigma", [0, 4, 12])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_csr5_product(name, sigma, dtype):
+    a = _case5(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.csr5_from_device(dtype, a.m, a.n, rst, col, val, sigma)
+    dev = _device()
+    x64, y064 = inputs(a, 29)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, -0.5, 1.25)
+    torch.cuda.synchronize()
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), -0.5, 1.25, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), -0.5, 1.25,
+                        y064.astype(dt), dt, f"device-built CSR5 {name} sigma={sigma}")
+
+
+def test_device_csr5_refuses_bad_input():
+    a = case("rand_long")
+    rst, col, val = _device_csr(a, R.F64)
+    bad = col.clone()
+    bad[10] = -1
+    h = C.c_void_p()
+    rc = R.lib.rvvla_csr5_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), bad.data_ptr(),
+                                      val.data_ptr(), 16, None, C.byref(h))
+    assert rc == -10 and not h.value
+    rc = R.lib.rvvla_csr5_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), col.data_ptr(),
+                                      val.data_ptr(), 65, None, C.byref(h))
+    assert rc == -3 and not h.value  # sigma above 64
