@@ -141,6 +141,19 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int sigma, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> VNEC (the compaction of reference include/VNEC.h:
+ * 78-227, which the reference computes per call on the host, for a CSR
+ * already on the device).  nparts merge-path partitions (<= 0: as many
+ * rows + entries per partition as one workgroup stages, the default of
+ * convert(RVVLA_FMT_VNEC)); per partition the ascending touched-column list
+ * and every entry's rank in it, the same arrays as convert + upload
+ * (readable with rvvla_mat_read: "cx", "cy" -- the merge-path coordinates
+ * -- "nec_ptr", "use_x_indices", "ecr_indices").  The IRD variant selector
+ * is not evaluated (one kernel serves its three variants). */
+int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                           int64_t nparts, void* stream, rvvla_mat_t* out);
+
 /* Device-side CSR -> CSR5 (omega = 64; replaces spMtxCSR5::inputCSR +
  * asCSR5, declared without a body in reference include/CSR5.h:61-65, for a
  * CSR already on the device).  Binary searches, scans and one wavefront per
@@ -169,7 +182,8 @@ int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
 
 /* Copy of one array of an uploaded matrix into host memory (device or
  * host-validation matrices): SELL "vCl", "vCs", "vBs", "vSBs", "vCol",
- * "vPerm", "vVal"; CSR "rst", "col", "val"; CSR5 "_csr5_tile_pointer",
+ * "vPerm", "vVal"; CSR "rst", "col", "val"; VNEC "cx", "cy", "nec_ptr",
+ * "use_x_indices", "ecr_indices"; CSR5 "_csr5_tile_pointer",
  * "_csr5_tile_descriptor", "_csr5_tile_empty_offset_pointer",
  * "_csr5_tile_empty_offset", "_csr_column_index", "_csr_value"; HCSR built
  * on the device "block_ptr", "row_offset", "row_ptr", "col_idx", "values".

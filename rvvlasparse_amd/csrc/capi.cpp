@@ -918,6 +918,7 @@ rvvla_mat_t upload_vnec(int dtype, const spMtxVNEC<V>& a, const spMtxCRS<V>& src
   d.cy = M->put_vec(cy);
   d.nec_ptr = M->put_vec(a.nec_ptr);
   d.use_x = M->put_vec(a.use_x_indices);
+  d.nlist = (int64_t)a.use_x_indices.size();
   d.ecr = M->put_vec(ecr);
   d.rst = (const int32_t*)M->put(src.Rst, (size_t)(m + 1) * 4);
   d.val = put_values(*M, dtype, src.Val, (size_t)nnz);
@@ -1367,6 +1368,36 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                           const int32_t* d_col, const void* d_val, int64_t nparts, void* stream,
+                           rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (nparts <= 0) {  // the default: as many rows + entries per partition as one workgroup stages
+      const int64_t cap = rvvla::vnec_items(dtype == RVVLA_F64 ? 8 : 4);
+      nparts = std::max<int64_t>(1, (m + nnz + cap - 1) / cap);
+    }
+    if (nparts > 2147483647LL) throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: too many partitions");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_VNEC, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_vnec_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, nparts, s)
+        : rvvla::build_vnec_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, nparts, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_csr5_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
                            const int32_t* d_col, const void* d_val, int sigma, void* stream,
                            rvvla_mat_t* out) {
@@ -1464,6 +1495,11 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     const rvvla::DevCSR& c = A->csr;
     v = f == "rst" ? V{c.rst, A->m + 1, 4} : f == "col" ? V{c.col, A->nnz, 4}
       : f == "val" ? V{c.val, A->nnz, vb} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_VNEC) {
+    const rvvla::DevVNEC& c = A->vnec;
+    v = f == "cx" ? V{c.cx, c.nparts + 1, 4} : f == "cy" ? V{c.cy, c.nparts + 1, 4}
+      : f == "nec_ptr" ? V{c.nec_ptr, c.nparts + 1, 4} : f == "use_x_indices" ? V{c.use_x, c.nlist, 4}
+      : f == "ecr_indices" ? V{c.ecr, A->nnz, 2} : V{nullptr, -1, 0};
   } else if (A->format == RVVLA_FMT_CSR5) {
     const rvvla::DevCSR5& c = A->csr5;
     v = f == "_csr5_tile_pointer" ? V{c.tile_ptr, c.np + 1, 4}

@@ -198,6 +198,7 @@ struct DevVNEC {
   const int32_t* cy = nullptr;       // nparts + 1 merge-path entry coordinates
   const int32_t* nec_ptr = nullptr;  // nparts + 1
   const int32_t* use_x = nullptr;
+  int64_t nlist = 0;                 // entries of use_x
   const uint16_t* ecr = nullptr;     // compacted column rank per entry
   const int32_t* rst = nullptr;
   const void* val = nullptr;
@@ -305,6 +306,11 @@ int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 template <class V>
 int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int sigma, hipStream_t s);
+// CSR -> VNEC on the device into M.vnec (k_convert_vnec.hip): the host
+// builder's partitions, touched-column lists and ranks.
+template <class V>
+int build_vnec_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
+                      const int32_t* col, const V* val, int64_t nparts, hipStream_t s);
 // CSR -> CSR5 (omega = 64) on the device into M.csr5 (k_convert_csr5.hip):
 // the host builder's layout bit for bit, plus the kernels' extras.
 template <class V>

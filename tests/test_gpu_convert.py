@@ -1,10 +1,11 @@
 """Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device),
-CSR -> HCSR (rvvla_hcsr_from_device) and CSR -> CSR5
-(rvvla_csr5_from_device).  The arrays read back from the device equal the
-host builders' bit for bit (SELL: for the default sigma rule and explicit
-sigmas; HCSR: the reference layout for several block sizes and p; CSR5:
-tile pointers, descriptors, empty offsets and transposed entries for
-several sigmas), in every precision; the products match the oracle; a bad
+CSR -> HCSR (rvvla_hcsr_from_device), CSR -> CSR5 (rvvla_csr5_from_device)
+and CSR -> VNEC (rvvla_vnec_from_device).  The arrays read back from the
+device equal the host builders' bit for bit (SELL: for the default sigma
+rule and explicit sigmas; HCSR: the reference layout for several block
+sizes and p; CSR5: tile pointers, descriptors, empty offsets and transposed
+entries for several sigmas; VNEC: merge-path coordinates, touched-column
+lists and ranks), in every precision; the products match the oracle; a bad
 device CSR is refused."""
 from __future__ import annotations
 
@@ -246,3 +247,61 @@ def test_device_csr5_refuses_bad_input():
     rc = R.lib.rvvla_csr5_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), col.data_ptr(),
                                       val.data_ptr(), 65, None, C.byref(h))
     assert rc == -3 and not h.value  # sigma above 64
+
+
+# ---------------------------------------------------------------- VNEC
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1"])
+@pytest.mark.parametrize("threads", [0, 97])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_vnec_equals_host(name, threads, dtype):
+    """Merge-path coordinates, touched-column lists and ranks built on the
+    device equal the host builder's."""
+    a = case(name)
+    cap = 1024 if dtype == R.F64 else 2048
+    if threads and -(-(a.m + a.nnz) // threads) > cap:
+        threads = 0  # partitions larger than one workgroup stages: the default count
+    print("device vnec", name, threads, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.vnec_from_device(dtype, a.m, a.n, rst, col, val, threads)
+    torch.cuda.synchronize()
+    L = R.convert("vnec", a, dtype, f"threads={threads}" if threads else "")
+    st = L.array("thread_coord_start").view(np.int32).reshape(-1, 2)
+    en = L.array("thread_coord_end").view(np.int32).reshape(-1, 2)
+    cx = np.concatenate([st[:, 0], en[-1:, 0]])
+    cy = np.concatenate([st[:, 1], en[-1:, 1]])
+    assert np.array_equal(D.read("cx").view(np.int32), cx)
+    assert np.array_equal(D.read("cy").view(np.int32), cy)
+    assert np.array_equal(D.read("nec_ptr").view(np.int32), L.array("nec_ptr").view(np.int32))
+    ux = D.read("use_x_indices")
+    assert np.array_equal(ux.view(np.int32) if ux.size else ux,
+                          L.array("use_x_indices").view(np.int32))
+    assert np.array_equal(D.read("ecr_indices").astype(np.int64),
+                          L.array("ecr_indices").view(np.int32).astype(np.int64))
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_device_vnec_product(name, dtype):
+    a = case(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.vnec_from_device(dtype, a.m, a.n, rst, col, val)
+    dev = _device()
+    x64, y064 = inputs(a, 31)
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, 1.0, 0.5)
+    torch.cuda.synchronize()
+    if dtype == R.BF16:
+        x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+        y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 1.0, 0.5,
+                         y032.astype(np.float64))
+        if np.linalg.norm(y64) > 0:
+            assert np.linalg.norm(y.cpu().numpy() - y64) / np.linalg.norm(y64) <= 5e-3
+        return
+    dt = np.float64 if dtype == R.F64 else np.float32
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.0, 0.5, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.0, 0.5,
+                        y064.astype(dt), dt, f"device-built VNEC {name}")
