@@ -161,7 +161,7 @@ int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
  * entries -- bit for bit what convert(RVVLA_FMT_CSR5) gives for the same
  * sigma (readable with rvvla_mat_read under the spMtxCSR5 field names) --
  * and the kernels' extras, on `stream`, synchronously.  sigma <= 0: the
- * default (16).  The CSR is checked on the device first. */
+ * default (8).  The CSR is checked on the device first. */
 int rvvla_csr5_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int sigma, void* stream, rvvla_mat_t* out);
