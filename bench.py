@@ -100,12 +100,15 @@ def copy_probe(torch, nbytes=1 << 30, steps=20):
 
 
 def cpu_baseline(a, x64, byts, seconds=10.0):
-    """Scalar OpenMP CSR restatement of the reference (reference
-    src/scalar/spmv_mtx_crs.cpp:37-43), FP64, on the host cores, in a child
-    process with OMP_NUM_THREADS = the physical cores allowed to this
-    process, OMP_PROC_BIND=close, OMP_PLACES=cores: the median of as many
-    full Laplacian SpMVs as fit in `seconds` (at least 20) after 5 warm-ups;
-    then the same on one thread for about a third of that time."""
+    """The reference's scalar OpenMP CSR SpMV (reference
+    src/scalar/spmv_mtx_crs.cpp:22-71, compiled from its sources into
+    oracle/_ref with the reference's flags; kind "reference"), or where that
+    library is absent the oracle's restatement of it (kind "port"), FP64, on
+    the host cores, in a child process with OMP_NUM_THREADS = the physical
+    cores allowed to this process, OMP_PROC_BIND=close, OMP_PLACES=cores:
+    the median of as many full Laplacian SpMVs as fit in `seconds` (at least
+    20) after 5 warm-ups; then the same on one thread for about a third of
+    that time."""
     cores = physical_cores(HOST_CPUS)
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
@@ -118,8 +121,8 @@ def cpu_baseline(a, x64, byts, seconds=10.0):
         res = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-child", path,
                               "--cpu-seconds", str(seconds)],
                              env=env, stdout=subprocess.PIPE, text=True, timeout=600, check=True)
-    med, cnt, med1, cnt1 = json.loads(res.stdout.strip().splitlines()[-1])
-    return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+    med, cnt, med1, cnt1, kind = json.loads(res.stdout.strip().splitlines()[-1])
+    return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": kind,
             "sample": f"full Laplacian FP64 SpMV, median of {cnt} runs after 5 warm-ups "
                       f"({med * 1e3:.3f} ms each, {cores} threads, one per physical core, "
                       f"OMP_PROC_BIND=close)",
@@ -135,25 +138,39 @@ def cpu_child(path, seconds):
         import importlib
         importlib.reload(O)
     d = np.load(path)
-    rst, col, val, x = d["rst"], d["col"], d["val"], d["x"]
+    rst, col, val, x = (np.ascontiguousarray(d[k]) for k in ("rst", "col", "val", "x"))
     m = len(rst) - 1
+    if O.ref is not None:  # the reference library: matrix and vectors built once
+        kind = "reference"
+        h = O.ref.ref_csr_bench_new_f64(m, len(x), len(col), rst.ctypes.data, col.ctypes.data,
+                                        val.ctypes.data, x.ctypes.data)
+
+        def run():
+            O.ref.ref_csr_bench_run_f64(h)
+    else:
+        kind = "port"
+        y = np.zeros(m)
+
+        def run():
+            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
 
     def sample(threads, budget):
-        O.orc.orc_set_threads(threads)
-        y = np.zeros(m)
+        O.orc.orc_set_threads(threads)  # the process's OpenMP thread count
         for _ in range(5):
-            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
+            run()
         ts = []
         t_end = time.perf_counter() + budget
         while len(ts) < 20 or time.perf_counter() < t_end:
             t0 = time.perf_counter()
-            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
+            run()
             ts.append(time.perf_counter() - t0)
         return statistics.median(ts), len(ts)
 
     med, cnt = sample(int(os.environ.get("OMP_NUM_THREADS", "1")), seconds)
     med1, cnt1 = sample(1, seconds / 3)
-    print(json.dumps([med, cnt, med1, cnt1]), flush=True)
+    if kind == "reference":
+        O.ref.ref_csr_bench_free(h)
+    print(json.dumps([med, cnt, med1, cnt1, kind]), flush=True)
 
 
 def powerlaw_leg(R, torch, steps, warmup, fmts):

@@ -94,6 +94,9 @@ _REF_SIGS = {
                                  _vp, _vp, _d, _d, _vp]),
     "ref_sell_spmv_f32": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                  _vp, _vp, _f, _f, _vp]),
+    "ref_csr_bench_new_f64": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "ref_csr_bench_run_f64": (_i32, [_vp]),
+    "ref_csr_bench_free": (None, [_vp]),
 }
 
 orc = _load(ORACLE_SO, _ORC_SIGS) if ORACLE_SO.exists() else None

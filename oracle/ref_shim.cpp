@@ -48,6 +48,27 @@ int ref_csr_spmv_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const in
   return 0;
 }
 
+// The reference CSR SpMV on a matrix and vectors built once, for timing
+// (bench.py's CPU baseline): new (copies the matrix and b in), run (one
+// y = A*b through sparse_mv<double, spMtxCRS, false>, reference
+// src/scalar/spmv_mtx_crs.cpp:22-71), free.
+struct RefCsrBench {
+  spMtxCRS<double> a;
+  std::vector<double> b, y;
+};
+void* ref_csr_bench_new_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                            const double* val, const double* b) {
+  return new RefCsrBench{make_crs<double>(m, n, nnz, rst, col, val),
+                         std::vector<double>(b, b + n), std::vector<double>((size_t)m, 0.0)};
+}
+int ref_csr_bench_run_f64(void* h) {
+  auto* p = static_cast<RefCsrBench*>(h);
+  sparse_mv<double, spMtxCRS, false>(SPARSE_OPERATION_NON_TRANSPOSE, 1.0, p->a,
+                                     sparse_matrix_descr{}, p->b, 0.0, p->y);
+  return 0;
+}
+void ref_csr_bench_free(void* h) { delete static_cast<RefCsrBench*>(h); }
+
 // reference src/scalar/crs_to_hcsr.cpp:137-169 through convert<T, spMtxHCSR, false>.
 // Returns a handle; *err = 0, or -10/-11/-12 for the three exception kinds.
 static void* hcsr_convert_any(int dtype, int64_t m, int64_t n, int64_t nnz, const int* rst,
