@@ -141,6 +141,19 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int sigma, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> LAV (SIMD_Lanes 64; the layout of reference
+ * include/LAV.h:10-94 that the reference leaves to the caller to build, for
+ * a CSR already on the device).  Column histogram, stable radix sorts of
+ * columns and rows, scans and scatters give the layout of
+ * convert(RVVLA_FMT_LAV) with the same T and segment_len (T < 0 and
+ * segment_len <= 0: the convert defaults 0.2 and 2^20), readable with
+ * rvvla_mat_read under the spMtxLAV field names ("chunk_offsets",
+ * "chunk_offsets_full_mask", "segment_ptr", "out_order", "mask", "val",
+ * "col_id", "sparse_part.Rst" / ".Col" / ".Val"), and the kernels' image. */
+int rvvla_lav_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                          const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                          double T, int64_t segment_len, void* stream, rvvla_mat_t* out);
+
 /* Device-side CSR -> VNEC (the compaction of reference include/VNEC.h:
  * 78-227, which the reference computes per call on the host, for a CSR
  * already on the device).  nparts merge-path partitions (<= 0: as many

@@ -79,6 +79,8 @@ _sigs = {
                                       C.POINTER(_vp)]),
     "rvvla_vnec_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp,
                                       C.POINTER(_vp)]),
+    "rvvla_lav_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _d, _i64, _vp,
+                                     C.POINTER(_vp)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -335,6 +337,17 @@ def vnec_from_device(dtype: int, m: int, n: int, rst, col, val, nparts: int = 0,
     nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
     check(lib.rvvla_vnec_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), nparts,
                                      _stream(stream, val), C.byref(h)))
+    return DeviceMatrix(h)
+
+
+def lav_from_device(dtype: int, m: int, n: int, rst, col, val, T: float = -1.0,
+                    segment_len: int = 0, stream=None) -> DeviceMatrix:
+    """LAV built on the device from device CSR arrays (torch tensors or raw
+    addresses).  T < 0, segment_len <= 0: the convert defaults."""
+    h = C.c_void_p()
+    nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
+    check(lib.rvvla_lav_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), T,
+                                    segment_len, _stream(stream, val), C.byref(h)))
     return DeviceMatrix(h)
 
 

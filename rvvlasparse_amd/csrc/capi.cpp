@@ -762,6 +762,48 @@ rvvla_mat_t upload_vhcc(int dtype, const spMtxVHCC<V>& a) {
   return M.release();
 }
 
+// Chunks with more than kLavPiece steps over all segments -> pieces of at
+// most kLavPiece steps (within one segment), from the chunk offsets
+// (S * (cc + 1), per segment) and the segment bases (in steps).
+static void attach_lav_pieces(rvvla_mat_s& M, int64_t S, int64_t cc, const int* coff,
+                              const int64_t* sptr) {
+  DevLAV& d = M.lav;
+  std::vector<int32_t> whole, pc_chunk, sp_chunk, sp_first{0};
+  std::vector<int64_t> pc_k0, pc_k1;
+  for (int64_t c = 0; c < cc; ++c) {
+    int64_t steps = 0;
+    for (int64_t g = 0; g < S; ++g)
+      steps += coff[(size_t)(g * (cc + 1) + c + 1)] - coff[(size_t)(g * (cc + 1) + c)];
+    if (steps <= kLavPiece) {
+      whole.push_back((int32_t)c);
+      continue;
+    }
+    for (int64_t g = 0; g < S; ++g) {
+      const int64_t k0 = sptr[(size_t)g] + coff[(size_t)(g * (cc + 1) + c)];
+      const int64_t k1 = sptr[(size_t)g] + coff[(size_t)(g * (cc + 1) + c + 1)];
+      for (int64_t k = k0; k < k1; k += kLavPiece) {
+        pc_chunk.push_back((int32_t)c);
+        pc_k0.push_back(k);
+        pc_k1.push_back(std::min(k1, k + kLavPiece));
+      }
+    }
+    sp_chunk.push_back((int32_t)c);
+    sp_first.push_back((int32_t)pc_chunk.size());
+  }
+  d.nwhole = (int64_t)whole.size();
+  if (!sp_chunk.empty()) {
+    d.whole = M.put_vec(whole);
+    d.npieces = (int64_t)pc_chunk.size();
+    d.nsplit = (int64_t)sp_chunk.size();
+    d.pc_chunk = M.put_vec(pc_chunk);
+    d.pc_k0 = M.put_vec(pc_k0);
+    d.pc_k1 = M.put_vec(pc_k1);
+    d.sp_chunk = M.put_vec(sp_chunk);
+    d.sp_first = M.put_vec(sp_first);
+    d.part = M.put(nullptr, (size_t)d.npieces * 64 * (M.dtype == RVVLA_F64 ? 8 : 4));
+  }
+}
+
 template <class V>
 rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
@@ -831,41 +873,7 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
                                         S > 0 ? (size_t)(cc * L) * 4 : 0);
   d.col = M->put_vec(a.col_id);
   d.val = put_values(*M, dtype, vals.data(), vals.size());
-  // long chunks -> pieces of at most kLavPiece steps
-  std::vector<int32_t> whole, pc_chunk, sp_chunk, sp_first{0};
-  std::vector<int64_t> pc_k0, pc_k1;
-  for (int64_t c = 0; c < cc; ++c) {
-    int64_t steps = 0;
-    for (int64_t g = 0; g < S; ++g)
-      steps += a.chunk_offsets[(size_t)(g * (cc + 1) + c + 1)] - a.chunk_offsets[(size_t)(g * (cc + 1) + c)];
-    if (steps <= kLavPiece) {
-      whole.push_back((int32_t)c);
-      continue;
-    }
-    for (int64_t g = 0; g < S; ++g) {
-      const int64_t k0 = sptr[(size_t)g] + a.chunk_offsets[(size_t)(g * (cc + 1) + c)];
-      const int64_t k1 = sptr[(size_t)g] + a.chunk_offsets[(size_t)(g * (cc + 1) + c + 1)];
-      for (int64_t k = k0; k < k1; k += kLavPiece) {
-        pc_chunk.push_back((int32_t)c);
-        pc_k0.push_back(k);
-        pc_k1.push_back(std::min(k1, k + kLavPiece));
-      }
-    }
-    sp_chunk.push_back((int32_t)c);
-    sp_first.push_back((int32_t)pc_chunk.size());
-  }
-  d.nwhole = (int64_t)whole.size();
-  if (!sp_chunk.empty()) {
-    d.whole = M->put_vec(whole);
-    d.npieces = (int64_t)pc_chunk.size();
-    d.nsplit = (int64_t)sp_chunk.size();
-    d.pc_chunk = M->put_vec(pc_chunk);
-    d.pc_k0 = M->put_vec(pc_k0);
-    d.pc_k1 = M->put_vec(pc_k1);
-    d.sp_chunk = M->put_vec(sp_chunk);
-    d.sp_first = M->put_vec(sp_first);
-    d.part = M->put(nullptr, (size_t)d.npieces * 64 * (dtype == RVVLA_F64 ? 8 : 4));
-  }
+  attach_lav_pieces(*M, S, cc, a.chunk_offsets.data(), sptr.data());
   return M.release();
 }
 
@@ -1368,6 +1376,73 @@ int rvvla_sell_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_lav_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                          const int32_t* d_col, const void* d_val, double T, int64_t segment_len,
+                          void* stream, rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (T < 0.0) T = 0.2;  // the convert default
+    if (segment_len <= 0) segment_len = 1 << 20;
+    if (!(T <= 1.0)) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: T in [0, 1]");
+    if (segment_len > 2147483647LL) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: segment_len");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_LAV, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    rvvla::LavDeviceHost h;
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_lav_device<double>(*M, m, n, nnz, d_rst, d_col, (const double*)d_val, T, segment_len, h, s)
+        : rvvla::build_lav_device<float>(*M, m, n, nnz, d_rst, d_col, (const float*)d_val, T, segment_len, h, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    M->free_temps();
+    // sparse_part: CSR image (row blocks from its row pointers, on the host)
+    std::vector<int32_t> srst((size_t)m + 1);
+    int rc2 = rvvla::hip_check(hipMemcpy(srst.data(), h.srst, srst.size() * 4, hipMemcpyDeviceToHost), "read");
+    if (rc2) throw Error(rc2, rvvla::last_error());
+    const std::vector<int32_t> blocks = rvvla::csr_row_blocks(srst.data(), 0, m);
+    M->csr.rst = h.srst;
+    M->csr.col = h.scol;
+    M->csr.val = h.sval;
+    M->csr.blocks = M->put_vec(blocks);
+    M->csr.nblocks = (int64_t)blocks.size() / 2;
+    rvvla::DevLAV& d = M->lav;
+    d.nseg = h.S;
+    d.nchunks = h.S > 0 ? h.cc : 0;
+    int64_t longest = 0;
+    for (int64_t r = 0; r < m; ++r) longest = std::max<int64_t>(longest, srst[(size_t)r + 1] - srst[(size_t)r]);
+    d.fused = h.S > 0 && longest <= rvvla::kLavFuseRow;
+    d.device_built = true;
+    if (h.S == 0) {  // segment_ptr of an empty dense part: one 0
+      const int64_t zero = 0;
+      d.sptr = (const int64_t*)M->put(&zero, 8);
+    } else {
+      d.coff = h.coff;
+      d.sptr = h.sptr;
+      d.out_order = h.out_order;
+      d.col = h.col_id;
+      d.val = h.dval;
+      d.ref_cfull = h.cfull;
+      d.ref_mask = h.mask;
+      d.total_steps = h.total;
+      std::vector<int32_t> coff((size_t)(h.S * (h.cc + 1)));
+      std::vector<int64_t> sptr((size_t)h.S + 1);
+      rc2 = rvvla::hip_check(hipMemcpy(coff.data(), h.coff, coff.size() * 4, hipMemcpyDeviceToHost), "read");
+      if (!rc2) rc2 = rvvla::hip_check(hipMemcpy(sptr.data(), h.sptr, sptr.size() * 8, hipMemcpyDeviceToHost), "read");
+      if (rc2) throw Error(rc2, rvvla::last_error());
+      rvvla::attach_lav_pieces(*M, h.S, h.cc, coff.data(), sptr.data());
+    }
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
                            const int32_t* d_col, const void* d_val, int64_t nparts, void* stream,
                            rvvla_mat_t* out) {
@@ -1495,6 +1570,24 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     const rvvla::DevCSR& c = A->csr;
     v = f == "rst" ? V{c.rst, A->m + 1, 4} : f == "col" ? V{c.col, A->nnz, 4}
       : f == "val" ? V{c.val, A->nnz, vb} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_LAV && A->lav.device_built) {
+    // built on the device: the layout of convert(RVVLA_FMT_LAV)
+    const rvvla::DevLAV& c = A->lav;
+    const int64_t S = c.nseg, cc = c.nchunks, tot = c.total_steps;
+    v = f == "chunk_offsets" ? V{c.coff, S * (cc + 1), 4}
+      : f == "chunk_offsets_full_mask" ? V{c.ref_cfull, S * cc, 4}
+      : f == "segment_ptr" ? V{c.sptr, S + 1, 8}
+      : f == "out_order" ? V{c.out_order, S * cc * 64, 4} : f == "mask" ? V{c.ref_mask, tot, 2}
+      : f == "val" ? V{c.val, tot * 64, vb} : f == "col_id" ? V{c.col, tot * 64, 4}
+      : f == "sparse_part.Rst" ? V{A->csr.rst, A->m + 1, 4}
+      : V{nullptr, -1, 0};
+    if (f == "sparse_part.Col" || f == "sparse_part.Val") {
+      int32_t last = 0;
+      int rc = A->host_only ? (last = A->csr.rst[A->m], 0)
+                            : rvvla::hip_check(hipMemcpy(&last, A->csr.rst + A->m, 4, hipMemcpyDeviceToHost), "read");
+      if (rc) return rc;
+      v = f == "sparse_part.Col" ? V{A->csr.col, last, 4} : V{A->csr.val, last, vb};
+    }
   } else if (A->format == RVVLA_FMT_VNEC) {
     const rvvla::DevVNEC& c = A->vnec;
     v = f == "cx" ? V{c.cx, c.nparts + 1, 4} : f == "cy" ? V{c.cy, c.nparts + 1, 4}

@@ -183,6 +183,13 @@ struct DevLAV {
   // entries are added by the lane that finishes its dense sum (one pass,
   // y written once) instead of by a CSR multiply before the dense kernels
   bool fused = false;
+  // built on the device (rvvla_lav_from_device): the rest of the layout,
+  // for read-back (chunk_offsets_full_mask, mask, all S copies of
+  // out_order) and the dense step count
+  bool device_built = false;
+  const int32_t* ref_cfull = nullptr;
+  const uint16_t* ref_mask = nullptr;
+  int64_t total_steps = 0;
 };
 constexpr int kLavFuseRow = 32;
 
@@ -306,6 +313,25 @@ int check_csr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 template <class V>
 int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int sigma, hipStream_t s);
+// CSR -> LAV (SIMD_Lanes 64) on the device (k_convert_lav.hip): the host
+// builder's layout in device memory owned by M; the caller attaches it.
+struct LavDeviceHost {
+  int64_t S = 0, cc = 0, total = 0, nsp = 0;
+  int32_t* srst = nullptr;
+  int32_t* scol = nullptr;
+  void* sval = nullptr;
+  uint32_t* out_order = nullptr;
+  int32_t* coff = nullptr;
+  int32_t* cfull = nullptr;
+  int64_t* sptr = nullptr;
+  uint32_t* col_id = nullptr;
+  void* dval = nullptr;
+  uint16_t* mask = nullptr;
+};
+template <class V>
+int build_lav_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const int32_t* rst,
+                     const int32_t* col, const V* val, double T, int64_t seg_len,
+                     LavDeviceHost& host, hipStream_t s);
 // CSR -> VNEC on the device into M.vnec (k_convert_vnec.hip): the host
 // builder's partitions, touched-column lists and ranks.
 template <class V>

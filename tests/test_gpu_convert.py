@@ -305,3 +305,62 @@ def test_device_vnec_product(name, dtype):
     y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.0, 0.5, y064.astype(dt))
     assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.0, 0.5,
                         y064.astype(dt), dt, f"device-built VNEC {name}")
+
+
+# ---------------------------------------------------------------- LAV
+LAV_FIELDS = ("chunk_offsets", "chunk_offsets_full_mask", "segment_ptr", "out_order", "mask",
+              "col_id", "sparse_part.Rst", "sparse_part.Col")
+
+
+def _lav_params(params):
+    kv = dict(p.split("=") for p in params.split(",")) if params else {}
+    return float(kv.get("T", -1.0)), int(kv.get("segment_len", 0))
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1"])
+@pytest.mark.parametrize("params", ["", "T=0", "T=0.5", "segment_len=1000", "T=0.1,segment_len=300"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_device_lav_equals_host(name, params, dtype):
+    """Column ranking, sparse_part, row order, chunk tables, masks and lanes
+    built on the device equal the host builder's bit for bit."""
+    a = case(name)
+    print("device lav", name, params, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.lav_from_device(dtype, a.m, a.n, rst, col, val, *_lav_params(params))
+    torch.cuda.synchronize()
+    L = R.convert("lav", a, dtype, params)
+    for f in LAV_FIELDS:
+        h = L.array(f)
+        d = D.read(f)
+        d = d.view(h.dtype) if d.size else d
+        assert d.shape == h.shape and np.array_equal(d, h), f
+    vt = np.float64 if dtype == R.F64 else np.float32
+    for f in ("val", "sparse_part.Val"):
+        hv = L.array(f, vt)
+        dv = D.read(f)
+        if dtype in (R.F64, R.F32):
+            assert np.array_equal(dv.view(hv.dtype) if dv.size else dv, hv), f
+        else:
+            assert np.array_equal(dv, _round16(hv, dtype)), f
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty"])
+@pytest.mark.parametrize("params", ["", "T=0", "segment_len=1000"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_lav_product(name, params, dtype):
+    a = case(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.lav_from_device(dtype, a.m, a.n, rst, col, val, *_lav_params(params))
+    dev = _device()
+    x64, y064 = inputs(a, 37)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, 1.5, -0.25)
+    torch.cuda.synchronize()
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.5, -0.25, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.5, -0.25,
+                        y064.astype(dt), dt, f"device-built LAV {name} {params}")
