@@ -167,6 +167,19 @@ int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int64_t nparts, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> VHCC (vec_size 64; the container of reference
+ * include/VHCC.h:43-112 that the reference leaves to the caller to fill,
+ * for a CSR already on the device).  nparts partitions of (nearly) equal
+ * vector counts (<= 0: the default of convert(RVVLA_FMT_VHCC)); per entry
+ * its row by binary search, forced row ends at partition ends, the
+ * overflow redirect of a continued row, a scan of the row-end flags and
+ * one ballot per vector give the arrays of convert + upload, readable with
+ * rvvla_mat_read: "start_vec", "rbase", "eor" (one row-end mask per
+ * vector), "row_arr", "fwd_row", "pass", "col", "val", "empty_rows". */
+int rvvla_vhcc_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                           const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                           int64_t nparts, void* stream, rvvla_mat_t* out);
+
 /* Device-side CSR -> CSR5 (omega = 64; replaces spMtxCSR5::inputCSR +
  * asCSR5, declared without a body in reference include/CSR5.h:61-65, for a
  * CSR already on the device).  Binary searches, scans and one wavefront per

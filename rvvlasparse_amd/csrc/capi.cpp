@@ -747,6 +747,8 @@ rvvla_mat_t upload_vhcc(int dtype, const spMtxVHCC<V>& a) {
   DevVHCC& d = M->vhcc;
   d.nparts = T;
   d.nempty = (int64_t)empty.size();
+  d.nvec = nvec;
+  d.nends = (int64_t)nends;
   d.start_vec = M->put_vec(sv);
   d.rbase = M->put_vec(rb);
   d.eor = M->put_vec(eor);
@@ -1473,6 +1475,33 @@ int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_vhcc_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                           const int32_t* d_col, const void* d_val, int64_t nparts, void* stream,
+                           rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (nparts <= 0) nparts = rvvla::default_parts(m, nnz, rvvla::kVhccPartEntries);  // the convert default
+    if (nparts > 2147483647LL) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: too many partitions");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_VHCC, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_vhcc_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, nparts, s)
+        : rvvla::build_vhcc_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, nparts, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_csr5_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
                            const int32_t* d_col, const void* d_val, int sigma, void* stream,
                            rvvla_mat_t* out) {
@@ -1593,6 +1622,14 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     v = f == "cx" ? V{c.cx, c.nparts + 1, 4} : f == "cy" ? V{c.cy, c.nparts + 1, 4}
       : f == "nec_ptr" ? V{c.nec_ptr, c.nparts + 1, 4} : f == "use_x_indices" ? V{c.use_x, c.nlist, 4}
       : f == "ecr_indices" ? V{c.ecr, A->nnz, 2} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_VHCC) {
+    const rvvla::DevVHCC& c = A->vhcc;
+    const int64_t T = c.nparts, slots = c.nvec * 64;
+    v = f == "start_vec" ? V{c.start_vec, T + 1, 4} : f == "rbase" ? V{c.rbase, T, 4}
+      : f == "eor" ? V{c.eor, c.nvec, 8} : f == "row_arr" ? V{c.row_arr, c.nends, 4}
+      : f == "fwd_row" ? V{c.fwd_row, T, 4} : f == "pass" ? V{c.pass, T, 1}
+      : f == "col" ? V{c.col, slots, 4} : f == "val" ? V{c.val, slots, vb}
+      : f == "empty_rows" ? V{c.empty_rows, c.nempty, 4} : V{nullptr, -1, 0};
   } else if (A->format == RVVLA_FMT_CSR5) {
     const rvvla::DevCSR5& c = A->csr5;
     v = f == "_csr5_tile_pointer" ? V{c.tile_ptr, c.np + 1, 4}

@@ -144,6 +144,7 @@ struct DevCVR {
 // every vector, the target rows, and the partition hand-over description.
 struct DevVHCC {
   int64_t nparts = 0, nempty = 0;
+  int64_t nvec = 0, nends = 0;         // vectors; row ends (row_arr length)
   const int32_t* start_vec = nullptr;  // nparts + 1
   const int32_t* rbase = nullptr;
   const uint64_t* eor = nullptr;       // one row-end mask per vector
@@ -336,6 +337,12 @@ int build_lav_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 // builder's partitions, touched-column lists and ranks.
 template <class V>
 int build_vnec_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
+                      const int32_t* col, const V* val, int64_t nparts, hipStream_t s);
+// CSR -> VHCC (vec_size = 64) on the device into M.vhcc
+// (k_convert_vhcc.hip): the host builder's partitions, row ends and
+// hand-over flags.
+template <class V>
+int build_vhcc_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int64_t nparts, hipStream_t s);
 // CSR -> CSR5 (omega = 64) on the device into M.csr5 (k_convert_csr5.hip):
 // the host builder's layout bit for bit, plus the kernels' extras.

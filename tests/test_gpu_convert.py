@@ -1,12 +1,15 @@
 """Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device),
-CSR -> HCSR (rvvla_hcsr_from_device), CSR -> CSR5 (rvvla_csr5_from_device)
-and CSR -> VNEC (rvvla_vnec_from_device).  The arrays read back from the
+CSR -> HCSR (rvvla_hcsr_from_device), CSR -> CSR5 (rvvla_csr5_from_device),
+CSR -> VNEC (rvvla_vnec_from_device), CSR -> LAV (rvvla_lav_from_device)
+and CSR -> VHCC (rvvla_vhcc_from_device).  The arrays read back from the
 device equal the host builders' bit for bit (SELL: for the default sigma
 rule and explicit sigmas; HCSR: the reference layout for several block
 sizes and p; CSR5: tile pointers, descriptors, empty offsets and transposed
 entries for several sigmas; VNEC: merge-path coordinates, touched-column
-lists and ranks), in every precision; the products match the oracle; a bad
-device CSR is refused."""
+lists and ranks; LAV: chunk tables, row order, masks, lanes and
+sparse_part; VHCC: partitions, row-end masks and targets, hand-over flags,
+padded entries and empty rows), in every precision; the products match the
+oracle; a bad device CSR is refused."""
 from __future__ import annotations
 
 import ctypes as C
@@ -364,3 +367,79 @@ def test_device_lav_product(name, params, dtype):
     y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 1.5, -0.25, y064.astype(dt))
     assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 1.5, -0.25,
                         y064.astype(dt), dt, f"device-built LAV {name} {params}")
+
+
+# ---------------------------------------------------------------- VHCC
+VHCC_FIELDS = ("start_vec", "rbase", "eor", "row_arr", "fwd_row", "pass", "col", "val",
+               "empty_rows")
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1"])
+@pytest.mark.parametrize("threads", [0, 1, 7, 100000])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16, R.F16])
+def test_device_vhcc_equals_host(name, threads, dtype):
+    """Partitions, row-end masks, row targets (overflow redirects included),
+    hand-over flags, padded entries and empty rows built on the device equal
+    convert + upload bit for bit."""
+    a = case(name)
+    print("device vhcc", name, threads, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.vhcc_from_device(dtype, a.m, a.n, rst, col, val, threads)
+    torch.cuda.synchronize()
+    H = R.convert("vhcc", a, dtype, f"threads={threads}" if threads else "").upload()
+    for f in VHCC_FIELDS:
+        d, h = D.read(f), H.read(f)
+        assert d.shape == h.shape and np.array_equal(d, h), f
+
+
+def test_device_vhcc_continued_rows():
+    """A row spanning several partitions: the middle partitions hand the
+    partial on (pass), the last forwards it to the row; the product matches
+    the oracle."""
+    a = case("one_long_row")
+    rst, col, val = _device_csr(a, R.F64)
+    D = R.vhcc_from_device(R.F64, a.m, a.n, rst, col, val, 64)
+    H = R.convert("vhcc", a, R.F64, "threads=64").upload()
+    assert np.array_equal(D.read("pass"), H.read("pass"))
+    assert D.read("pass").astype(np.int64).sum() > 0
+    dev = _device()
+    x64, y064 = inputs(a, 41)
+    x = torch.from_numpy(x64).to(dev)
+    y = torch.from_numpy(y064).to(dev)
+    D.spmv(x, y, 1.0, 0.5)
+    torch.cuda.synchronize()
+    y_ref = O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.5, y064)
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, a.val, x64, 1.0, 0.5, y064,
+                        np.float64, "device-built VHCC one_long_row")
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_vhcc_product(name, dtype):
+    a = case(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.vhcc_from_device(dtype, a.m, a.n, rst, col, val)
+    dev = _device()
+    x64, y064 = inputs(a, 43)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, -1.0, 2.0)
+    torch.cuda.synchronize()
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), -1.0, 2.0, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), -1.0, 2.0,
+                        y064.astype(dt), dt, f"device-built VHCC {name}")
+
+
+def test_device_vhcc_refuses_bad_csr():
+    a = case("lap64")
+    rst, col, val = _device_csr(a, R.F64)
+    bad = col.clone()
+    bad[5] = a.n
+    h = C.c_void_p()
+    rc = R.lib.rvvla_vhcc_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), bad.data_ptr(),
+                                      val.data_ptr(), 0, None, C.byref(h))
+    assert rc == -10 and not h.value
