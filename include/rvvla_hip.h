@@ -180,6 +180,21 @@ int rvvla_vhcc_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
                            const int32_t* d_rst, const int32_t* d_col, const void* d_val,
                            int64_t nparts, void* stream, rvvla_mat_t* out);
 
+/* Device-side CSR -> CVR (SIMD_LEN 64; the container of reference
+ * include/CVR.h:22-81 that the reference leaves to the caller to fill, for a CSR
+ * already on the device).  nstreams streams of contiguous rows with
+ * balanced entry counts (<= 0: the default of convert(RVVLA_FMT_CVR)); one
+ * wavefront per stream replays the greedy lane packing (a count pass, scans,
+ * a write pass), giving the arrays of convert + upload, readable with
+ * rvvla_mat_read: "pack_start", "rec_start", "flush" (one mask per pack of
+ * the lanes that finish a row before it), "rec_rows", "tail_rows" (the rows
+ * flushed after a stream's last pack, -1: none), "col", "val",
+ * "empty_rows".  Layouts the host builder refuses (rows far longer than a
+ * stream's share) are refused with RVVLA_ERR_UNSUPPORTED_SPMV. */
+int rvvla_cvr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz,
+                          const int32_t* d_rst, const int32_t* d_col, const void* d_val,
+                          int64_t nstreams, void* stream, rvvla_mat_t* out);
+
 /* Device-side CSR -> CSR5 (omega = 64; replaces spMtxCSR5::inputCSR +
  * asCSR5, declared without a body in reference include/CSR5.h:61-65, for a
  * CSR already on the device).  Binary searches, scans and one wavefront per

@@ -83,6 +83,8 @@ _sigs = {
                                      C.POINTER(_vp)]),
     "rvvla_vhcc_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp,
                                       C.POINTER(_vp)]),
+    "rvvla_cvr_from_device": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp,
+                                     C.POINTER(_vp)]),
     "rvvla_mat_free": (_i32, [_vp]),
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
@@ -339,6 +341,17 @@ def vnec_from_device(dtype: int, m: int, n: int, rst, col, val, nparts: int = 0,
     nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
     check(lib.rvvla_vnec_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), nparts,
                                      _stream(stream, val), C.byref(h)))
+    return DeviceMatrix(h)
+
+
+def cvr_from_device(dtype: int, m: int, n: int, rst, col, val, nstreams: int = 0,
+                    stream=None) -> DeviceMatrix:
+    """CVR (SIMD_LEN 64) built on the device from device CSR arrays (torch
+    tensors or raw addresses).  nstreams <= 0: the default stream count."""
+    h = C.c_void_p()
+    nnz = int(col.numel()) if hasattr(col, "numel") else int(len(col))
+    check(lib.rvvla_cvr_from_device(dtype, m, n, nnz, _ptr(rst), _ptr(col), _ptr(val), nstreams,
+                                    _stream(stream, val), C.byref(h)))
     return DeviceMatrix(h)
 
 

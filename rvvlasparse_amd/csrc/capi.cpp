@@ -673,6 +673,8 @@ rvvla_mat_t upload_cvr(int dtype, const spMtxCVR<V>& a, int64_t nnz) {
   DevCVR& d = M->cvr;
   d.nstreams = S;
   d.nempty = (int64_t)empty.size();
+  d.npacks = npacks;
+  d.nrec = (int64_t)rec_rows.size();
   d.pack_start = M->put_vec(pstart);
   d.rec_start = M->put_vec(rstart);
   d.flush = M->put_vec(flush);
@@ -1475,6 +1477,33 @@ int rvvla_vnec_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
   });
 }
 
+int rvvla_cvr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
+                          const int32_t* d_col, const void* d_val, int64_t nstreams, void* stream,
+                          rvvla_mat_t* out) {
+  return guarded([&]() {
+    if (!out) throw Error(RVVLA_ERR_INVALID_ARG, "null output");
+    *out = nullptr;
+    if (m < 0 || n < 0 || nnz < 0 || m > 2147483646LL || n > 2147483647LL || nnz > 2147483647LL)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: shape out of the int32 range");
+    if (!d_rst || (nnz > 0 && (!d_col || !d_val)))
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: null device array");
+    if (dtype < RVVLA_F64 || dtype > RVVLA_F16) throw Error(RVVLA_ERR_VALUE_TYPE, "unknown dtype");
+    if (nstreams <= 0) nstreams = rvvla::default_parts(m, nnz, rvvla::kCvrStreamEntries);  // the convert default
+    if (nstreams > 2147483647LL) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: too many streams");
+    if (rvvla::g_host_validation.load())
+      throw Error(RVVLA_ERR_NO_DEVICE, "device-side conversion needs a device");
+    auto M = rvvla::new_mat(RVVLA_FMT_CVR, dtype, m, n, nnz);
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc = rvvla::check_csr_device(*M, m, n, nnz, d_rst, d_col, s)) throw Error(rc, rvvla::last_error());
+    const int rc = dtype == RVVLA_F64
+        ? rvvla::build_cvr_device<double>(*M, m, nnz, d_rst, d_col, (const double*)d_val, nstreams, s)
+        : rvvla::build_cvr_device<float>(*M, m, nnz, d_rst, d_col, (const float*)d_val, nstreams, s);
+    if (rc) throw Error(rc, rvvla::last_error());
+    *out = M.release();
+    return RVVLA_OK;
+  });
+}
+
 int rvvla_vhcc_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* d_rst,
                            const int32_t* d_col, const void* d_val, int64_t nparts, void* stream,
                            rvvla_mat_t* out) {
@@ -1622,6 +1651,14 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     v = f == "cx" ? V{c.cx, c.nparts + 1, 4} : f == "cy" ? V{c.cy, c.nparts + 1, 4}
       : f == "nec_ptr" ? V{c.nec_ptr, c.nparts + 1, 4} : f == "use_x_indices" ? V{c.use_x, c.nlist, 4}
       : f == "ecr_indices" ? V{c.ecr, A->nnz, 2} : V{nullptr, -1, 0};
+  } else if (A->format == RVVLA_FMT_CVR) {
+    const rvvla::DevCVR& c = A->cvr;
+    const int64_t S = c.nstreams;
+    v = f == "pack_start" ? V{c.pack_start, S + 1, 8} : f == "rec_start" ? V{c.rec_start, S, 4}
+      : f == "flush" ? V{c.flush, c.npacks, 8} : f == "rec_rows" ? V{c.rec_rows, c.nrec, 4}
+      : f == "tail_rows" ? V{c.tail_rows, S * 64, 4} : f == "col" ? V{c.col, c.npacks * 64, 4}
+      : f == "val" ? V{c.val, c.npacks * 64, vb} : f == "empty_rows" ? V{c.empty_rows, c.nempty, 4}
+      : V{nullptr, -1, 0};
   } else if (A->format == RVVLA_FMT_VHCC) {
     const rvvla::DevVHCC& c = A->vhcc;
     const int64_t T = c.nparts, slots = c.nvec * 64;

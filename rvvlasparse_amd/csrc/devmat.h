@@ -130,6 +130,7 @@ constexpr int kCsr5Serial = 8;
 // flush mask per pack plus the flushed rows in record order.
 struct DevCVR {
   int64_t nstreams = 0, nempty = 0;
+  int64_t npacks = 0, nrec = 0;         // packs; records in rec_rows
   const int64_t* pack_start = nullptr;  // nstreams + 1
   const int32_t* rec_start = nullptr;   // nstreams
   const uint64_t* flush = nullptr;      // one mask per pack
@@ -338,6 +339,11 @@ int build_lav_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
 template <class V>
 int build_vnec_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
                       const int32_t* col, const V* val, int64_t nparts, hipStream_t s);
+// CSR -> CVR (SIMD_LEN = 64) on the device into M.cvr (k_convert_cvr.hip):
+// the host builder's streams, packs and records.
+template <class V>
+int build_cvr_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst,
+                     const int32_t* col, const V* val, int64_t nstreams, hipStream_t s);
 // CSR -> VHCC (vec_size = 64) on the device into M.vhcc
 // (k_convert_vhcc.hip): the host builder's partitions, row ends and
 // hand-over flags.

@@ -1,14 +1,15 @@
 """Device-side conversion: CSR -> SELL-C-sigma (rvvla_sell_from_device),
 CSR -> HCSR (rvvla_hcsr_from_device), CSR -> CSR5 (rvvla_csr5_from_device),
-CSR -> VNEC (rvvla_vnec_from_device), CSR -> LAV (rvvla_lav_from_device)
-and CSR -> VHCC (rvvla_vhcc_from_device).  The arrays read back from the
+CSR -> VNEC (rvvla_vnec_from_device), CSR -> LAV (rvvla_lav_from_device),
+CSR -> VHCC (rvvla_vhcc_from_device) and CSR -> CVR (rvvla_cvr_from_device).  The arrays read back from the
 device equal the host builders' bit for bit (SELL: for the default sigma
 rule and explicit sigmas; HCSR: the reference layout for several block
 sizes and p; CSR5: tile pointers, descriptors, empty offsets and transposed
 entries for several sigmas; VNEC: merge-path coordinates, touched-column
 lists and ranks; LAV: chunk tables, row order, masks, lanes and
 sparse_part; VHCC: partitions, row-end masks and targets, hand-over flags,
-padded entries and empty rows), in every precision; the products match the
+padded entries and empty rows; CVR: streams, packs, flush masks, records,
+final-flush rows and empty rows, refusals included), in every precision; the products match the
 oracle; a bad device CSR is refused."""
 from __future__ import annotations
 
@@ -442,4 +443,68 @@ def test_device_vhcc_refuses_bad_csr():
     h = C.c_void_p()
     rc = R.lib.rvvla_vhcc_from_device(R.F64, a.m, a.n, a.nnz, rst.data_ptr(), bad.data_ptr(),
                                       val.data_ptr(), 0, None, C.byref(h))
+    assert rc == -10 and not h.value
+
+
+# ---------------------------------------------------------------- CVR
+CVR_FIELDS = ("pack_start", "rec_start", "flush", "rec_rows", "tail_rows", "col", "val",
+              "empty_rows")
+
+
+@pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",
+                                  "random16_c1"])
+@pytest.mark.parametrize("streams", [0, 1, 3, 8, 50])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16, R.F16])
+def test_device_cvr_equals_host(name, streams, dtype):
+    """Stream boundaries, packs, flush masks, records, final-flush rows and
+    empty rows built on the device equal convert + upload bit for bit; the
+    layouts the host builder refuses are refused with the same code."""
+    a = case(name)
+    print("device cvr", name, streams, dtype, flush=True)
+    rst, col, val = _device_csr(a, dtype)
+    params = f"streams={streams}" if streams else ""
+    try:
+        H = R.convert("cvr", a, dtype, params).upload()
+    except R.RvvlaError as e:
+        assert e.code == -4
+        with pytest.raises(R.RvvlaError) as d:
+            R.cvr_from_device(dtype, a.m, a.n, rst, col, val, streams)
+        assert d.value.code == -4
+        return
+    D = R.cvr_from_device(dtype, a.m, a.n, rst, col, val, streams)
+    torch.cuda.synchronize()
+    for f in CVR_FIELDS:
+        d, h = D.read(f), H.read(f)
+        assert d.shape == h.shape and np.array_equal(d, h), f
+
+
+@pytest.mark.parametrize("name,streams", [("lap64", 0), ("powerlaw", 8), ("rand_long", 0),
+                                          ("rand_long", 1), ("empty", 0), ("random16_c1", 0)])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_device_cvr_product(name, streams, dtype):
+    a = case(name)
+    rst, col, val = _device_csr(a, dtype)
+    D = R.cvr_from_device(dtype, a.m, a.n, rst, col, val, streams)
+    dev = _device()
+    x64, y064 = inputs(a, 47)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    D.spmv(x, y, 0.75, -1.0)
+    torch.cuda.synchronize()
+    v = a.values_for(dtype)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x64.astype(dt), 0.75, -1.0, y064.astype(dt))
+    assert_within_bound(y.cpu().numpy(), y_ref, a.rst, a.col, v, x64.astype(dt), 0.75, -1.0,
+                        y064.astype(dt), dt, f"device-built CVR {name} streams={streams}")
+
+
+def test_device_cvr_refuses_bad_csr():
+    a = case("lap64")
+    rst, col, val = _device_csr(a, R.F64)
+    bad = rst.clone()
+    bad[7] = bad[8] + 1  # row pointers out of order
+    h = C.c_void_p()
+    rc = R.lib.rvvla_cvr_from_device(R.F64, a.m, a.n, a.nnz, bad.data_ptr(), col.data_ptr(),
+                                     val.data_ptr(), 0, None, C.byref(h))
     assert rc == -10 and not h.value
