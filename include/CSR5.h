@@ -71,6 +71,9 @@ class spMtxCSR5 {
   mutable DeviceImage device;
 
   int format() const { return _format; }
+  /* Marks the tile arrays as the CSR5 layout when they were filled from a
+   * device-side build (convert with "device" = 1) instead of asCSR5. */
+  void setCSR5Format() { _format = CSR5_FORMAT_CSR5; }
   void freeMem() {
     _csr_row_pointer.clear(); _csr_column_index.clear(); _csr_value.clear();
     _csr5_tile_pointer.clear(); _csr5_tile_descriptor.clear();

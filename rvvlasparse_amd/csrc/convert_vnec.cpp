@@ -80,8 +80,15 @@ void build_vnec(spMtxVNEC<V>& d, const spMtxCRS<V>& s, int nparts) {
   for (int t = 0; t < nparts; ++t)
     std::copy(cols[(size_t)t].begin(), cols[(size_t)t].end(),
               d.use_x_indices.begin() + d.nec_ptr[(size_t)t]);
-  // IRD: lane groups over the compacted columns, partitions in order; rows
-  // [start.x, end.x) of a partition, each from max(row start, start.y)
+  vnec_select_variant(d, s);
+}
+
+// IRD: lane groups over the compacted columns, partitions in order; rows
+// [start.x, end.x) of a partition, each from max(row start, start.y)
+template <class V>
+void vnec_select_variant(spMtxVNEC<V>& d, const spMtxCRS<V>& s) {
+  const int64_t m = (int64_t)s.m, nnz = (int64_t)s.nz;
+  const int nparts = d.NUM_THREADS_VNEC;
   const int lanes = std::is_same<V, float>::value ? nLanes_f32 : nLanes_f64;
   int64_t groups = 0;
   for (int t = 0; t < nparts; ++t) {
@@ -104,6 +111,8 @@ void build_vnec(spMtxVNEC<V>& d, const spMtxCRS<V>& s, int nparts) {
   else d.vnec_t = SpB_VNEC_L;
 }
 
+template void vnec_select_variant<double>(spMtxVNEC<double>&, const spMtxCRS<double>&);
+template void vnec_select_variant<float>(spMtxVNEC<float>&, const spMtxCRS<float>&);
 template void build_vnec<double>(spMtxVNEC<double>&, const spMtxCRS<double>&, int);
 template void build_vnec<float>(spMtxVNEC<float>&, const spMtxCRS<float>&, int);
 

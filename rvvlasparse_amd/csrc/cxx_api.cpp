@@ -4,14 +4,17 @@
 // Mirrors the reference API (reference include/spmv_mtx.h:21-29,
 // include/convert.h:30-31): the same template names, arguments and
 // std::vector signature.  convert builds the format's host fields and
-// uploads its device image once (dist.device); sparse_mv uploads lazily when
-// a container was filled some other way.  Stages (reference
+// uploads its device image once (dist.device), or with "device" = 1 builds
+// the image on the device and reads the host fields back from it;
+// sparse_mv uploads lazily when a container was filled some other way.  Stages (reference
 // include/types.h:33-38): PREPARATION copies b and y to the device,
 // MULT runs the kernel, GET_RESULTS copies y back, ALL_STAGES does all
 // three.  Status codes are the RVVLA_* codes (0 = success).  The reference
 // ignores type_op; here SPARSE_OPERATION_TRANSPOSE computes A^T b for CRS
 // matrices and is answered with RVVLA_ERR_UNSUPPORTED_SPMV by the other
 // formats instead of computing the untransposed product.
+#include <omp.h>
+
 #include <algorithm>
 #include <type_traits>
 
@@ -177,21 +180,11 @@ double dparam(const convertParams& p, const char* k, double dflt) {
 
 }  // namespace
 
-#define RVVLA_CONVERT(T, FMT, BUILD, UPLOAD)                                                       \
-  template <>                                                                                      \
-  void convert<T, FMT, true>(FMT<T> & d, const spMtxCRS<T>& s, const convertParams& p) {           \
-    BUILD;                                                                                         \
-    d.device.reset(UPLOAD);                                                                        \
-  }
-
 // merge-path partitions of at most one workgroup's LDS staging each
 static int vnec_parts(int64_t items, size_t acc_bytes) {
   const int64_t cap = rvvla::vnec_items((int)acc_bytes);
   return (int)std::max<int64_t>(1, (items + cap - 1) / cap);
 }
-
-#define RVVLA_CONVERT_BOTH(FMT, BUILD, UPLOAD)                                                     \
-  RVVLA_CONVERT(double, FMT, BUILD, UPLOAD) RVVLA_CONVERT(float, FMT, BUILD, UPLOAD)
 
 // SELL-C-sigma keys "C" (64), "sigma", "device" (1: build on the device from
 // the uploaded CSR, then read the fields back; same arrays as the host build)
@@ -353,28 +346,324 @@ void convert<float, spMtxSELL_C_Sigma, true>(spMtxSELL_C_Sigma<float>& d,
                                              const spMtxCRS<float>& s, const convertParams& p) {
   convert_sell<float>(d, s, p);
 }
+// The remaining formats take "device" = 1 as SELL-C-sigma and HCSR do:
+// build on the device from the uploaded CSR, keep that image for the
+// products, and fill the container from it (the same arrays as the host
+// build).
+namespace {
+
+template <class T>
+int64_t field_count(rvvla_mat_t A, const char* f) {
+  int64_t cnt = 0;
+  int eb = 0;
+  const int rc = rvvla_mat_read(A, f, nullptr, 0, &cnt, &eb);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  return cnt;
+}
+
+template <class V, class T>
+std::vector<V> read_vec(rvvla_mat_t A, const char* f) {
+  std::vector<V> v((size_t)field_count<T>(A, f));
+  if (!v.empty()) read_field<T>(A, f, v.data(), v.size() * sizeof(V));
+  return v;
+}
+
+struct MatHandle {
+  rvvla_mat_t A = nullptr;
+  ~MatHandle() {
+    if (A) (void)rvvla_mat_free(A);
+  }
+  rvvla_mat_t release() {
+    rvvla_mat_t a = A;
+    A = nullptr;
+    return a;
+  }
+};
+
 // CSR5 keys "omega" (64), "sigma" (0 = chosen from the row lengths)
-RVVLA_CONVERT_BOTH(spMtxCSR5, rvvla::build_csr5(d, s, iparam(p, "omega", 64), iparam(p, "sigma", 0)),
-                   rvvla::upload_csr5(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+template <class T>
+void convert_csr5(spMtxCSR5<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int omega = iparam(p, "omega", 64);
+  int sigma = iparam(p, "sigma", 0);
+  if (!iparam(p, "device", 0)) {
+    rvvla::build_csr5(d, s, omega, sigma);
+    d.device.reset(rvvla::upload_csr5(dtype_of<T>(), d));
+    return;
+  }
+  if (omega != 64) throw Error(RVVLA_ERR_UNSUPPORTED_OMEGA, "CSR5: omega must be 64");
+  if (sigma <= 0) sigma = rvvla::default_csr5_sigma((int64_t)s.m, (int64_t)s.nz);
+  if (sigma > 64) throw Error(RVVLA_ERR_CONVERSION, "CSR5: sigma must be in [1, 64]");
+  DeviceCsr<T> dc(s);
+  MatHandle H;
+  int rc = rvvla_csr5_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                  (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val, sigma,
+                                  nullptr, &H.A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  d.freeMem();
+  rc = d.inputCSR((int)s.m, (int)s.n, (int)s.nz, s.Rst, s.Col, s.Val);
+  if (rc) throw Error(RVVLA_ERR_CONVERSION, "CSR5: inputCSR failed");
+  const int64_t nnz = (int64_t)s.nz, W = 64 * (int64_t)sigma;
+  d._omega = 64;
+  d._sigma = sigma;
+  d._p = nnz > 0 ? (int)((nnz + W - 1) / W) : 1;
+  int by = 0, bs = 0;
+  while ((int64_t(1) << by) < W) ++by;
+  while ((1 << bs) < 64) ++bs;
+  d._bit_y_offset = by;
+  d._bit_seg_offset = bs;
+  d._tile_desc_len = (by + bs + sigma + 31) / 32;
+  d._csr5_tile_pointer = read_vec<uint32_t, T>(H.A, "_csr5_tile_pointer");
+  d._csr5_tile_descriptor = read_vec<uint32_t, T>(H.A, "_csr5_tile_descriptor");
+  d._csr5_tile_empty_offset_pointer = read_vec<uint32_t, T>(H.A, "_csr5_tile_empty_offset_pointer");
+  d._csr5_tile_empty_offset = read_vec<uint32_t, T>(H.A, "_csr5_tile_empty_offset");
+  d._num_offsets = (int)d._csr5_tile_empty_offset.size();
+  d._csr_column_index = read_vec<uint32_t, T>(H.A, "_csr_column_index");
+  d._csr_value = read_vec<T, T>(H.A, "_csr_value");
+  d._tail_tile_start = (int)(d._csr5_tile_pointer[(size_t)d._p - 1] & 0x7FFFFFFFu);
+  d._temp_calibrator.assign((size_t)omp_get_max_threads() * (CSR5_CACHELINE / sizeof(T)), T(0));
+  d.setCSR5Format();
+  d.device.reset(H.release());
+}
+
 // LAV keys "SIMD_Lanes" (64), "T" (double, 0.2), "segment_len"
-RVVLA_CONVERT_BOTH(spMtxLAV,
-                   rvvla::build_lav(d, s, iparam(p, "SIMD_Lanes", 64), dparam(p, "T", 0.2),
-                                    iparam(p, "segment_len", 1 << 20)),
-                   rvvla::upload_lav(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+template <class T>
+void convert_lav(spMtxLAV<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int lanes = iparam(p, "SIMD_Lanes", 64);
+  const double Tv = dparam(p, "T", 0.2);
+  const int seg = iparam(p, "segment_len", 1 << 20);
+  if (!iparam(p, "device", 0) || lanes != 64) {
+    rvvla::build_lav(d, s, lanes, Tv, seg);
+    d.device.reset(rvvla::upload_lav(dtype_of<T>(), d));
+    return;
+  }
+  if (seg < 1) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: segment_len must be positive");
+  if (!(Tv >= 0.0 && Tv <= 1.0)) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: T in [0, 1]");
+  DeviceCsr<T> dc(s);
+  MatHandle H;
+  const int rc = rvvla_lav_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                       (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val, Tv,
+                                       seg, nullptr, &H.A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  d = spMtxLAV<T>();
+  d.SIMD_Lanes = 64;
+  d.m = (int)s.m;
+  d.n = (int)s.n;
+  d.nz = (int)s.nz;
+  d.T = Tv;
+  d.segment_len = seg;
+  d.chunk_count = (s.m + 63) / 64;
+  d.chunk_offsets = read_vec<int, T>(H.A, "chunk_offsets");
+  d.chunk_offsets_full_mask = read_vec<int, T>(H.A, "chunk_offsets_full_mask");
+  d.segment_ptr = read_vec<size_t, T>(H.A, "segment_ptr");
+  d.segment_count = d.segment_ptr.size() - 1;
+  d.out_order = read_vec<uint32_t, T>(H.A, "out_order");
+  d.mask = read_vec<uint16_t, T>(H.A, "mask");
+  d.val = read_vec<T, T>(H.A, "val");
+  d.col_id = read_vec<uint32_t, T>(H.A, "col_id");
+  const std::vector<int> srst = read_vec<int, T>(H.A, "sparse_part.Rst");
+  const std::vector<int> scol = read_vec<int, T>(H.A, "sparse_part.Col");
+  const std::vector<T> sval = read_vec<T, T>(H.A, "sparse_part.Val");
+  d.sparse_part = spMtxCRS<T>(s.m, s.n, scol.size(), srst.data(), scol.data(), sval.data());
+  d.device.reset(H.release());
+}
+
 // CVR keys "SIMD_LEN" (64), "streams"
-RVVLA_CONVERT_BOTH(spMtxCVR,
-                   rvvla::build_cvr(d, s, iparam(p, "SIMD_LEN", 64),
-                                    iparam(p, "streams", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kCvrStreamEntries))),
-                   rvvla::upload_cvr(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, (int64_t)s.nz))
+template <class T>
+void convert_cvr(spMtxCVR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int L = iparam(p, "SIMD_LEN", 64);
+  const int S = iparam(p, "streams",
+                       rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kCvrStreamEntries));
+  if (!iparam(p, "device", 0) || L != 64) {
+    rvvla::build_cvr(d, s, L, S);
+    d.device.reset(rvvla::upload_cvr(dtype_of<T>(), d, (int64_t)s.nz));
+    return;
+  }
+  if (S < 1) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: at least one stream");
+  DeviceCsr<T> dc(s);
+  MatHandle H;
+  const int rc = rvvla_cvr_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                       (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val, S,
+                                       nullptr, &H.A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  const std::vector<int64_t> ps = read_vec<int64_t, T>(H.A, "pack_start");
+  const std::vector<uint64_t> flush = read_vec<uint64_t, T>(H.A, "flush");
+  const std::vector<int32_t> rec_rows = read_vec<int32_t, T>(H.A, "rec_rows");
+  const std::vector<int32_t> tails = read_vec<int32_t, T>(H.A, "tail_rows");
+  d = spMtxCVR<T>();
+  d.SIMD_LEN = 64;
+  d.numRows = s.m;
+  d.numCols = s.n;
+  d.Nthreads = d.Nthreads_avail = S;
+  d.vPack_vec_vals = read_vec<T, T>(H.A, "val");
+  d.vPack_vec_cols = read_vec<uint32_t, T>(H.A, "col");
+  d.nItems = d.vPack_vec_vals.size();
+  // records: per stream the lanes of each pack's flush mask in lane order,
+  // then the rows flushed after its last pack
+  d.writeback_pos_record.assign((size_t)S * 66, -1);
+  size_t q = 0;
+  for (int t = 0; t < S; ++t) {
+    int* wb = d.writeback_pos_record.data() + (size_t)t * 66;
+    wb[0] = (int)ps[(size_t)t];
+    wb[1] = (int)(d.vPack_vec_record.size() / 2);
+    for (int64_t k = ps[(size_t)t]; k < ps[(size_t)t + 1]; ++k)
+      for (uint64_t f = flush[(size_t)k]; f; f &= f - 1) {
+        d.vPack_vec_record.push_back((int)(k * 64 + __builtin_ctzll(f)));
+        d.vPack_vec_record.push_back(rec_rows[q++]);
+      }
+    for (int l = 0; l < 64; ++l)
+      if (tails[(size_t)t * 64 + l] >= 0) {
+        d.vPack_vec_record.push_back((int)(ps[(size_t)t + 1] * 64 + l));
+        d.vPack_vec_record.push_back(tails[(size_t)t * 64 + l]);
+      }
+  }
+  d.vPack_vec_record.push_back((int)d.nItems);
+  d.vPack_vec_record.push_back(-1);
+  if (S == 1) {  // single stream: idle lanes name the last non-empty row
+    int last = -1;
+    for (int64_t r = (int64_t)s.m - 1; r >= 0; --r)
+      if (s.Rst[r + 1] > s.Rst[r]) {
+        last = (int)r;
+        break;
+      }
+    for (int l = 0; l < 64; ++l) d.writeback_pos_record[2 + (size_t)l] = last < 0 ? 0 : last;
+  }
+  d.device.reset(H.release());
+}
+
 // VHCC keys "vec_size" (64), "threads"
-RVVLA_CONVERT_BOTH(spMtxVHCC,
-                   rvvla::build_vhcc(d, s, iparam(p, "vec_size", 64),
-                                     iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz, rvvla::kVhccPartEntries))),
-                   rvvla::upload_vhcc(dtype_of<std::decay_t<decltype(*s.Val)>>(), d))
+template <class T>
+void convert_vhcc(spMtxVHCC<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int vec = iparam(p, "vec_size", 64);
+  const int threads = iparam(p, "threads", rvvla::default_parts((int64_t)s.m, (int64_t)s.nz,
+                                                                rvvla::kVhccPartEntries));
+  if (!iparam(p, "device", 0) || vec != 64) {
+    rvvla::build_vhcc(d, s, vec, threads);
+    d.device.reset(rvvla::upload_vhcc(dtype_of<T>(), d));
+    return;
+  }
+  if (threads < 1) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: at least one partition");
+  DeviceCsr<T> dc(s);
+  MatHandle H;
+  const int rc = rvvla_vhcc_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                        (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val,
+                                        threads, nullptr, &H.A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  const std::vector<int32_t> sv = read_vec<int32_t, T>(H.A, "start_vec");
+  const std::vector<int32_t> rbase = read_vec<int32_t, T>(H.A, "rbase");
+  const std::vector<uint64_t> eor = read_vec<uint64_t, T>(H.A, "eor");
+  const std::vector<int32_t> fwd = read_vec<int32_t, T>(H.A, "fwd_row");
+  const std::vector<uint8_t> pass = read_vec<uint8_t, T>(H.A, "pass");
+  const int64_t m = (int64_t)s.m, nnz = (int64_t)s.nz, Tn = (int64_t)rbase.size();
+  const int64_t nvec = (int64_t)eor.size();
+  d.freeMem();
+  d.set_params(vhcc_params((size_t)Tn, 1, 512, 512, 64));
+  d.set_properties((size_t)m, s.n, (size_t)nnz, (size_t)nvec, (size_t)(nvec * 64 - nnz),
+                   (size_t)(m + Tn));
+  d.get_vals_arr_reference() = read_vec<T, T>(H.A, "val");
+  d.get_col_arr_reference() = read_vec<uint32_t, T>(H.A, "col");
+  d.get_row_arr_reference() = read_vec<uint32_t, T>(H.A, "row_arr");
+  // row-end vectors and their masks: lane j adds lane j - 2^st in step st
+  // when no row ends in lanes [j - 2^st, j); the write mask is the row ends
+  auto& veceor = d.get_veceor_reference();
+  auto& masks = d.get_scan_mask_reference();
+  auto& thr = d.get_thr_info_reference();
+  thr.assign((size_t)Tn, thr_info_t{});
+  int last_row = -1;
+  for (int64_t t = 0; t < Tn; ++t) {
+    thr_info_t& I = thr[(size_t)t];
+    I.start_vec = sv[(size_t)t];
+    I.end_vec = sv[(size_t)t + 1];
+    I.panel_id = 0;
+    I.merge_start = (int)std::min<int64_t>((int64_t)I.start_vec * 64, nnz);
+    I.merge_end = (int)std::min<int64_t>((int64_t)I.end_vec * 64, nnz);
+    I.last_row = I.overflow_row = (int)(m + t);
+    // partition t continues a row iff t - 1 hands its partial on
+    if (t > 0 && (fwd[(size_t)t - 1] >= 0 || pass[(size_t)t - 1])) {
+      last_row = fwd[(size_t)t - 1] >= 0 ? fwd[(size_t)t - 1] : last_row;
+      I.last_row = last_row;
+    }
+    I.vbase = (int)veceor.size();
+    I.rbase = rbase[(size_t)t];
+    I.sbase = (int)(veceor.size() * 7);
+    for (int64_t v = I.start_vec; v < I.end_vec; ++v) {
+      const uint64_t E = eor[(size_t)v];
+      if (!E) continue;
+      veceor.push_back((uint32_t)v);
+      uint64_t G = E << 1;  // bit j: a row ends in lanes [j - span, j)
+      for (int st = 0; st <= 6; ++st) {
+        uint64_t mk = E;
+        if (st < 6) {
+          if (st > 0) G |= G << (1 << (st - 1));
+          mk = ~G & ~((uint64_t(1) << (1 << st)) - 1);
+        }
+        for (int b = 0; b < 8; ++b) masks.push_back((uint8_t)(mk >> (8 * b)));
+      }
+    }
+  }
+  veceor.push_back((uint32_t)nvec);
+  d.device.reset(H.release());
+}
+
 // VNEC key "threads" (merge-path partitions; default: as many rows+entries
 // each as one workgroup stages, 1024 for double, 2048 for float)
-RVVLA_CONVERT_BOTH(spMtxVNEC,
-                   rvvla::build_vnec(d, s, iparam(p, "threads", vnec_parts((int64_t)s.m + (int64_t)s.nz, sizeof(*s.Val)))),
-                   rvvla::upload_vnec(dtype_of<std::decay_t<decltype(*s.Val)>>(), d, s))
+template <class T>
+void convert_vnec(spMtxVNEC<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
+  const int nparts =
+      iparam(p, "threads", vnec_parts((int64_t)s.m + (int64_t)s.nz, sizeof(*s.Val)));
+  if (!iparam(p, "device", 0)) {
+    rvvla::build_vnec(d, s, nparts);
+    d.device.reset(rvvla::upload_vnec(dtype_of<T>(), d, s));
+    return;
+  }
+  if (nparts < 1) throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: at least one partition");
+  DeviceCsr<T> dc(s);
+  MatHandle H;
+  const int rc = rvvla_vnec_from_device(dtype_of<T>(), (int64_t)s.m, (int64_t)s.n, (int64_t)s.nz,
+                                        (const int32_t*)dc.rst, (const int32_t*)dc.col, dc.val,
+                                        nparts, nullptr, &H.A);
+  if (rc) throw Error(rc, rvvla_strerror(rc));
+  const std::vector<int32_t> cx = read_vec<int32_t, T>(H.A, "cx");
+  const std::vector<int32_t> cy = read_vec<int32_t, T>(H.A, "cy");
+  const std::vector<uint16_t> ecr = read_vec<uint16_t, T>(H.A, "ecr_indices");
+  d = spMtxVNEC<T>();
+  d.G = &s;
+  d.m = (int)s.m;
+  d.n = (int)s.n;
+  d.nnz = (int)s.nz;
+  d.NUM_THREADS_VNEC = nparts;
+  d.thread_coord_start.resize((size_t)nparts);
+  d.thread_coord_end.resize((size_t)nparts);
+  for (int t = 0; t < nparts; ++t) {
+    d.thread_coord_start[(size_t)t] = coord{cx[(size_t)t], cy[(size_t)t]};
+    d.thread_coord_end[(size_t)t] = coord{cx[(size_t)t + 1], cy[(size_t)t + 1]};
+  }
+  d.ecr_indices.assign(ecr.begin(), ecr.end());
+  d.nec_ptr = read_vec<int, T>(H.A, "nec_ptr");
+  d.NEC_NUM.resize((size_t)nparts);
+  for (int t = 0; t < nparts; ++t) d.NEC_NUM[(size_t)t] = d.nec_ptr[(size_t)t + 1] - d.nec_ptr[(size_t)t];
+  d.use_x_indices = read_vec<int, T>(H.A, "use_x_indices");
+  rvvla::vnec_select_variant(d, s);
+  d.device.reset(H.release());
+}
+
+}  // namespace
+
+#define RVVLA_CONVERT_FN(FMT, FN)                                                                 \
+  template <>                                                                                     \
+  void convert<double, FMT, true>(FMT<double> & d, const spMtxCRS<double>& s,                     \
+                                  const convertParams& p) {                                       \
+    FN<double>(d, s, p);                                                                          \
+  }                                                                                               \
+  template <>                                                                                     \
+  void convert<float, FMT, true>(FMT<float> & d, const spMtxCRS<float>& s,                        \
+                                 const convertParams& p) {                                        \
+    FN<float>(d, s, p);                                                                           \
+  }
+
+RVVLA_CONVERT_FN(spMtxCSR5, convert_csr5)
+RVVLA_CONVERT_FN(spMtxLAV, convert_lav)
+RVVLA_CONVERT_FN(spMtxCVR, convert_cvr)
+RVVLA_CONVERT_FN(spMtxVHCC, convert_vhcc)
+RVVLA_CONVERT_FN(spMtxVNEC, convert_vnec)
 
 }  // namespace SparseMatrixLib

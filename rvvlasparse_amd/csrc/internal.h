@@ -54,6 +54,10 @@ void build_vhcc(SparseMatrixLib::spMtxVHCC<V>& d, const SparseMatrixLib::spMtxCR
 template <class V>
 void build_vnec(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCRS<V>& s,
                 int nparts);
+// The IRD variant selector of a VNEC whose partitions, coordinates and ranks
+// are filled (IRD_mat, vnec_t).
+template <class V>
+void vnec_select_variant(SparseMatrixLib::spMtxVNEC<V>& d, const SparseMatrixLib::spMtxCRS<V>& s);
 
 // Default partition counts and parameters chosen for gfx950 (256 CUs).
 int default_sell_sigma(int64_t m, int64_t nnz, const int* rst);

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -34,7 +35,45 @@ spMtxCRS<T> laplacian(int g) {
   return a;
 }
 
+// A ragged matrix: every 7th row of the Laplacian emptied, row 3 made
+// 3000 entries long (columns spread over the grid).
+template <class T>
+spMtxCRS<T> ragged(int g) {
+  const spMtxCRS<T> lap = laplacian<T>(g);
+  const int n = (int)lap.n, extra = 3000;
+  std::vector<int> rst(1, 0), col;
+  std::vector<T> val;
+  for (int r = 0; r < n; ++r) {
+    if (r % 7 == 6) {
+      rst.push_back((int)col.size());
+      continue;
+    }
+    if (r == 3)
+      for (int k = 0; k < extra; ++k) {
+        col.push_back((int)((int64_t)k * n / extra));
+        val.push_back((T)(1.0 + 0.001 * k));
+      }
+    else
+      for (int e = lap.Rst[r]; e < lap.Rst[r + 1]; ++e) {
+        col.push_back(lap.Col[e]);
+        val.push_back(lap.Val[e]);
+      }
+    rst.push_back((int)col.size());
+  }
+  return spMtxCRS<T>((size_t)n, (size_t)n, col.size(), rst.data(), col.data(), val.data());
+}
+
 static int failures = 0;
+
+static void report(const char* fmt, bool dbl, const char* what, bool same) {
+  if (!same) ++failures;
+  std::printf("%-5s %-6s %s %s\n", fmt, dbl ? "double" : "float", what, same ? "ok" : "FAIL");
+}
+
+template <class V>
+static bool same_bytes(const std::vector<V>& a, const std::vector<V>& b) {
+  return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(V)) == 0);
+}
 
 template <class T>
 void check(const char* fmt, const char* form, const spMtxCRS<T>& a, const std::vector<T>& b,
@@ -154,6 +193,87 @@ void transpose_crs(const spMtxCRS<T>& lap) {
   }
 }
 
+// CSR5, LAV, VNEC, VHCC and CVR built on the device ("device" = 1): the
+// same fields as the host build, and the products of the device-built
+// containers
+template <class T>
+void device_built(const spMtxCRS<T>& a) {
+  const bool dbl = std::is_same<T, double>::value;
+  convertParams none, dev;
+  dev.param["device"].i = 1;
+  {
+    spMtxCSR5<T> h, d;
+    convert<T, spMtxCSR5, true>(h, a, none);
+    convert<T, spMtxCSR5, true>(d, a, dev);
+    report("csr5", dbl, "device-built fields",
+           h.format() == d.format() && h._p == d._p && h._sigma == d._sigma &&
+               h._bit_y_offset == d._bit_y_offset && h._bit_seg_offset == d._bit_seg_offset &&
+               h._tile_desc_len == d._tile_desc_len && h._tail_tile_start == d._tail_tile_start &&
+               h._num_offsets == d._num_offsets && h._csr_row_pointer == d._csr_row_pointer &&
+               h._csr_column_index == d._csr_column_index && same_bytes(h._csr_value, d._csr_value) &&
+               h._csr5_tile_pointer == d._csr5_tile_pointer &&
+               h._csr5_tile_descriptor == d._csr5_tile_descriptor &&
+               h._csr5_tile_empty_offset_pointer == d._csr5_tile_empty_offset_pointer &&
+               h._csr5_tile_empty_offset == d._csr5_tile_empty_offset);
+    run<T, spMtxCSR5>("csr5", a, dev);
+  }
+  {
+    spMtxLAV<T> h, d;
+    convert<T, spMtxLAV, true>(h, a, none);
+    convert<T, spMtxLAV, true>(d, a, dev);
+    const auto& hs = h.sparse_part;
+    const auto& ds = d.sparse_part;
+    const bool sp = hs.m == ds.m && hs.nz == ds.nz &&
+                    std::memcmp(hs.Rst, ds.Rst, (hs.m + 1) * sizeof(int)) == 0 &&
+                    (hs.nz == 0 || (std::memcmp(hs.Col, ds.Col, hs.nz * sizeof(int)) == 0 &&
+                                    std::memcmp(hs.Val, ds.Val, hs.nz * sizeof(T)) == 0));
+    report("lav", dbl, "device-built fields",
+           sp && h.segment_count == d.segment_count && h.chunk_count == d.chunk_count &&
+               h.chunk_offsets == d.chunk_offsets &&
+               h.chunk_offsets_full_mask == d.chunk_offsets_full_mask &&
+               h.segment_ptr == d.segment_ptr && h.out_order == d.out_order && h.mask == d.mask &&
+               same_bytes(h.val, d.val) && h.col_id == d.col_id);
+    run<T, spMtxLAV>("lav", a, dev);
+  }
+  {
+    spMtxVNEC<T> h, d;
+    convert<T, spMtxVNEC, true>(h, a, none);
+    convert<T, spMtxVNEC, true>(d, a, dev);
+    report("vnec", dbl, "device-built fields",
+           h.NUM_THREADS_VNEC == d.NUM_THREADS_VNEC &&
+               same_bytes(h.thread_coord_start, d.thread_coord_start) &&
+               same_bytes(h.thread_coord_end, d.thread_coord_end) && h.ecr_indices == d.ecr_indices &&
+               h.NEC_NUM == d.NEC_NUM && h.nec_ptr == d.nec_ptr &&
+               h.use_x_indices == d.use_x_indices && h.IRD_mat == d.IRD_mat && h.vnec_t == d.vnec_t);
+    run<T, spMtxVNEC>("vnec", a, dev);
+  }
+  {
+    spMtxVHCC<T> h, d;
+    convert<T, spMtxVHCC, true>(h, a, none);
+    convert<T, spMtxVHCC, true>(d, a, dev);
+    const auto hp = h.get_params(), dp = d.get_params();
+    report("vhcc", dbl, "device-built fields",
+           hp.num_threads == dp.num_threads && hp.vec_size == dp.vec_size &&
+               hp.n_masks == dp.n_masks && h.get_num_vectors() == d.get_num_vectors() &&
+               h.get_pad_entries() == d.get_pad_entries() && h.get_pad_rows() == d.get_pad_rows() &&
+               same_bytes(h.thr_info(), d.thr_info()) && h.veceor() == d.veceor() &&
+               h.scan_mask() == d.scan_mask() && h.row_arr() == d.row_arr() &&
+               h.col_arr() == d.col_arr() && same_bytes(h.vals_arr(), d.vals_arr()));
+    run<T, spMtxVHCC>("vhcc", a, dev);
+  }
+  {
+    spMtxCVR<T> h, d;
+    convert<T, spMtxCVR, true>(h, a, none);
+    convert<T, spMtxCVR, true>(d, a, dev);
+    report("cvr", dbl, "device-built fields",
+           h.nItems == d.nItems && h.Nthreads == d.Nthreads && h.SIMD_LEN == d.SIMD_LEN &&
+               same_bytes(h.vPack_vec_vals, d.vPack_vec_vals) &&
+               h.vPack_vec_cols == d.vPack_vec_cols && h.vPack_vec_record == d.vPack_vec_record &&
+               h.writeback_pos_record == d.writeback_pos_record);
+    run<T, spMtxCVR>("cvr", a, dev);
+  }
+}
+
 template <class T>
 void all_formats(int g) {
   const spMtxCRS<T> a = laplacian<T>(g);
@@ -202,6 +322,8 @@ void all_formats(int g) {
     run<T, spMtxSELL_C_Sigma>("sell", a, dev16);
   }
   run<T, spMtxSELL_C_Sigma>("sell", a, none);
+  device_built<T>(a);
+  device_built<T>(ragged<T>(g));
   run<T, spMtxCSR5>("csr5", a, none);
   run<T, spMtxLAV>("lav", a, none);
   run<T, spMtxVNEC>("vnec", a, none);
