@@ -17,10 +17,12 @@
 //     chunk) the longest and shortest of its 64 rows give the steps and the
 //     full-mask steps, a scan of the steps the chunk offsets and segment
 //     bases;
-//  4. one thread per row places its dense entries (lane = its position in
-//     the chunk, step = running count within the segment) and pads its
-//     remaining slots with its last dense column; one thread per chunk
-//     writes the lane masks of its partial steps.
+//  4. each row places its dense entries (lane = its position in the chunk,
+//     step = running count within the segment) and pads its remaining slots
+//     with its last dense column -- a wavefront per row in the chunks that
+//     hold a row of more than 32 dense entries, a thread per row in the
+//     others; one wavefront per (segment, chunk) writes the lane masks of
+//     its partial steps from the suffix maxima of its lane counts.
 // Then the kernels' image as the upload builds it (sparse_part as CSR with
 // its row blocks, long chunks as pieces, the fused form when every
 // sparse_part row has at most kLavFuseRow entries).
@@ -184,14 +186,17 @@ __global__ void k_lv_offsets(const int64_t* __restrict__ G, const int64_t* __res
   if (g == S - 1 && c == cc) sptr[S] = total;
 }
 
-// one thread per row: its dense entries into its lane, then the pads
+// one thread per row (sorted rows rp0 .. m-1, in chunks whose rows have at
+// most kLvShortRow dense entries): its dense entries into its lane, then
+// the pads
+constexpr int kLvShortRow = 32;
 template <class V, class SV, int NARROW>
 __global__ void k_lv_place(const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
                            const V* __restrict__ val, const int32_t* __restrict__ seg_of,
                            const uint32_t* __restrict__ perm, const int32_t* __restrict__ coff,
                            const int64_t* __restrict__ sptr, int64_t m, int64_t S, int64_t cc,
-                           uint32_t* __restrict__ col_id, SV* __restrict__ dval) {
-  const int64_t rp = (int64_t)blockIdx.x * kWG + threadIdx.x;
+                           int64_t rp0, uint32_t* __restrict__ col_id, SV* __restrict__ dval) {
+  const int64_t rp = rp0 + (int64_t)blockIdx.x * kWG + threadIdx.x;
   if (rp >= m) return;
   const uint32_t r = perm[rp];
   const int64_t c = rp / kLavL, i = rp % kLavL;
@@ -212,23 +217,101 @@ __global__ void k_lv_place(const int32_t* __restrict__ rst, const int32_t* __res
   }
 }
 
-// lane masks of the partial steps of chunk c in segment g
-__global__ void k_lv_masks(const int32_t* __restrict__ rcnt, const uint32_t* __restrict__ perm,
-                           const int32_t* __restrict__ coff, const int32_t* __restrict__ cfull,
-                           const int64_t* __restrict__ sptr, int64_t m, int64_t S, int64_t cc,
-                           uint16_t* __restrict__ mask) {
-  const int64_t q = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (q >= S * cc) return;
-  const int64_t g = q / cc, c = q % cc;
-  const int64_t off = coff[g * (cc + 1) + c], b = coff[g * (cc + 1) + c + 1];
-  for (int64_t k = cfull[q]; k < b; ++k) {
-    int hi = 0;
-    for (int i = 0; i < kLavL; ++i) {
-      const int64_t rp = c * kLavL + i;
-      if (rp < m && rcnt[(int64_t)perm[rp] * S + g] > k - off) hi = i + 1;
-    }
-    mask[sptr[g] + k] = (uint16_t)hi;
+// one wavefront per sorted row rp < rp_end (the chunks holding rows with
+// more than kLvShortRow dense entries): per segment the row's entries of
+// that segment in entry order (positions by ballot and prefix count), then
+// the pads with the row's last dense column
+template <class V, class SV, int NARROW>
+__global__ __launch_bounds__(kWG) void k_lv_place_wave(
+    const int32_t* __restrict__ rst, const int32_t* __restrict__ col, const V* __restrict__ val,
+    const int32_t* __restrict__ seg_of, const uint32_t* __restrict__ perm,
+    const int32_t* __restrict__ coff, const int64_t* __restrict__ sptr, int64_t rp_end, int64_t S,
+    int64_t cc, uint32_t* __restrict__ col_id, SV* __restrict__ dval) {
+  const int64_t rp = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  if (rp >= rp_end) return;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t r = perm[rp];
+  const int64_t c = rp / kLavL, i = rp % kLavL;
+  const int e0 = rst[r], e1 = rst[r + 1];
+  int best = -1;  // the row's last dense entry
+  for (int e = e0 + lane; e < e1; e += 64)
+    if (seg_of[col[e]] >= 0) best = e;
+  for (int d = 32; d > 0; d >>= 1) {
+    const int o = __shfl_xor(best, d, 64);
+    best = o > best ? o : best;
   }
+  const uint32_t last_col = best >= 0 ? (uint32_t)col[best] : 0u;
+  for (int64_t g = 0; g < S; ++g) {
+    const int64_t pos0 = sptr[g] + coff[g * (cc + 1) + c];
+    const int64_t end = sptr[g] + coff[g * (cc + 1) + c + 1];
+    int64_t run = 0;
+    for (int b = e0; b < e1; b += 64) {
+      const int e = b + lane;
+      const int32_t cl = e < e1 ? col[e] : 0;
+      const bool mine = e < e1 && seg_of[cl] == g;
+      const uint64_t bal = __ballot(mine);
+      if (mine) {
+        const int64_t p = pos0 + run + __popcll(bal & below);
+        col_id[p * kLavL + i] = (uint32_t)cl;
+        lv_store<V, SV, NARROW>(dval, p * kLavL + i, val[e]);
+      }
+      run += __popcll(bal);
+    }
+    for (int64_t p = pos0 + run + lane; p < end; p += 64) col_id[p * kLavL + i] = last_col;
+  }
+}
+
+// lane masks of the partial steps of chunk c in segment g, one wavefront
+// per (g, c): with M_j the largest count of lanes j .. 63, the mask of
+// step k (t = k - offset) is the last lane with a count above t, plus one,
+// = the number of j with M_j > t (M is non-increasing in j)
+__global__ __launch_bounds__(kWG) void k_lv_masks(
+    const int32_t* __restrict__ rcnt, const uint32_t* __restrict__ perm,
+    const int32_t* __restrict__ coff, const int32_t* __restrict__ cfull,
+    const int64_t* __restrict__ sptr, int64_t m, int64_t S, int64_t cc,
+    uint16_t* __restrict__ mask) {
+  __shared__ int32_t sm[kWG];
+  const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  int32_t* M = sm + (threadIdx.x & ~63);
+  const bool live = q < S * cc;
+  const int64_t g = live ? q / cc : 0, c = live ? q % cc : 0;
+  const int64_t rp = c * kLavL + lane;
+  int32_t v = live && rp < m ? rcnt[(int64_t)perm[rp] * S + g] : 0;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t o = __shfl_down(v, d, 64);
+    if (lane + d < 64) v = o > v ? o : v;
+  }
+  M[lane] = v;
+  __syncthreads();
+  if (!live) return;
+  const int64_t off = coff[g * (cc + 1) + c], b = coff[g * (cc + 1) + c + 1];
+  for (int64_t k = cfull[q] + lane; k < b; k += 64) {
+    const int64_t t = k - off;
+    int lo = 0, hi = 64;  // the first j with M[j] <= t
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((int64_t)M[mid] > t) lo = mid + 1;
+      else hi = mid;
+    }
+    mask[sptr[g] + k] = (uint16_t)lo;
+  }
+}
+
+// the number of rows with more than kLvShortRow dense entries (sorted keys
+// are 0xFFFFFFFF - dense length, ascending)
+__global__ void k_lv_count_long(const uint32_t* __restrict__ skey, int64_t m,
+                                int64_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t lim = 0xFFFFFFFFu - (uint32_t)kLvShortRow;
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (skey[mid] < lim) lo = mid + 1;
+    else hi = mid;
+  }
+  *out = lo;
 }
 
 namespace {
@@ -397,21 +480,40 @@ int build_lav_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const in
                        (uint16_t)kLavL);
     lv_launched("k_lv_fill_u16 launch");
   }
-  {
-    const unsigned g = lv_grid(m);
+  // rows of the chunks that hold a row of more than kLvShortRow dense
+  // entries: a wavefront each; the rest: a thread each
+  int64_t* nlong = steps;  // reused: its scan is done
+  hipLaunchKernelGGL(k_lv_count_long, dim3(1), dim3(64), 0, s, rkey2, m, nlong);
+  lv_launched("k_lv_count_long launch");
+  const int64_t rp0 = std::min<int64_t>(m, (lv_read64(nlong, s) + kLavL - 1) / kLavL * kLavL);
+  if (rp0 > 0) {
+    const unsigned g = (unsigned)((rp0 + kWG / 64 - 1) / (kWG / 64));
+    if (narrow == 1)
+      hipLaunchKernelGGL((k_lv_place_wave<V, uint16_t, 1>), dim3(g), dim3(kWG), 0, s, rst, col,
+                         val, seg_of, perm, coff, sptr, rp0, S, cc, col_id, (uint16_t*)dval);
+    else if (narrow == 2)
+      hipLaunchKernelGGL((k_lv_place_wave<V, uint16_t, 2>), dim3(g), dim3(kWG), 0, s, rst, col,
+                         val, seg_of, perm, coff, sptr, rp0, S, cc, col_id, (uint16_t*)dval);
+    else
+      hipLaunchKernelGGL((k_lv_place_wave<V, V, 0>), dim3(g), dim3(kWG), 0, s, rst, col, val,
+                         seg_of, perm, coff, sptr, rp0, S, cc, col_id, (V*)dval);
+    lv_launched("k_lv_place_wave launch");
+  }
+  if (rp0 < m) {
+    const unsigned g = lv_grid(m - rp0);
     if (narrow == 1)
       hipLaunchKernelGGL((k_lv_place<V, uint16_t, 1>), dim3(g), dim3(kWG), 0, s, rst, col, val,
-                         seg_of, perm, coff, sptr, m, S, cc, col_id, (uint16_t*)dval);
+                         seg_of, perm, coff, sptr, m, S, cc, rp0, col_id, (uint16_t*)dval);
     else if (narrow == 2)
       hipLaunchKernelGGL((k_lv_place<V, uint16_t, 2>), dim3(g), dim3(kWG), 0, s, rst, col, val,
-                         seg_of, perm, coff, sptr, m, S, cc, col_id, (uint16_t*)dval);
+                         seg_of, perm, coff, sptr, m, S, cc, rp0, col_id, (uint16_t*)dval);
     else
       hipLaunchKernelGGL((k_lv_place<V, V, 0>), dim3(g), dim3(kWG), 0, s, rst, col, val, seg_of,
-                         perm, coff, sptr, m, S, cc, col_id, (V*)dval);
+                         perm, coff, sptr, m, S, cc, rp0, col_id, (V*)dval);
     lv_launched("k_lv_place launch");
   }
-  hipLaunchKernelGGL(k_lv_masks, dim3(lv_grid(S * cc)), dim3(kWG), 0, s, rcnt, perm, coff, cfull,
-                     sptr, m, S, cc, mask);
+  hipLaunchKernelGGL(k_lv_masks, dim3((unsigned)((S * cc + kWG / 64 - 1) / (kWG / 64))), dim3(kWG),
+                     0, s, rcnt, perm, coff, cfull, sptr, m, S, cc, mask);
   lv_launched("k_lv_masks launch");
   if (int rc = hip_check(hipStreamSynchronize(s), "device LAV build")) return rc;
   host.total = total;
