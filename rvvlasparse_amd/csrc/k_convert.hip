@@ -143,26 +143,47 @@ __global__ void k_conv_not_identity(const uint32_t* __restrict__ perm, int64_t m
   if (r < m && perm[r] != (uint32_t)r) atomicOr(flag, 1);
 }
 
+// slots of a slice written by one wavefront: slices wider than this are cut
+// into column blocks of kScatterCols, each its own wavefront
+constexpr int kScatterCols = 256;
+
+__global__ void k_conv_col_blocks(const int32_t* __restrict__ cl, int64_t nslices,
+                                  int64_t* __restrict__ nblk) {
+  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (i < nslices) nblk[i] = cl[i] > kScatterCols ? (cl[i] + kScatterCols - 1) / kScatterCols : 1;
+}
+
 // V: input value type (double for F64, float otherwise); S: stored type
-// (V, or uint16_t bits for BF16 / F16 with NARROW = 1 / 2)
+// (V, or uint16_t bits for BF16 / F16 with NARROW = 1 / 2).  Wavefront wv
+// writes column block wv - bbase[slice] of the slice whose blocks hold it
+// (bbase: exclusive scan of the per-slice block counts).
 template <class V, class S, int NARROW>
 __global__ void k_conv_scatter(const int32_t* __restrict__ rst, const int32_t* __restrict__ col,
                                const V* __restrict__ val, const uint32_t* __restrict__ perm,
                                const int32_t* __restrict__ cl, const int32_t* __restrict__ cs,
-                               int64_t m, int64_t nslices, uint32_t* __restrict__ scol,
+                               const int64_t* __restrict__ bbase, int64_t nwaves, int64_t m,
+                               int64_t nslices, uint32_t* __restrict__ scol,
                                S* __restrict__ sval) {
-  const int64_t slice = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
+  const int64_t wv = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (slice >= nslices) return;
+  if (wv >= nwaves) return;
+  int64_t lo = 0, hi = nslices;  // the last slice with bbase <= wv
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (bbase[mid] <= wv) lo = mid;
+    else hi = mid;
+  }
+  const int64_t slice = lo;
   const int64_t k = slice * 64 + lane;
   if (k >= m) return;
   const int height = (int)min<int64_t>(64, m - slice * 64);
-  const int w = cl[slice];
+  const int j0 = (int)(wv - bbase[slice]) * kScatterCols;
+  const int w = cl[slice] > kScatterCols ? min(cl[slice], j0 + kScatterCols) : cl[slice];
   const int64_t base = cs[slice];
   const uint32_t r = perm[k];
   const int e0 = rst[r], l = rst[r + 1] - e0;
   const uint32_t pad_col = l > 0 ? (uint32_t)col[e0 + l - 1] : 0u;
-  for (int j = 0; j < w; ++j) {
+  for (int j = j0; j < w; ++j) {
     const int64_t slot = base + (int64_t)j * height + lane;
     const bool in = j < l;
     scol[slot] = in ? (uint32_t)col[e0 + j] : pad_col;
@@ -274,15 +295,37 @@ int build_sell_device(rvvla_mat_s& M, int64_t m, int64_t nnz, const int32_t* rst
   const size_t vbytes = M.dtype == RVVLA_F64 ? 8 : M.dtype == RVVLA_F32 ? 4 : 2;
   uint32_t* scol = (uint32_t*)M.put(nullptr, (size_t)total * 4);
   void* sval = M.put(nullptr, (size_t)total * vbytes);
+  // column blocks of the wide slices, a wavefront each
+  int64_t* nblk = (int64_t*)M.temp((size_t)nslices * 16);
+  int64_t* bbase = nblk + nslices;
+  hipLaunchKernelGGL(k_conv_col_blocks, dim3((unsigned)((nslices + kWG - 1) / kWG)), dim3(kWG), 0,
+                     s, cl, nslices, nblk);
+  tmp = 0;
+  if (int rc = hip_check(rocprim::exclusive_scan(nullptr, tmp, nblk, bbase, (int64_t)0,
+                                                  (size_t)nslices, rocprim::plus<int64_t>(), s),
+                         "scan size"))
+    return rc;
+  tbuf = M.temp(tmp);
+  if (int rc = hip_check(rocprim::exclusive_scan(tbuf, tmp, nblk, bbase, (int64_t)0,
+                                                  (size_t)nslices, rocprim::plus<int64_t>(), s),
+                         "scan"))
+    return rc;
+  if (int rc = hip_check(hipMemcpyAsync(&last[0], bbase + nslices - 1, 8, hipMemcpyDeviceToHost, s), "blocks"))
+    return rc;
+  if (int rc = hip_check(hipMemcpyAsync(&last[1], nblk + nslices - 1, 8, hipMemcpyDeviceToHost, s), "blocks"))
+    return rc;
+  if (int rc = hip_check(hipStreamSynchronize(s), "blocks")) return rc;
+  const int64_t nwaves = last[0] + last[1];
+  const unsigned g_waves = (unsigned)((nwaves + kWG / 64 - 1) / (kWG / 64));
   if (M.dtype == RVVLA_BF16)
-    hipLaunchKernelGGL((k_conv_scatter<V, uint16_t, 1>), dim3(g_slices), dim3(kWG), 0, s, rst, col,
-                       val, perm, cl, cs, m, nslices, scol, (uint16_t*)sval);
+    hipLaunchKernelGGL((k_conv_scatter<V, uint16_t, 1>), dim3(g_waves), dim3(kWG), 0, s, rst, col,
+                       val, perm, cl, cs, bbase, nwaves, m, nslices, scol, (uint16_t*)sval);
   else if (M.dtype == RVVLA_F16)
-    hipLaunchKernelGGL((k_conv_scatter<V, uint16_t, 2>), dim3(g_slices), dim3(kWG), 0, s, rst, col,
-                       val, perm, cl, cs, m, nslices, scol, (uint16_t*)sval);
+    hipLaunchKernelGGL((k_conv_scatter<V, uint16_t, 2>), dim3(g_waves), dim3(kWG), 0, s, rst, col,
+                       val, perm, cl, cs, bbase, nwaves, m, nslices, scol, (uint16_t*)sval);
   else
-    hipLaunchKernelGGL((k_conv_scatter<V, V, 0>), dim3(g_slices), dim3(kWG), 0, s, rst, col, val,
-                       perm, cl, cs, m, nslices, scol, (V*)sval);
+    hipLaunchKernelGGL((k_conv_scatter<V, V, 0>), dim3(g_waves), dim3(kWG), 0, s, rst, col, val,
+                       perm, cl, cs, bbase, nwaves, m, nslices, scol, (V*)sval);
   int flag = 0;
   if (sigma > 1) {
     int* dflag = (int*)M.temp(8);
