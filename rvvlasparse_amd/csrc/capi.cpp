@@ -241,8 +241,36 @@ static int choose_xslices(const rvvla_mat_s& M, const spMtxCRS<V>& a, int reques
   return xbytes / xcc > (int64_t(4) << 20) ? 2 * xcc : xcc;
 }
 
+// CUs of the current device (256 assumed in host-validation mode).
+static int device_cus(const rvvla_mat_s& M) {
+  int cus = 256;
+  if (!M.host_only &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, M.device) != hipSuccess)
+    cus = 256;
+  return std::max(1, cus);
+}
+
+// The LDS-sliced image: `request` 1 = on, 0 = off, -1 = chosen from the
+// matrix: FP64 values where the column-sliced image would be chosen by
+// default (an explicit xslices keeps the column-sliced image), with at
+// least one slice width of entries per workgroup (to amortise loading x
+// into LDS) and at least 3 entries per (row, slice) pair (each pair costs
+// a partial written and read back).  The power-law matrix of 4.2 M rows and
+// 64 M entries (3.9 entries per pair): 357 us against 440 us on MI355X;
+// with 32-bit x the column-sliced image stays faster (339 us against
+// 483 us FP32).
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices) {
+static bool choose_xlds(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request, int xslices) {
+  if (request >= 0) return request > 0 && a.n > 0 && a.m > 0;
+  if (xslices >= 0 || M.dtype != RVVLA_F64 || a.n == 0 || a.m == 0) return false;
+  if (choose_xslices(M, a, -1) == 0) return false;
+  const int64_t nwg = (int64_t)device_cus(M) * xl_wg_per_cu(8);
+  if ((int64_t)a.nz < nwg * kXlSliceCols) return false;
+  return (int64_t)a.nz >= 3 * xlds_pairs((int64_t)a.m, a.Rst, a.Col, kXlSliceCols);
+}
+
+template <class V>
+rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   check_csr((int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, a.Rst, a.Col);
   auto M = new_mat(RVVLA_FMT_CSR, dtype, (int64_t)a.m, (int64_t)a.n, (int64_t)a.nz);
@@ -252,6 +280,30 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices) {
   M->csr.val = put_values(*M, dtype, a.Val, a.nz);
   M->csr.blocks = M->put_vec(blocks);
   M->csr.nblocks = (int64_t)blocks.size() / 2;
+  if (choose_xlds(*M, a, xlds, xslices)) {
+    XLdsHost<V> X;
+    const int64_t nwg = (int64_t)device_cus(*M) * xl_wg_per_cu(dtype == RVVLA_F64 ? 8 : 4);
+    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, kXlSliceCols, nwg);
+    DevXLds& d = M->csr.xl;
+    d.S = X.S;
+    d.W = X.W;
+    d.n = (int64_t)a.n;
+    d.npairs = X.npairs;
+    d.nunits = (int64_t)X.units.size() / 3;
+    d.nwg = (int64_t)X.wg.size() - 1;
+    d.nblocks = (int64_t)X.cblk.size() / 4;
+    d.col = M->put_vec(X.col);
+    d.val = put_values(*M, dtype, X.val.data(), X.val.size());
+    d.pst = M->put_vec(X.pst);
+    d.wpos = M->put_vec(X.wpos);
+    d.lpos = M->put_vec(X.lpos);
+    d.rowptr = M->put_vec(X.rowptr);
+    d.cblk = M->put_vec(X.cblk);
+    d.units = M->put_vec(X.units);
+    d.wg = M->put_vec(X.wg);
+    d.part = M->put(nullptr, (size_t)std::max<int64_t>(d.npairs, 1) * (dtype == RVVLA_F64 ? 8 : 4));
+    return M.release();
+  }
   const int P = choose_xslices(*M, a, xslices);
   if (P > 0) {
     XSliceHost<V> X;
@@ -944,8 +996,8 @@ template rvvla_mat_t upload_vnec<float>(int, const spMtxVNEC<float>&, const spMt
 
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
-template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int);
-template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int);
+template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int, int);
+template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int, int);
 RVVLA_INST_UPLOAD(upload_hcsr, spMtxHCSR)
 RVVLA_INST_UPLOAD(upload_sell, spMtxSELL_C_Sigma)
 RVVLA_INST_UPLOAD(upload_csr5, spMtxCSR5)
@@ -994,7 +1046,8 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
       L.arrays["Val"] = ArrayView{a->Val, (int64_t)a->nz, (int)sizeof(V)};
       L.scalars = {{"m", (double)a->m}, {"n", (double)a->n}, {"nz", (double)a->nz}};
       const int xs = param_int(P, "xslices", -1);
-      L.upload = [a, dtype, xs]() { return upload_csr<V>(dtype, *a, xs); };
+      const int xl = param_int(P, "xlds", -1);
+      L.upload = [a, dtype, xs, xl]() { return upload_csr<V>(dtype, *a, xs, xl); };
       break;
     }
     case RVVLA_FMT_HCSR: {
@@ -1748,7 +1801,16 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
             : f == "xs.mask" ? V{x.mask, x.P ? A->m : 0, 2}
             : f == "xs.base" ? V{x.base, x.P ? x.P + 1 : 0, 4}
             : f == "xs.pref" ? V{x.pref, x.P * x.nq, 4}
-                             : V{nullptr, -1, 0};
+            : f == "xl.col"    ? V{c.xl.col, c.xl.S ? A->nnz : 0, 2}
+            : f == "xl.val"    ? V{c.xl.val, c.xl.S ? A->nnz : 0, vb}
+            : f == "xl.pst"    ? V{c.xl.pst, c.xl.S ? c.xl.npairs + 1 : 0, 4}
+            : f == "xl.wpos"   ? V{c.xl.wpos, c.xl.npairs, 4}
+            : f == "xl.lpos"   ? V{c.xl.lpos, c.xl.npairs, 2}
+            : f == "xl.rowptr" ? V{c.xl.rowptr, c.xl.S ? A->m + 1 : 0, 4}
+            : f == "xl.cblk"   ? V{c.xl.cblk, 4 * c.xl.nblocks, 4}
+            : f == "xl.units"  ? V{c.xl.units, 3 * c.xl.nunits, 4}
+            : f == "xl.wg"     ? V{c.xl.wg, c.xl.S ? c.xl.nwg + 1 : 0, 4}
+                               : V{nullptr, -1, 0};
   if (v.n < 0) {
     rvvla::set_last_error("rvvla_mat_view: unknown field " + f);
     return RVVLA_ERR_NOT_FOUND;

@@ -12,4 +12,9 @@ constexpr int kStreamCap = 2048;
 constexpr int kStreamRows = 256;
 constexpr int kLongRow = 1024;
 
+// Combine blocks of the LDS-sliced image: up to kXlCombRows rows holding
+// at most kXlCombPairs partials (staged in LDS), or one row with more.
+constexpr int kXlCombRows = 1024;
+constexpr int kXlCombPairs = 8192;
+
 }  // namespace rvvla

@@ -28,6 +28,30 @@ struct DevXSlice {
   void* part = nullptr;
 };
 
+// LDS-sliced image (csr_xlds.cpp): S slices of W columns, each unit of
+// pairs run by one workgroup holding its slice of x in LDS.
+constexpr int kXlWG = 1024;          // threads of the slice kernel
+constexpr int kXlSliceCols = 16384;  // x slice in LDS: 128 KiB (FP64) or 64 KiB (FP32)
+constexpr int kXlTile = 248;         // pairs per wavefront tile (LDS accumulators)
+constexpr int kXlLdsLimit = 163840;  // LDS of one CU
+// slice-pass workgroups per CU: as many as the CU's LDS holds
+constexpr int xl_wg_per_cu(int io_bytes) {
+  return kXlLdsLimit / (kXlSliceCols * io_bytes + (kXlWG / 64) * kXlTile * io_bytes + 16);
+}
+struct DevXLds {
+  int64_t S = 0, W = 0, npairs = 0, nunits = 0, n = 0, nwg = 0, nblocks = 0;
+  const uint16_t* col = nullptr;
+  const void* val = nullptr;
+  const int32_t* pst = nullptr;
+  const int32_t* wpos = nullptr;
+  const uint16_t* lpos = nullptr;
+  const int32_t* rowptr = nullptr;
+  const int32_t* cblk = nullptr;
+  const int32_t* units = nullptr;
+  const int32_t* wg = nullptr;
+  void* part = nullptr;
+};
+
 // CSR: adaptive row blocks.  blocks[b] .. blocks[b+1] is the row range of
 // workgroup b (see csr_blocks.h).  When xs.P > 0 the single-vector product
 // runs on the column-sliced image; the plain arrays serve SpMM and the
@@ -39,6 +63,7 @@ struct DevCSR {
   const int32_t* blocks = nullptr;
   int64_t nblocks = 0;
   DevXSlice xs;
+  DevXLds xl;
 };
 
 // HCSR: non-empty blocks only, grouped by block-row.

@@ -76,9 +76,12 @@ std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts
 // Upload of each format's host container; value arrays are given as V
 // (double for F64, float otherwise) and narrowed for BF16/F16.
 // xslices: column slices of the sliced image (-1 = chosen from the matrix
-// and the device, 0 or 1 = plain CSR only, 2..8 = that many).
+// and the device, 0 or 1 = plain CSR only, 2..8 = that many).  xlds: the
+// LDS-sliced image (-1 = chosen from the matrix, 0 = off, 1 = on; it takes
+// the place of the sliced image).
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a, int xslices = -1);
+rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a, int xslices = -1,
+                       int xlds = -1);
 template <class V>
 rvvla_mat_t upload_hcsr(int dtype, const SparseMatrixLib::spMtxHCSR<V>& a);
 template <class V>
@@ -114,6 +117,21 @@ struct XSliceHost {
 template <class V>
 void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
                   const V* val, int P, int xcd);
+// LDS-sliced CSR image (csr_xlds.cpp): x cut into slices of W elements
+// that fit one workgroup's LDS; entries slice-major with 16-bit columns.
+template <class V>
+struct XLdsHost {
+  int64_t W = 0, S = 0, npairs = 0;
+  std::vector<uint16_t> col, lpos;
+  std::vector<V> val;
+  std::vector<int32_t> pst, wpos, rowptr, cblk, units, wg;
+};
+// (row, slice) pairs of a CSR matrix for slices of W columns.
+int64_t xlds_pairs(int64_t m, const int* rst, const int* col, int64_t W);
+// nwg: the workgroups of the slice pass
+template <class V>
+void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col, const V* val,
+                int64_t W, int64_t nwg);
 // Fraction of entries whose column is more than `window` away from the
 // row's diagonal position r * n / m.
 double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window);

@@ -172,6 +172,167 @@ __global__ __launch_bounds__(kWG) void k_xs_combine(
   }
 }
 
+// LDS-sliced image (csr_xlds.cpp).  One workgroup per CU runs its units
+// (slice p, pairs g0 .. g1) in turn: x[p W, p W + W) into LDS (when the
+// slice changes), then tiles of kXlTile pairs claimed by the wavefronts in
+// turn.  A wavefront streams its tile's entries 64 at a time (kXlAhead
+// chunks of loads in flight), each lane one entry; bit 15 of the column
+// marks the first entry of a pair, so a ballot and a count of the marks
+// below each lane give every entry its pair.  Each lane keeps the sum of
+// its entries of its current pair in a register and adds it to the pair's
+// LDS accumulator when its pair changes (lanes in order, so the order of
+// the additions is fixed by the data); at the end of the tile the
+// accumulators go to the pairs' slots.
+constexpr int kXlAhead = 8;
+static_assert(xl_wg_per_cu(8) >= 1 && xl_wg_per_cu(4) >= 2, "LDS of k_xl_slices");
+template <class VT>
+__global__ __launch_bounds__(kXlWG) void k_xl_slices(
+    const uint16_t* __restrict__ col, const VT* __restrict__ val, const int32_t* __restrict__ pst,
+    const int32_t* __restrict__ wpos, const int32_t* __restrict__ units,
+    const int32_t* __restrict__ wg, const typename Val<VT>::io* __restrict__ x, int64_t n,
+    int64_t W, typename Val<VT>::acc* __restrict__ part) {
+  using A = typename Val<VT>::acc;
+  using IO = typename Val<VT>::io;
+  constexpr int XW = kXlSliceCols;
+  constexpr int XP = XW / kXlWG;
+  static_assert(XW % kXlWG == 0, "x slice per thread");
+  __shared__ IO xs[XW];
+  __shared__ A acc[(kXlWG / 64) * kXlTile];
+  __shared__ int next_tile;
+  const int tid = threadIdx.x, lane = tid & 63;
+  A* pacc = acc + (tid >> 6) * kXlTile;
+  const int u0 = wg[blockIdx.x], u1 = wg[blockIdx.x + 1];
+  int64_t loaded = -1;
+  uint32_t ca[kXlAhead];
+  A va[kXlAhead];
+  for (int u = u0; u < u1; ++u) {
+    const int64_t p = units[3 * u];
+    const int64_t g0 = units[3 * u + 1], g1 = units[3 * u + 2];
+    __syncthreads();  // every wavefront is done with the previous unit
+    if (p != loaded) {
+      const int64_t c0 = p * W;
+      const int wl = (int)(n - c0 < W ? n - c0 : W);
+      IO tmp[XP];
+#pragma unroll
+      for (int j = 0; j < XP; ++j) {
+        const int i = tid + j * kXlWG;
+        tmp[j] = i < wl ? x[c0 + i] : IO(0);
+      }
+#pragma unroll
+      for (int j = 0; j < XP; ++j) xs[tid + j * kXlWG] = tmp[j];
+      loaded = p;
+    }
+    if (tid == 0) next_tile = kXlWG / 64;  // tiles 0 .. waves-1 go one per wavefront
+    __syncthreads();
+    const int64_t ntiles = (g1 - g0 + kXlTile - 1) / kXlTile;
+    int64_t t = tid >> 6;
+    while (t < ntiles) {
+      int nt = 0;  // the next tile, claimed now
+      if (lane == 0) nt = atomicAdd(&next_tile, 1);
+      const int64_t gt = g0 + t * kXlTile;
+      const int np = (int)(gt + kXlTile < g1 ? kXlTile : g1 - gt);
+      const int e_beg = pst[gt], e_end = pst[gt + np];
+      for (int i = lane; i < kXlTile; i += 64) pacc[i] = A(0);
+      int cur = -1;   // tile-local pair of the lane's running sum
+      A run = A(0);
+      int open = -1;  // tile-local pair open before the chunk
+      for (int cb = e_beg; cb < e_end; cb += 64 * kXlAhead) {
+        // loads without branches (an index past the tile reads its last
+        // entry and is masked), so no wait for all loads lands among them
+#pragma unroll
+        for (int k = 0; k < kXlAhead; ++k) {
+          const int e = cb + k * 64 + lane < e_end ? cb + k * 64 + lane : e_end - 1;
+          ca[k] = (uint32_t)col[e];
+          va[k] = Val<VT>::get(val, e);
+        }
+        A prod[kXlAhead];  // the group's x gathers issued together
+#pragma unroll
+        for (int k = 0; k < kXlAhead; ++k) prod[k] = va[k] * (A)xs[ca[k] & 0x7FFFu];
+#pragma unroll
+        for (int k = 0; k < kXlAhead; ++k) {
+          const bool in = cb + k * 64 + lane < e_end;
+          const uint64_t H = __ballot(in && (ca[k] & 0x8000u));
+          const int below = (int)__builtin_amdgcn_mbcnt_hi(
+              (uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
+          const int lp = open + below + (int)((H >> lane) & 1ull);
+          if (in) {
+            if (lp != cur) {
+              if (cur >= 0) atomicAdd(&pacc[cur], run);
+              run = A(0);
+              cur = lp;
+            }
+            run += prod[k];
+          }
+          open += __popcll(H);
+        }
+      }
+      if (cur >= 0) atomicAdd(&pacc[cur], run);
+      for (int i = lane; i < np; i += 64) part[wpos[gt + i]] = pacc[i];
+      t = __shfl(nt, 0, 64);
+    }
+  }
+}
+
+// Combine block b of the LDS-sliced image: rows r0 .. r1-1 (at most
+// kXlCombRows, one per thread) with slots base .. end-1.  Each thread loads
+// its (at most kXlCombPairs / kXlWG) partials together and puts each at its
+// row-major place in LDS (lpos); then each row's partials are added in
+// slice order and alpha and beta applied.  A block of one row with more
+// than kXlCombPairs partials (then already in slice order) is summed by
+// the whole workgroup.
+static_assert(kXlCombRows <= kXlWG && kXlCombPairs % kXlWG == 0, "combine block shape");
+template <class A, class IO>
+__global__ __launch_bounds__(kXlWG) void k_xl_combine(const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ cblk,
+                                                      const uint16_t* __restrict__ lpos,
+                                                      const A* __restrict__ part,
+                                                      IO* __restrict__ y, A alpha, A beta) {
+  constexpr int PT = kXlCombPairs / kXlWG;
+  __shared__ A prod[kXlCombPairs];
+  __shared__ A wsum[kXlWG / 64];
+  const int tid = threadIdx.x;
+  const int r0 = cblk[4 * blockIdx.x], r1 = cblk[4 * blockIdx.x + 1];
+  const int base = cblk[4 * blockIdx.x + 2], cnt = cblk[4 * blockIdx.x + 3];
+  if (cnt > kXlCombPairs) {
+    Acc<A, false> a;
+    for (int q = tid; q < cnt; q += kXlWG) a.add(part[base + q]);
+    a = group_reduce(a, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = a.s;
+    __syncthreads();
+    if (tid == 0) {
+      A t = wsum[0];
+#pragma unroll
+      for (int w = 1; w < kXlWG / 64; ++w) t += wsum[w];
+      y[r0] = alpha * t + scaled_y(beta, y, r0);
+    }
+    return;
+  }
+  const int r = r0 + tid;  // this thread's row: its bounds fetched with the partials
+  int k0 = 0, k1 = 0;
+  if (r < r1) {
+    k0 = rowptr[r] - base;
+    k1 = rowptr[r + 1] - base;
+  }
+  A v[PT];
+  int at[PT];
+#pragma unroll
+  for (int j = 0; j < PT; ++j) {
+    const int q = tid + j * kXlWG;
+    const int qq = q < cnt ? q : (cnt > 0 ? cnt - 1 : 0);  // in range, masked below
+    v[j] = cnt > 0 ? part[base + qq] : A(0);
+    at[j] = cnt > 0 ? (int)lpos[base + qq] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < PT; ++j)
+    if (tid + j * kXlWG < cnt) prod[at[j]] = v[j];
+  __syncthreads();
+  if (r < r1) {
+    A s = A(0);
+    for (int k = k0; k < k1; ++k) s += prod[k];
+    y[r] = alpha * s + scaled_y(beta, y, r);
+  }
+}
+
 // SpMM: Y = alpha*A*X + beta*Y, X n x k, Y m x k (row-major).  One
 // wavefront per row; lanes cover the k right-hand sides (k <= 64 per pass),
 // so X rows are read as contiguous k-vectors.
@@ -203,6 +364,20 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const bool comp = sizeof(VT) == 2;
+  if (d.xl.S > 0) {
+    const DevXLds& xl = d.xl;
+    if (m == 0) return RVVLA_OK;
+    if (xl.nunits > 0) {
+      hipLaunchKernelGGL((k_xl_slices<VT>), dim3((unsigned)xl.nwg), dim3(kXlWG), 0, s, xl.col,
+                         (const VT*)xl.val, xl.pst, xl.wpos, xl.units, xl.wg, (const IO*)x, xl.n,
+                         xl.W, (AC*)xl.part);
+      if (int rc = hip_check(hipGetLastError(), "k_xl_slices launch")) return rc;
+    }
+    hipLaunchKernelGGL((k_xl_combine<AC, IO>), dim3((unsigned)xl.nblocks), dim3(kXlWG), 0, s,
+                       xl.rowptr, xl.cblk, xl.lpos, (const AC*)xl.part, (IO*)y, (AC)alpha,
+                       (AC)beta);
+    return hip_check(hipGetLastError(), "k_xl_combine launch");
+  }
   if (d.xs.P > 0) {
     const DevXSlice& xs = d.xs;
     if (m == 0) return RVVLA_OK;

@@ -168,6 +168,53 @@ def test_csr_column_slices_16bit(dtype):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
+@pytest.mark.parametrize("name", CASES + ["wide"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_csr_lds_slices(name, dtype):
+    """The LDS-sliced CSR image (x slices held in LDS, a partial per
+    (row, slice) pair, partials added in slice order) against the oracle,
+    including long pairs summed by a whole wavefront and empty rows."""
+    a = case(name) if name != "wide" else _wide()
+    print("csr xlds", name, dtype, flush=True)
+    x64, y064 = inputs(a, 8)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
+        y = run_gpu("csr", dtype, a, x64, y064, alpha, beta, "xlds=1")
+        v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        y_ref = O.csr_spmv(a.rst, a.col, v, x, alpha, beta, y0)
+        assert_within_bound(y, y_ref, a.rst, a.col, v, x, alpha, beta, y0, dt,
+                            f"csr xlds/{name}/a={alpha},b={beta}")
+        # the order of the additions is fixed by the data: the same bits again
+        assert np.array_equal(y, run_gpu("csr", dtype, a, x64, y064, alpha, beta, "xlds=1"))
+
+
+def _wide():
+    """Uniform columns over n = 1.5 M (92 FP64 slices), rows of 0..40 entries
+    and a few of 20000."""
+    if "wide" not in _CACHE:
+        rng = np.random.default_rng(12)
+        m, n = 20000, 1_500_000
+        lens = rng.integers(0, 41, m)
+        lens[[3, 9000, 19999]] = 20000
+        rst = np.zeros(m + 1, np.int32)
+        rst[1:] = np.cumsum(lens)
+        col = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int32)
+        _CACHE["wide"] = R.CSR(m, n, rst, col, rng.uniform(-1, 1, len(col)))
+    return _CACHE["wide"]
+
+
+@pytest.mark.parametrize("dtype", [R.BF16, R.F16])
+def test_csr_lds_slices_16bit(dtype):
+    a = case("powerlaw")
+    x64, y064 = inputs(a, 9)
+    y = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, "xlds=1")
+    x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+    y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 1.0, 0.5,
+                     y032.astype(np.float64))
+    rel = np.linalg.norm(y - y64) / np.linalg.norm(y64)
+    assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
+
+
 @pytest.mark.parametrize("params", ["R=16,C=16,p=2", "R=64,C=32,p=1", "R=300,C=400,p=2"])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 @pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64"])
