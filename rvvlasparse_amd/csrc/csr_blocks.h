@@ -14,7 +14,7 @@ constexpr int kLongRow = 1024;
 
 // Combine blocks of the LDS-sliced image: up to kXlCombRows rows holding
 // at most kXlCombPairs partials (staged in LDS), or one row with more.
-constexpr int kXlCombRows = 1024;
-constexpr int kXlCombPairs = 8192;
+constexpr int kXlCombRows = 256;
+constexpr int kXlCombPairs = 2048;
 
 }  // namespace rvvla

@@ -15,7 +15,7 @@ import rvvlasparse_amd as R
 from helpers import assert_within_bound, laplacian, rand_csr
 from oracle import oracle as O
 
-COMB_PAIRS, COMB_ROWS, CUS, W = 8192, 1024, 256, 16384
+COMB_PAIRS, COMB_ROWS, CUS, W = 2048, 256, 256, 16384
 
 
 @pytest.fixture(autouse=True)
@@ -79,7 +79,8 @@ CASES = {
 
 def _many_pairs():
     """One row with 9000 pairs (one entry in each of 9000 slices) among
-    short rows: a combine block of its own, summed in place."""
+    short rows: a combine block of its own (more than 2048 partials),
+    summed in place."""
     n = 9000 * W
     cols = [np.array([5]), np.arange(9000, dtype=np.int64) * W + 7, np.array([n - 1])]
     rst = np.array([0, 1, 9001, 9002], np.int32)
