@@ -251,20 +251,19 @@ static int device_cus(const rvvla_mat_s& M) {
 }
 
 // The LDS-sliced image: `request` 1 = on, 0 = off, -1 = chosen from the
-// matrix: FP64 values where the column-sliced image would be chosen by
-// default (an explicit xslices keeps the column-sliced image), with at
-// least one slice width of entries per workgroup (to amortise loading x
-// into LDS) and at least 3 entries per (row, slice) pair (each pair costs
-// a partial written and read back).  The power-law matrix of 4.2 M rows and
-// 64 M entries (3.9 entries per pair): 357 us against 440 us on MI355X;
-// with 32-bit x the column-sliced image stays faster (339 us against
-// 483 us FP32).
+// matrix: where the column-sliced image would be chosen by default (an
+// explicit xslices keeps the column-sliced image), with at least one slice
+// width of entries per workgroup (to amortise loading x into LDS) and at
+// least 3 entries per (row, slice) pair (each pair costs a partial written
+// and read back).  The power-law matrix of 4.2 M rows and 64 M entries
+// (3.9 entries per pair) on MI355X: FP64 348 us against 440 us, FP32 289
+// against 339, BF16 288 against 321.
 template <class V>
 static bool choose_xlds(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request, int xslices) {
   if (request >= 0) return request > 0 && a.n > 0 && a.m > 0;
-  if (xslices >= 0 || M.dtype != RVVLA_F64 || a.n == 0 || a.m == 0) return false;
+  if (xslices >= 0 || a.n == 0 || a.m == 0) return false;
   if (choose_xslices(M, a, -1) == 0) return false;
-  const int64_t nwg = (int64_t)device_cus(M) * xl_wg_per_cu(8);
+  const int64_t nwg = (int64_t)device_cus(M) * xl_wg_per_cu(M.dtype == RVVLA_F64 ? 8 : 4);
   if ((int64_t)a.nz < nwg * kXlSliceCols) return false;
   return (int64_t)a.nz >= 3 * xlds_pairs((int64_t)a.m, a.Rst, a.Col, kXlSliceCols);
 }

@@ -36,7 +36,7 @@ constexpr int kXlTile = 248;         // pairs per wavefront tile (LDS accumulato
 constexpr int kXlLdsLimit = 163840;  // LDS of one CU
 // slice-pass workgroups per CU: as many as the CU's LDS holds
 constexpr int xl_wg_per_cu(int io_bytes) {
-  return kXlLdsLimit / (kXlSliceCols * io_bytes + (kXlWG / 64) * kXlTile * io_bytes + 16);
+  return kXlLdsLimit / (kXlSliceCols * io_bytes + (kXlWG / 64) * kXlTile * 8 + 16);
 }
 struct DevXLds {
   int64_t S = 0, W = 0, npairs = 0, nunits = 0, n = 0, nwg = 0, nblocks = 0;

@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kWG) void k_xs_combine(
 // the additions is fixed by the data); at the end of the tile the
 // accumulators go to the pairs' slots.
 constexpr int kXlAhead = 8;
-static_assert(xl_wg_per_cu(8) >= 1 && xl_wg_per_cu(4) >= 2, "LDS of k_xl_slices");
+static_assert(xl_wg_per_cu(8) >= 1 && xl_wg_per_cu(4) >= 1, "LDS of k_xl_slices");
 template <class VT>
 __global__ __launch_bounds__(kXlWG) void k_xl_slices(
     const uint16_t* __restrict__ col, const VT* __restrict__ val, const int32_t* __restrict__ pst,
@@ -197,10 +197,12 @@ __global__ __launch_bounds__(kXlWG) void k_xl_slices(
   constexpr int XP = XW / kXlWG;
   static_assert(XW % kXlWG == 0, "x slice per thread");
   __shared__ IO xs[XW];
-  __shared__ A acc[(kXlWG / 64) * kXlTile];
+  // pair accumulators in double for every value type: LDS FP32 atomic adds
+  // kept the wavefronts waiting nine times longer than FP64 ones on MI355X
+  __shared__ double acc[(kXlWG / 64) * kXlTile];
   __shared__ int next_tile;
   const int tid = threadIdx.x, lane = tid & 63;
-  A* pacc = acc + (tid >> 6) * kXlTile;
+  double* pacc = acc + (tid >> 6) * kXlTile;
   const int u0 = wg[blockIdx.x], u1 = wg[blockIdx.x + 1];
   int64_t loaded = -1;
   uint32_t ca[kXlAhead];
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(kXlWG) void k_xl_slices(
       const int64_t gt = g0 + t * kXlTile;
       const int np = (int)(gt + kXlTile < g1 ? kXlTile : g1 - gt);
       const int e_beg = pst[gt], e_end = pst[gt + np];
-      for (int i = lane; i < kXlTile; i += 64) pacc[i] = A(0);
+      for (int i = lane; i < kXlTile; i += 64) pacc[i] = 0.0;
       int cur = -1;   // tile-local pair of the lane's running sum
       A run = A(0);
       int open = -1;  // tile-local pair open before the chunk
@@ -257,7 +259,7 @@ __global__ __launch_bounds__(kXlWG) void k_xl_slices(
           const int lp = open + below + (int)((H >> lane) & 1ull);
           if (in) {
             if (lp != cur) {
-              if (cur >= 0) atomicAdd(&pacc[cur], run);
+              if (cur >= 0) atomicAdd(&pacc[cur], (double)run);
               run = A(0);
               cur = lp;
             }
@@ -266,8 +268,8 @@ __global__ __launch_bounds__(kXlWG) void k_xl_slices(
           open += __popcll(H);
         }
       }
-      if (cur >= 0) atomicAdd(&pacc[cur], run);
-      for (int i = lane; i < np; i += 64) part[wpos[gt + i]] = pacc[i];
+      if (cur >= 0) atomicAdd(&pacc[cur], (double)run);
+      for (int i = lane; i < np; i += 64) part[wpos[gt + i]] = (A)pacc[i];
       t = __shfl(nt, 0, 64);
     }
   }

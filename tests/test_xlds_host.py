@@ -95,7 +95,7 @@ def test_xlds_layout_and_product(name, dtype):
     M = R.convert("csr", a, dtype, "xlds=1").upload()
     v = _view(M, dtype)
     m, npairs = a.m, len(v["wpos"])
-    nwg = CUS * (1 if dtype == R.F64 else 2)  # workgroups the CU's LDS holds
+    nwg = CUS  # one workgroup per CU (its LDS holds one x slice and the accumulators)
     # entries and pairs
     pst = v["pst"]
     assert pst[0] == 0 and pst[-1] == a.nnz and np.all(np.diff(pst) > 0)

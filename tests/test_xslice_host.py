@@ -144,7 +144,7 @@ def test_sliced_16bit_values():
 def test_auto_choice():
     """Auto: sliced when x exceeds one L2 and columns are far from the
     diagonal; plain for a banded matrix or a small x; xslices=1 disables;
-    LDS-sliced for FP64 with enough entries per pair and per workgroup."""
+    LDS-sliced with enough entries per pair and per workgroup."""
     n = 600_000
     g = R.gen_random_rows(n, 2, seed=1)  # diagonal + one uniform column
     assert len(R.convert("csr", g, R.F64).upload().view("xs.base")) == 9
@@ -156,13 +156,16 @@ def test_auto_choice():
     assert len(R.convert("csr", lap, R.F64).upload().view("xs.base")) == 0
     small = _matrix()
     assert len(R.convert("csr", small, R.F64).upload().view("xs.base")) == 0
-    # FP64 with several entries per (row, 16384-column slice) pair and at
-    # least 16384 entries per workgroup: the LDS-sliced image instead
+    # several entries per (row, 16384-column slice) pair and at least 16384
+    # entries per workgroup: the LDS-sliced image instead
     pl = R.gen_powerlaw(1_000_000, 1.8, 20000, 16_000_000, 1024, seed=3)
     M = R.convert("csr", pl, R.F64).upload()
     assert len(M.view("xs.base")) == 0 and len(M.view("xl.rowptr")) == pl.m + 1
-    assert len(R.convert("csr", pl, R.F32).upload().view("xl.rowptr")) == 0
+    # 32-bit x of 1 M columns (4 MiB) fits one L2: neither sliced image
+    M = R.convert("csr", pl, R.F32).upload()
+    assert len(M.view("xs.base")) == 0 and len(M.view("xl.rowptr")) == 0
     assert len(R.convert("csr", pl, R.F64, "xlds=0").upload().view("xl.rowptr")) == 0
+    assert len(R.convert("csr", pl, R.F64, "xslices=8").upload().view("xl.rowptr")) == 0
 
 
 def test_empty_and_bad_request():
