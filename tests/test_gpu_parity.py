@@ -203,6 +203,18 @@ def _wide():
     return _CACHE["wide"]
 
 
+def test_csr_lds_slices_full_size():
+    """The power-law matrix of the benchmark (4.2 M rows, 64 M entries): the
+    image chosen by default (LDS-sliced, FP64) against the oracle."""
+    a = R.gen_powerlaw(4194304, 1.8, 50000, 64_000_000, 1024, seed=0)
+    print("csr xlds full size", a.m, a.nnz, flush=True)
+    x64, y064 = inputs(a, 10)
+    y = run_gpu("csr", R.F64, a, x64, y064, 1.0, 0.5, "")
+    y_ref = O.csr_spmv(a.rst, a.col, a.val, x64, 1.0, 0.5, y064)
+    assert_within_bound(y, y_ref, a.rst, a.col, a.val, x64, 1.0, 0.5, y064, np.float64,
+                        "csr default image, power-law 4.2 M rows")
+
+
 @pytest.mark.parametrize("dtype", [R.BF16, R.F16])
 def test_csr_lds_slices_16bit(dtype):
     a = case("powerlaw")
