@@ -259,8 +259,8 @@ int rvvla_mat_info(rvvla_mat_t A, int64_t* m, int64_t* n, int64_t* nnz,
  * y is write-only (BLAS convention); the reference's scalar CSR reads y
  * there too (spmv_mtx_crs.cpp:37-43), so the two differ only when y already
  * holds NaN or Inf.  Products are ordered on `stream`; some formats keep
- * per-matrix partial sums on the device (CSR5, VHCC, VNEC, the column-sliced
- * CSR image, the split slices of SELL and chunks of LAV), so products of one
+ * per-matrix partial sums on the device (CSR5, VHCC, VNEC, the column- and
+ * LDS-sliced CSR images, the split slices of SELL and chunks of LAV), so products of one
  * matrix must not run concurrently on different streams.  The environment
  * variable RVVLA_NT=0 turns off the nontemporal loads of the FP64 SELL and
  * CSR kernels (for comparisons). */
