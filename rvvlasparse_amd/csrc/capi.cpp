@@ -282,7 +282,9 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds) {
   if (choose_xlds(*M, a, xlds, xslices)) {
     XLdsHost<V> X;
     const int64_t nwg = (int64_t)device_cus(*M) * xl_wg_per_cu(dtype == RVVLA_F64 ? 8 : 4);
-    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, kXlSliceCols, nwg);
+    const int acc = dtype == RVVLA_F64 ? 8 : 4;
+    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, kXlSliceCols, nwg,
+                  xl_comb_rows(acc), xl_comb_pairs(acc));
     DevXLds& d = M->csr.xl;
     d.S = X.S;
     d.W = X.W;

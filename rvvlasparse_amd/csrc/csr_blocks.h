@@ -12,9 +12,11 @@ constexpr int kStreamCap = 2048;
 constexpr int kStreamRows = 256;
 constexpr int kLongRow = 1024;
 
-// Combine blocks of the LDS-sliced image: up to kXlCombRows rows holding
-// at most kXlCombPairs partials (staged in LDS), or one row with more.
-constexpr int kXlCombRows = 256;
-constexpr int kXlCombPairs = 2048;
+// Combine blocks of the LDS-sliced image: up to xl_comb_rows rows (one
+// thread each) holding at most 8 partials a row on average (staged in LDS),
+// or one row with more.  FP64 partials: 256 rows; 32-bit ones: 512 (longer
+// runs of consecutive slots for the slice pass to write, measured faster).
+constexpr int xl_comb_rows(int acc_bytes) { return acc_bytes == 8 ? 256 : 512; }
+constexpr int xl_comb_pairs(int acc_bytes) { return 8 * xl_comb_rows(acc_bytes); }
 
 }  // namespace rvvla

@@ -12,7 +12,7 @@
 // loads each slice of x it meets into LDS once (a coalesced read) and
 // streams the entries, gathering x from LDS.  Each (row, slice) pair leaves
 // one partial sum.  The partials are grouped by row blocks (at most
-// kXlCombRows rows and kXlCombPairs pairs, or one longer row); inside a
+// comb_rows rows and comb_pairs pairs, or one longer row); inside a
 // block they are ordered by slice, then row, so that each slice writes
 // runs of consecutive slots.  A second kernel loads a block's partials,
 // puts each at its row-major place in LDS and adds each row's partials in
@@ -47,7 +47,7 @@ namespace rvvla {
 
 template <class V>
 void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col, const V* val,
-                int64_t W, int64_t nwg) {
+                int64_t W, int64_t nwg, int64_t comb_rows, int64_t comb_pairs) {
   const int64_t nnz = m > 0 ? rst[m] : 0;
   const int64_t S = n > 0 ? (n + W - 1) / W : 0;
   nwg = std::max<int64_t>(1, nwg);
@@ -137,11 +137,10 @@ void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int*
   X.cblk.clear();
   for (int64_t r = 0; r < m;) {
     const int64_t r0 = r;
-    if (rp[(size_t)r] > kXlCombPairs) {
+    if (rp[(size_t)r] > comb_pairs) {
       ++r;
     } else {
-      while (r < m && r - r0 < kXlCombRows &&
-             X.rowptr[(size_t)r + 1] - X.rowptr[(size_t)r0] <= kXlCombPairs)
+      while (r < m && r - r0 < comb_rows && X.rowptr[(size_t)r + 1] - X.rowptr[(size_t)r0] <= comb_pairs)
         ++r;
     }
     X.cblk.push_back((int32_t)r0);
@@ -216,8 +215,8 @@ int64_t xlds_pairs(int64_t m, const int* rst, const int* col, int64_t W) {
 }
 
 template void build_xlds<double>(XLdsHost<double>&, int64_t, int64_t, const int*, const int*,
-                                 const double*, int64_t, int64_t);
+                                 const double*, int64_t, int64_t, int64_t, int64_t);
 template void build_xlds<float>(XLdsHost<float>&, int64_t, int64_t, const int*, const int*,
-                                const float*, int64_t, int64_t);
+                                const float*, int64_t, int64_t, int64_t, int64_t);
 
 }  // namespace rvvla

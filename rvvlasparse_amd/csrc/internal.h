@@ -128,10 +128,11 @@ struct XLdsHost {
 };
 // (row, slice) pairs of a CSR matrix for slices of W columns.
 int64_t xlds_pairs(int64_t m, const int* rst, const int* col, int64_t W);
-// nwg: the workgroups of the slice pass
+// nwg: the workgroups of the slice pass; combine blocks of at most
+// comb_rows rows and comb_pairs partials (or one longer row)
 template <class V>
 void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col, const V* val,
-                int64_t W, int64_t nwg);
+                int64_t W, int64_t nwg, int64_t comb_rows, int64_t comb_pairs);
 // Fraction of entries whose column is more than `window` away from the
 // row's diagonal position r * n / m.
 double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window);

@@ -276,35 +276,36 @@ __global__ __launch_bounds__(kXlWG) void k_xl_slices(
 }
 
 // Combine block b of the LDS-sliced image: rows r0 .. r1-1 (at most
-// kXlCombRows, one per thread) with slots base .. end-1.  Each thread loads
-// its (at most kXlCombPairs / kWG) partials together and puts each at its
-// row-major place in LDS (lpos); then each row's partials are added in
-// slice order and alpha and beta applied.  A block of one row with more
-// than kXlCombPairs partials (then already in slice order) is summed by
-// the whole workgroup.
-static_assert(kXlCombRows <= kWG && kXlCombPairs % kWG == 0, "combine block shape");
+// xl_comb_rows, one per thread) with slots base .. end-1.  Each thread
+// loads its (at most 8) partials together and puts each at its row-major
+// place in LDS (lpos); then each row's partials are added in slice order
+// and alpha and beta applied.  A block of one row with more than
+// xl_comb_pairs partials (then already in slice order) is summed by the
+// whole workgroup.
 template <class A, class IO>
-__global__ __launch_bounds__(kWG) void k_xl_combine(const int32_t* __restrict__ rowptr,
+__global__ __launch_bounds__(xl_comb_rows(sizeof(A))) void k_xl_combine(const int32_t* __restrict__ rowptr,
                                                       const int32_t* __restrict__ cblk,
                                                       const uint16_t* __restrict__ lpos,
                                                       const A* __restrict__ part,
                                                       IO* __restrict__ y, A alpha, A beta) {
-  constexpr int PT = kXlCombPairs / kWG;
+  constexpr int kXlCombWG = xl_comb_rows(sizeof(A));
+  constexpr int kXlCombPairs = xl_comb_pairs(sizeof(A));
+  constexpr int PT = kXlCombPairs / kXlCombWG;
   __shared__ A prod[kXlCombPairs];
-  __shared__ A wsum[kWG / 64];
+  __shared__ A wsum[kXlCombWG / 64];
   const int tid = threadIdx.x;
   const int r0 = cblk[4 * blockIdx.x], r1 = cblk[4 * blockIdx.x + 1];
   const int base = cblk[4 * blockIdx.x + 2], cnt = cblk[4 * blockIdx.x + 3];
   if (cnt > kXlCombPairs) {
     Acc<A, false> a;
-    for (int q = tid; q < cnt; q += kWG) a.add(part[base + q]);
+    for (int q = tid; q < cnt; q += kXlCombWG) a.add(part[base + q]);
     a = group_reduce(a, 64);
     if ((tid & 63) == 0) wsum[tid >> 6] = a.s;
     __syncthreads();
     if (tid == 0) {
       A t = wsum[0];
 #pragma unroll
-      for (int w = 1; w < kWG / 64; ++w) t += wsum[w];
+      for (int w = 1; w < kXlCombWG / 64; ++w) t += wsum[w];
       y[r0] = alpha * t + scaled_y(beta, y, r0);
     }
     return;
@@ -319,14 +320,14 @@ __global__ __launch_bounds__(kWG) void k_xl_combine(const int32_t* __restrict__ 
   int at[PT];
 #pragma unroll
   for (int j = 0; j < PT; ++j) {
-    const int q = tid + j * kWG;
+    const int q = tid + j * kXlCombWG;
     const int qq = q < cnt ? q : (cnt > 0 ? cnt - 1 : 0);  // in range, masked below
     v[j] = cnt > 0 ? part[base + qq] : A(0);
     at[j] = cnt > 0 ? (int)lpos[base + qq] : 0;
   }
 #pragma unroll
   for (int j = 0; j < PT; ++j)
-    if (tid + j * kWG < cnt) prod[at[j]] = v[j];
+    if (tid + j * kXlCombWG < cnt) prod[at[j]] = v[j];
   __syncthreads();
   if (r < r1) {
     A s = A(0);
@@ -375,7 +376,8 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
                          xl.W, (AC*)xl.part);
       if (int rc = hip_check(hipGetLastError(), "k_xl_slices launch")) return rc;
     }
-    hipLaunchKernelGGL((k_xl_combine<AC, IO>), dim3((unsigned)xl.nblocks), dim3(kWG), 0, s,
+    hipLaunchKernelGGL((k_xl_combine<AC, IO>), dim3((unsigned)xl.nblocks),
+                       dim3(xl_comb_rows(sizeof(AC))), 0, s,
                        xl.rowptr, xl.cblk, xl.lpos, (const AC*)xl.part, (IO*)y, (AC)alpha,
                        (AC)beta);
     return hip_check(hipGetLastError(), "k_xl_combine launch");

@@ -15,7 +15,7 @@ import rvvlasparse_amd as R
 from helpers import assert_within_bound, laplacian, rand_csr
 from oracle import oracle as O
 
-COMB_PAIRS, COMB_ROWS, CUS, W = 2048, 256, 256, 16384
+CUS, W = 256, 16384
 
 
 @pytest.fixture(autouse=True)
@@ -39,7 +39,12 @@ def _view(M, dtype):
     return v
 
 
-def _emulate(v, m, n, W, x, alpha, beta, y0):
+def _comb(dtype):
+    """Combine block limits: 256 rows / 2048 partials (FP64), 512 / 4096."""
+    return (256, 2048) if dtype == R.F64 else (512, 4096)
+
+
+def _emulate(v, m, n, W, x, alpha, beta, y0, COMB_PAIRS):
     """The slice pass (a partial per pair into its slot) and the combine
     (partials to their row-major places, each row's added in order)."""
     npairs = len(v["wpos"])
@@ -79,7 +84,7 @@ CASES = {
 
 def _many_pairs():
     """One row with 9000 pairs (one entry in each of 9000 slices) among
-    short rows: a combine block of its own (more than 2048 partials),
+    short rows: a combine block of its own (more than 4096 partials),
     summed in place."""
     n = 9000 * W
     cols = [np.array([5]), np.arange(9000, dtype=np.int64) * W + 7, np.array([n - 1])]
@@ -95,6 +100,7 @@ def test_xlds_layout_and_product(name, dtype):
     M = R.convert("csr", a, dtype, "xlds=1").upload()
     v = _view(M, dtype)
     m, npairs = a.m, len(v["wpos"])
+    COMB_ROWS, COMB_PAIRS = _comb(dtype)
     nwg = CUS  # one workgroup per CU (its LDS holds one x slice and the accumulators)
     # entries and pairs
     pst = v["pst"]
@@ -147,7 +153,7 @@ def test_xlds_layout_and_product(name, dtype):
     x = rng.uniform(-1, 1, n).astype(dt)
     y0 = rng.uniform(-1, 1, m).astype(dt)
     vals = a.values_for(dtype)
-    y = _emulate(v, m, n, W, x.astype(np.float64), 1.25, -0.5, y0.astype(np.float64))
+    y = _emulate(v, m, n, W, x.astype(np.float64), 1.25, -0.5, y0.astype(np.float64), COMB_PAIRS)
     y_ref = O.csr_spmv(a.rst, a.col, vals, x, 1.25, -0.5, y0)
     assert_within_bound(y.astype(dt), y_ref, a.rst, a.col, vals, x, 1.25, -0.5, y0, dt,
                         f"xlds {name}")
