@@ -203,6 +203,27 @@ def _wide():
     return _CACHE["wide"]
 
 
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_csr_lds_slices_many_pairs(dtype):
+    """A row with an entry in each of 5000 slices of 16384 columns (more
+    partials than a combine block holds: summed by the whole workgroup)
+    between short rows, against the oracle."""
+    W, S = 16384, 5000
+    n = W * S
+    rng = np.random.default_rng(13)
+    cols = [np.array([3, 70000]), np.arange(S, dtype=np.int64) * W + rng.integers(0, W, S),
+            np.array([n - 2])]
+    rst = np.array([0, 2, 2 + S, 3 + S], np.int32)
+    a = R.CSR(3, n, rst, np.concatenate(cols).astype(np.int32), rng.uniform(-1, 1, S + 3))
+    print("csr xlds many pairs", dtype, flush=True)
+    x64, y064 = inputs(a, 14)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    y = run_gpu("csr", dtype, a, x64, y064, 0.5, -1.0, "xlds=1")
+    v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x, 0.5, -1.0, y0)
+    assert_within_bound(y, y_ref, a.rst, a.col, v, x, 0.5, -1.0, y0, dt, "csr xlds many pairs")
+
+
 def test_csr_lds_slices_full_size():
     """The power-law matrix of the benchmark (4.2 M rows, 64 M entries): the
     image chosen by default (LDS-sliced, FP64) against the oracle."""
