@@ -230,6 +230,29 @@ static int device_xcds(const rvvla_mat_s& M) {
 // over 65536 columns away from the row's diagonal position: one slice per
 // XCD, or two per XCD when a slice of x would exceed the 4 MiB L2.
 template <class V>
+static void fill_sell(rvvla_mat_s& M, int dtype, const spMtxSELL_C_Sigma<V>& a);
+
+// The SELL-64 image of a CSR matrix (rows in natural order, sigma = 1):
+// `request` 1 = on, 0 = off, -1 = chosen from the matrix: at least 65536
+// rows whose 64-row slices pad by at most 1 % of the entries (the 5-point
+// Laplacian 2048^2: 53 us against 59 us for the adaptive CSR kernel on
+// MI355X, FP64).  Each row's entries are added in the same order.
+template <class V>
+static bool choose_csr_sell(const spMtxCRS<V>& a, int request) {
+  if (request >= 0) return request > 0 && a.m > 0;
+  const int64_t m = (int64_t)a.m, nnz = (int64_t)a.nz;
+  if (m < 65536 || nnz == 0) return false;
+  int64_t padded = 0;
+  for (int64_t r0 = 0; r0 < m; r0 += 64) {
+    const int64_t r1 = std::min<int64_t>(m, r0 + 64);
+    int64_t w = 0;
+    for (int64_t r = r0; r < r1; ++r) w = std::max<int64_t>(w, a.Rst[r + 1] - a.Rst[r]);
+    padded += w * (r1 - r0);
+  }
+  return (double)(padded - nnz) <= 0.01 * (double)nnz;
+}
+
+template <class V>
 static int choose_xslices(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request) {
   if (request > 16) throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xslices must be at most 16");
   if (request == 0 || request == 1) return 0;
@@ -269,7 +292,7 @@ static bool choose_xlds(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request,
 }
 
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds) {
+rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, int sell) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   check_csr((int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, a.Rst, a.Col);
   auto M = new_mat(RVVLA_FMT_CSR, dtype, (int64_t)a.m, (int64_t)a.n, (int64_t)a.nz);
@@ -306,6 +329,12 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds) {
     return M.release();
   }
   const int P = choose_xslices(*M, a, xslices);
+  if (P == 0 && choose_csr_sell(a, sell)) {
+    spMtxSELL_C_Sigma<V> S;
+    build_sell<V>(S, a, 64, 1);
+    fill_sell(*M, dtype, S);
+    M->csr.sell = true;
+  }
   if (P > 0) {
     XSliceHost<V> X;
     build_xslice<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, P,
@@ -553,6 +582,15 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
     }
   }
   auto M = new_mat(RVVLA_FMT_SELL, dtype, m, n, (int64_t)a.nz);
+  fill_sell(*M, dtype, a);
+  return M.release();
+}
+
+// The SELL-C-sigma device image of a checked container into M.sell.
+template <class V>
+static void fill_sell(rvvla_mat_s& Mr, int dtype, const spMtxSELL_C_Sigma<V>& a) {
+  const int64_t m = (int64_t)a.m, nb = (int64_t)a.cnt_b;
+  rvvla_mat_s* M = &Mr;
   DevSELL& d = M->sell;
   d.nslices = nb;
   d.cl = M->put_vec(a.vCl);
@@ -565,7 +603,6 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
   for (int64_t s = 0; s < m && d.perm_identity; ++s) d.perm_identity = a.vPerm[(size_t)s] == (uint32_t)s;
   d.val = put_values(*M, dtype, a.vVal.data(), a.vVal.size());
   attach_sell_pieces(*M, a.vCl);
-  return M.release();
 }
 
 static std::vector<int32_t> empty_row_list(int64_t m, const int* rst) {
@@ -997,8 +1034,8 @@ template rvvla_mat_t upload_vnec<float>(int, const spMtxVNEC<float>&, const spMt
 
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
-template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int, int);
-template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int, int);
+template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int, int, int);
+template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int, int, int);
 RVVLA_INST_UPLOAD(upload_hcsr, spMtxHCSR)
 RVVLA_INST_UPLOAD(upload_sell, spMtxSELL_C_Sigma)
 RVVLA_INST_UPLOAD(upload_csr5, spMtxCSR5)
@@ -1048,7 +1085,8 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
       L.scalars = {{"m", (double)a->m}, {"n", (double)a->n}, {"nz", (double)a->nz}};
       const int xs = param_int(P, "xslices", -1);
       const int xl = param_int(P, "xlds", -1);
-      L.upload = [a, dtype, xs, xl]() { return upload_csr<V>(dtype, *a, xs, xl); };
+      const int sl = param_int(P, "sell", -1);
+      L.upload = [a, dtype, xs, xl, sl]() { return upload_csr<V>(dtype, *a, xs, xl, sl); };
       break;
     }
     case RVVLA_FMT_HCSR: {
@@ -1811,6 +1849,7 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
             : f == "xl.cblk"   ? V{c.xl.cblk, 4 * c.xl.nblocks, 4}
             : f == "xl.units"  ? V{c.xl.units, 3 * c.xl.nunits, 4}
             : f == "xl.wg"     ? V{c.xl.wg, c.xl.S ? c.xl.nwg + 1 : 0, 4}
+            : f == "sell.cl"   ? V{A->sell.cl, c.sell ? A->sell.nslices : 0, 4}
                                : V{nullptr, -1, 0};
   if (v.n < 0) {
     rvvla::set_last_error("rvvla_mat_view: unknown field " + f);
@@ -1853,7 +1892,9 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
   if (int rc = need_device(A)) return rc;
   hipStream_t s = (hipStream_t)stream;
   switch (A->format) {
-    case RVVLA_FMT_CSR: return rvvla::launch_csr(*A, alpha, x, beta, y, s);
+    case RVVLA_FMT_CSR:
+      return A->csr.sell ? rvvla::launch_sell(*A, alpha, x, beta, y, s)
+                         : rvvla::launch_csr(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_HCSR: return rvvla::launch_hcsr(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_SELL: return rvvla::launch_sell(*A, alpha, x, beta, y, s);
     case RVVLA_FMT_CSR5: return rvvla::launch_csr5(*A, alpha, x, beta, y, s);

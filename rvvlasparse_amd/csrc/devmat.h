@@ -64,6 +64,7 @@ struct DevCSR {
   int64_t nblocks = 0;
   DevXSlice xs;
   DevXLds xl;
+  bool sell = false;  // products run on the matrix's SELL-64 image (rvvla_mat_s::sell)
 };
 
 // HCSR: non-empty blocks only, grouped by block-row.

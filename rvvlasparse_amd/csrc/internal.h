@@ -78,10 +78,12 @@ std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts
 // xslices: column slices of the sliced image (-1 = chosen from the matrix
 // and the device, 0 or 1 = plain CSR only, 2..8 = that many).  xlds: the
 // LDS-sliced image (-1 = chosen from the matrix, 0 = off, 1 = on; it takes
-// the place of the sliced image).
+// the place of the sliced image).  sell: the natural-order SELL-64 image
+// for matrices with even row lengths (-1 = chosen from the matrix, 0 = off,
+// 1 = on; only where no sliced image is used).
 template <class V>
 rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a, int xslices = -1,
-                       int xlds = -1);
+                       int xlds = -1, int sell = -1);
 template <class V>
 rvvla_mat_t upload_hcsr(int dtype, const SparseMatrixLib::spMtxHCSR<V>& a);
 template <class V>

@@ -168,6 +168,27 @@ def test_csr_column_slices_16bit(dtype):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("params", ["sell=1", "sell=0"])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+def test_csr_sell_image(name, params, dtype):
+    """CSR products with and without the matrix's SELL-64 image, against
+    the oracle."""
+    a = case(name)
+    x64, y064 = inputs(a, 15)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    y = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, params)
+    if dtype == R.BF16:
+        x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
+        y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 1.0, 0.5, y032.astype(np.float64))
+        if np.linalg.norm(y64) > 0:
+            assert np.linalg.norm(y - y64) / np.linalg.norm(y64) <= 5e-3
+        return
+    v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(a.rst, a.col, v, x, 1.0, 0.5, y0)
+    assert_within_bound(y, y_ref, a.rst, a.col, v, x, 1.0, 0.5, y0, dt, f"csr {params} {name}")
+
+
 @pytest.mark.parametrize("name", CASES + ["wide"])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 def test_csr_lds_slices(name, dtype):

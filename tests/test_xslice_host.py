@@ -176,3 +176,17 @@ def test_empty_and_bad_request():
     with pytest.raises(R.RvvlaError) as ex:
         R.convert("csr", _matrix(), R.F64, "xslices=17").upload()
     assert ex.value.code == -10
+
+
+def test_csr_sell_image_choice():
+    """CSR keeps a natural-order SELL-64 image (its products run the SELL
+    kernel) for matrices of at least 65536 rows whose 64-row slices pad by
+    at most 1 %; not for uneven rows, small matrices or sell=0."""
+    lap = R.gen_laplacian5(300)  # 90000 rows of 3..5 entries
+    assert len(R.convert("csr", lap, R.F64).upload().view("sell.cl")) == -(-lap.m // 64)
+    assert len(R.convert("csr", lap, R.F64, "sell=0").upload().view("sell.cl")) == 0
+    uneven = R.gen_random_rows_var(70000, 1, 64, seed=4)
+    assert len(R.convert("csr", uneven, R.F64).upload().view("sell.cl")) == 0
+    small = R.gen_laplacian5(100)  # 10000 rows
+    assert len(R.convert("csr", small, R.F32).upload().view("sell.cl")) == 0
+    assert len(R.convert("csr", small, R.F32, "sell=1").upload().view("sell.cl")) == -(-small.m // 64)
