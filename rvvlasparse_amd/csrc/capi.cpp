@@ -236,7 +236,8 @@ static void fill_sell(rvvla_mat_s& M, int dtype, const spMtxSELL_C_Sigma<V>& a);
 // `request` 1 = on, 0 = off, -1 = chosen from the matrix: at least 65536
 // rows whose 64-row slices pad by at most 1 % of the entries (the 5-point
 // Laplacian 2048^2: 53 us against 59 us for the adaptive CSR kernel on
-// MI355X, FP64).  Each row's entries are added in the same order.
+// MI355X, FP64).  The SELL kernel adds a row's entries in four interleaved
+// partial sums (entries j mod 4), a fixed order within the usual bound.
 template <class V>
 static bool choose_csr_sell(const spMtxCRS<V>& a, int request) {
   if (request >= 0) return request > 0 && a.m > 0;
