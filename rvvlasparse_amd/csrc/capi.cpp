@@ -279,17 +279,15 @@ static int device_cus(const rvvla_mat_s& M) {
 // explicit xslices keeps the column-sliced image), with at least one slice
 // width of entries per workgroup (to amortise loading x into LDS) and at
 // least 3 entries per (row, slice) pair (each pair costs a partial written
-// and read back).  The power-law matrix of 4.2 M rows and 64 M entries
-// (3.9 entries per pair) on MI355X: FP64 348 us against 440 us, FP32 289
-// against 339, BF16 288 against 321.
+// and read back).
 template <class V>
 static bool choose_xlds(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request, int xslices) {
   if (request >= 0) return request > 0 && a.n > 0 && a.m > 0;
   if (xslices >= 0 || a.n == 0 || a.m == 0) return false;
   if (choose_xslices(M, a, -1) == 0) return false;
-  const int64_t nwg = (int64_t)device_cus(M) * xl_wg_per_cu(M.dtype == RVVLA_F64 ? 8 : 4);
-  if ((int64_t)a.nz < nwg * kXlSliceCols) return false;
-  return (int64_t)a.nz >= 3 * xlds_pairs((int64_t)a.m, a.Rst, a.Col, kXlSliceCols);
+  const int64_t W = xl_slice_cols(M.dtype == RVVLA_F64 ? 8 : 4);
+  if ((int64_t)a.nz < (int64_t)device_cus(M) * W) return false;
+  return (int64_t)a.nz >= 3 * xlds_pairs((int64_t)a.m, a.Rst, a.Col, W);
 }
 
 template <class V>
@@ -305,28 +303,35 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, i
   M->csr.nblocks = (int64_t)blocks.size() / 2;
   if (choose_xlds(*M, a, xlds, xslices)) {
     XLdsHost<V> X;
-    const int64_t nwg = (int64_t)device_cus(*M) * xl_wg_per_cu(dtype == RVVLA_F64 ? 8 : 4);
-    const int acc = dtype == RVVLA_F64 ? 8 : 4;
-    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, kXlSliceCols, nwg,
-                  xl_comb_rows(acc), xl_comb_pairs(acc));
+    const int io = dtype == RVVLA_F64 ? 8 : 4;
+    const int64_t W = xl_slice_cols(io);
+    // the combine's run table holds (blocks + 1) * slices offsets
+    const int64_t est_blocks = (int64_t)a.m / kXlCombRows + 2 * (int64_t)a.nz / xl_comb_pieces(io) + 2;
+    if (est_blocks * (((int64_t)a.n + W - 1) / W) > (int64_t(1) << 27))
+      throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "CSR: LDS-sliced image run table too large");
+    const int vpw = dtype == RVVLA_F64 ? 2 : dtype == RVVLA_F32 ? 4 : 8;
+    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, W, device_cus(*M),
+                  kXlCombRows, xl_comb_pieces(io), vpw);
     DevXLds& d = M->csr.xl;
     d.S = X.S;
     d.W = X.W;
     d.n = (int64_t)a.n;
-    d.npairs = X.npairs;
+    d.npieces = X.npieces;
+    d.nent = X.nent;
+    d.ntasks = (int64_t)X.task.size() / 4;
     d.nunits = (int64_t)X.units.size() / 3;
     d.nwg = (int64_t)X.wg.size() - 1;
-    d.nblocks = (int64_t)X.cblk.size() / 4;
+    d.nblocks = (int64_t)X.cblk.size() / 2;
     d.col = M->put_vec(X.col);
     d.val = put_values(*M, dtype, X.val.data(), X.val.size());
-    d.pst = M->put_vec(X.pst);
-    d.wpos = M->put_vec(X.wpos);
-    d.lpos = M->put_vec(X.lpos);
-    d.rowptr = M->put_vec(X.rowptr);
-    d.cblk = M->put_vec(X.cblk);
+    d.task = M->put_vec(X.task);
     d.units = M->put_vec(X.units);
     d.wg = M->put_vec(X.wg);
-    d.part = M->put(nullptr, (size_t)std::max<int64_t>(d.npairs, 1) * (dtype == RVVLA_F64 ? 8 : 4));
+    d.rowptr = M->put_vec(X.rowptr);
+    d.cblk = M->put_vec(X.cblk);
+    d.off = M->put_vec(X.off);
+    d.lpos = M->put_vec(X.lpos);
+    d.part = M->put(nullptr, (size_t)std::max<int64_t>(d.npieces, 1) * io);
     return M.release();
   }
   const int P = choose_xslices(*M, a, xslices);
@@ -1841,15 +1846,15 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
             : f == "xs.mask" ? V{x.mask, x.P ? A->m : 0, 2}
             : f == "xs.base" ? V{x.base, x.P ? x.P + 1 : 0, 4}
             : f == "xs.pref" ? V{x.pref, x.P * x.nq, 4}
-            : f == "xl.col"    ? V{c.xl.col, c.xl.S ? A->nnz : 0, 2}
-            : f == "xl.val"    ? V{c.xl.val, c.xl.S ? A->nnz : 0, vb}
-            : f == "xl.pst"    ? V{c.xl.pst, c.xl.S ? c.xl.npairs + 1 : 0, 4}
-            : f == "xl.wpos"   ? V{c.xl.wpos, c.xl.npairs, 4}
-            : f == "xl.lpos"   ? V{c.xl.lpos, c.xl.npairs, 2}
-            : f == "xl.rowptr" ? V{c.xl.rowptr, c.xl.S ? A->m + 1 : 0, 4}
-            : f == "xl.cblk"   ? V{c.xl.cblk, 4 * c.xl.nblocks, 4}
+            : f == "xl.col"    ? V{c.xl.col, c.xl.S ? c.xl.nent : 0, 2}
+            : f == "xl.val"    ? V{c.xl.val, c.xl.S ? c.xl.nent : 0, vb}
+            : f == "xl.task"   ? V{c.xl.task, 4 * c.xl.ntasks, 4}
             : f == "xl.units"  ? V{c.xl.units, 3 * c.xl.nunits, 4}
             : f == "xl.wg"     ? V{c.xl.wg, c.xl.S ? c.xl.nwg + 1 : 0, 4}
+            : f == "xl.rowptr" ? V{c.xl.rowptr, c.xl.S ? A->m + 1 : 0, 4}
+            : f == "xl.cblk"   ? V{c.xl.cblk, 2 * c.xl.nblocks, 4}
+            : f == "xl.off"    ? V{c.xl.off, c.xl.S * (c.xl.nblocks + 1), 4}
+            : f == "xl.lpos"   ? V{c.xl.lpos, c.xl.npieces, 2}
             : f == "sell.cl"   ? V{A->sell.cl, c.sell ? A->sell.nslices : 0, 4}
                                : V{nullptr, -1, 0};
   if (v.n < 0) {

@@ -12,11 +12,15 @@ constexpr int kStreamCap = 2048;
 constexpr int kStreamRows = 256;
 constexpr int kLongRow = 1024;
 
-// Combine blocks of the LDS-sliced image: up to xl_comb_rows rows (one
-// thread each) holding at most 8 partials a row on average (staged in LDS),
-// or one row with more.  FP64 partials: 256 rows; 32-bit ones: 512 (longer
-// runs of consecutive slots for the slice pass to write, measured faster).
-constexpr int xl_comb_rows(int acc_bytes) { return acc_bytes == 8 ? 256 : 512; }
-constexpr int xl_comb_pairs(int acc_bytes) { return 8 * xl_comb_rows(acc_bytes); }
+// LDS-sliced image (csr_xlds.cpp): x slices of kXlLdsBytes in LDS (16384
+// doubles or 32768 floats), tasks of kXlTask slice-major entries (one
+// wavefront each, 8 entries a lane), combine blocks of at most kXlCombRows
+// rows holding at most xl_comb_pieces partials (64 KiB of LDS), or one row
+// with more.
+constexpr int kXlLdsBytes = 131072;
+constexpr int kXlTask = 512;
+constexpr int xl_slice_cols(int io_bytes) { return kXlLdsBytes / io_bytes; }
+constexpr int kXlCombRows = 2048;
+constexpr int xl_comb_pieces(int acc_bytes) { return 65536 / acc_bytes; }
 
 }  // namespace rvvla

@@ -28,27 +28,23 @@ struct DevXSlice {
   void* part = nullptr;
 };
 
-// LDS-sliced image (csr_xlds.cpp): S slices of W columns, each unit of
-// pairs run by one workgroup holding its slice of x in LDS.
-constexpr int kXlWG = 1024;          // threads of the slice kernel
-constexpr int kXlSliceCols = 16384;  // x slice in LDS: 128 KiB (FP64) or 64 KiB (FP32)
-constexpr int kXlTile = 248;         // pairs per wavefront tile (LDS accumulators)
-constexpr int kXlLdsLimit = 163840;  // LDS of one CU
-// slice-pass workgroups per CU: as many as the CU's LDS holds
-constexpr int xl_wg_per_cu(int io_bytes) {
-  return kXlLdsLimit / (kXlSliceCols * io_bytes + (kXlWG / 64) * kXlTile * 8 + 16);
-}
+// LDS-sliced image (csr_xlds.cpp): S slices of W columns; the slice pass
+// (one workgroup per CU, its LDS holding one slice of x) runs
+// tasks of kXlTask entries, one partial per piece; the combine adds each
+// row's partials in slice order.
+constexpr int kXcWG = 1024;  // threads of the combine kernel
+constexpr int kXcMaxSlices = kXcWG;  // slices the combine scans in LDS
 struct DevXLds {
-  int64_t S = 0, W = 0, npairs = 0, nunits = 0, n = 0, nwg = 0, nblocks = 0;
+  int64_t S = 0, W = 0, npieces = 0, ntasks = 0, nunits = 0, n = 0, nwg = 0, nblocks = 0, nent = 0;
   const uint16_t* col = nullptr;
   const void* val = nullptr;
-  const int32_t* pst = nullptr;
-  const int32_t* wpos = nullptr;
-  const uint16_t* lpos = nullptr;
-  const int32_t* rowptr = nullptr;
-  const int32_t* cblk = nullptr;
+  const int32_t* task = nullptr;
   const int32_t* units = nullptr;
   const int32_t* wg = nullptr;
+  const int32_t* rowptr = nullptr;
+  const int32_t* cblk = nullptr;
+  const int32_t* off = nullptr;
+  const uint16_t* lpos = nullptr;
   void* part = nullptr;
 };
 

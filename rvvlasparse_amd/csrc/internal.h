@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "csr_blocks.h"
 #include "rvvla_hip.h"
 #include "sparse_matrix.h"
 
@@ -120,21 +121,27 @@ template <class V>
 void build_xslice(XSliceHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col,
                   const V* val, int P, int xcd);
 // LDS-sliced CSR image (csr_xlds.cpp): x cut into slices of W elements
-// that fit one workgroup's LDS; entries slice-major with 16-bit columns.
+// that fill one workgroup's LDS; entries slice-major with 16-bit columns,
+// cut into tasks of kXlTask entries; one partial per piece (a (row, slice)
+// pair's entries inside one task).
 template <class V>
 struct XLdsHost {
-  int64_t W = 0, S = 0, npairs = 0;
+  int64_t W = 0, S = 0, npieces = 0, nent = 0;
   std::vector<uint16_t> col, lpos;
   std::vector<V> val;
-  std::vector<int32_t> pst, wpos, rowptr, cblk, units, wg;
+  std::vector<int32_t> task, units, wg, rowptr, cblk, off;
 };
 // (row, slice) pairs of a CSR matrix for slices of W columns.
 int64_t xlds_pairs(int64_t m, const int* rst, const int* col, int64_t W);
 // nwg: the workgroups of the slice pass; combine blocks of at most
-// comb_rows rows and comb_pairs partials (or one longer row)
+// comb_rows rows and comb_pieces partials (or one row with more); vpw:
+// device values per 16 bytes (2 FP64, 4 FP32, 8 for 16-bit values)
 template <class V>
 void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int* col, const V* val,
-                int64_t W, int64_t nwg, int64_t comb_rows, int64_t comb_pairs);
+                int64_t W, int64_t nwg, int64_t comb_rows, int64_t comb_pieces, int vpw);
+// Slot of stream position `at` in the value array (values of a task
+// permuted for contiguous 16-byte loads).
+int64_t xl_value_slot(int64_t at, int vpw);
 // Fraction of entries whose column is more than `window` away from the
 // row's diagonal position r * n / m.
 double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_t window);

@@ -24,6 +24,8 @@
 // with nontemporal loads, to leave L2 to x), and writes one partial per pair
 // (alpha = 1, beta = 0); k_xs_combine then adds each row's partials in
 // slice order and applies alpha and beta.
+#include <type_traits>
+
 #include "kcommon.h"
 
 namespace rvvla {
@@ -172,167 +174,388 @@ __global__ __launch_bounds__(kWG) void k_xs_combine(
   }
 }
 
-// LDS-sliced image (csr_xlds.cpp).  One workgroup per CU runs its units
-// (slice p, pairs g0 .. g1) in turn: x[p W, p W + W) into LDS (when the
-// slice changes), then tiles of kXlTile pairs claimed by the wavefronts in
-// turn.  A wavefront streams its tile's entries 64 at a time (kXlAhead
-// chunks of loads in flight), each lane one entry; bit 15 of the column
-// marks the first entry of a pair, so a ballot and a count of the marks
-// below each lane give every entry its pair.  Each lane keeps the sum of
-// its entries of its current pair in a register and adds it to the pair's
-// LDS accumulator when its pair changes (lanes in order, so the order of
-// the additions is fixed by the data); at the end of the tile the
-// accumulators go to the pairs' slots.
-constexpr int kXlAhead = 8;
-static_assert(xl_wg_per_cu(8) >= 1 && xl_wg_per_cu(4) >= 1, "LDS of k_xl_slices");
+// ---- LDS-sliced image (csr_xlds.cpp) --------------------------------------
+//
+// Slice pass: one workgroup per CU (shape: XlShape) runs its units (slice p,
+// tasks t0 .. t1-1) in turn: x[p W, p W + W) into LDS, then the tasks,
+// wavefront w taking tasks t0 + w, t0 + w + 16, ...  In a task lane l holds
+// entries l*8 .. l*8+7 (one 16-byte column load, 16-byte value loads).
+// Bit 15 marks the first entry of a piece and every task starts a piece.
+// A lane adds its entries in order; a piece that starts and ends inside
+// the lane is written at once.  The entries before the lane's first mark
+// (its head) belong to a piece opened in a lower lane, those from its last
+// mark on (its tail) to a piece that may continue in higher lanes.  A
+// segmented scan across the lanes (a segment starts at each lane holding a
+// mark) gives the running sum of the open piece at every lane: a lane with
+// marks completes the piece open below it (the scan of the lane below plus
+// its head), the last lane completes the task's last piece.  A piece's slot
+// is the count of marks before it, so a task fills consecutive slots.  The
+// 16-bit forms add with TwoSum compensation throughout.
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Launch shape of the slice pass: FP64 (x slice of 16384 doubles) runs 8
+// wavefronts a CU, each with 4 tasks in flight, and stages its partials in
+// LDS so that they leave in whole lines; the 32-bit-x forms run 16
+// wavefronts with one task ahead and store their partials directly.
 template <class VT>
-__global__ __launch_bounds__(kXlWG) void k_xl_slices(
-    const uint16_t* __restrict__ col, const VT* __restrict__ val, const int32_t* __restrict__ pst,
-    const int32_t* __restrict__ wpos, const int32_t* __restrict__ units,
-    const int32_t* __restrict__ wg, const typename Val<VT>::io* __restrict__ x, int64_t n,
-    int64_t W, typename Val<VT>::acc* __restrict__ part) {
-  using A = typename Val<VT>::acc;
-  using IO = typename Val<VT>::io;
-  constexpr int XW = kXlSliceCols;
-  constexpr int XP = XW / kXlWG;
-  static_assert(XW % kXlWG == 0, "x slice per thread");
-  __shared__ IO xs[XW];
-  // pair accumulators in double for every value type: LDS FP32 atomic adds
-  // kept the wavefronts waiting nine times longer than FP64 ones on MI355X
-  __shared__ double acc[(kXlWG / 64) * kXlTile];
-  __shared__ int next_tile;
-  const int tid = threadIdx.x, lane = tid & 63;
-  double* pacc = acc + (tid >> 6) * kXlTile;
-  const int u0 = wg[blockIdx.x], u1 = wg[blockIdx.x + 1];
-  int64_t loaded = -1;
-  uint32_t ca[kXlAhead];
-  A va[kXlAhead];
-  for (int u = u0; u < u1; ++u) {
-    const int64_t p = units[3 * u];
-    const int64_t g0 = units[3 * u + 1], g1 = units[3 * u + 2];
-    __syncthreads();  // every wavefront is done with the previous unit
-    if (p != loaded) {
-      const int64_t c0 = p * W;
-      const int wl = (int)(n - c0 < W ? n - c0 : W);
-      IO tmp[XP];
+struct XlShape {
+  static constexpr bool F64 = sizeof(VT) == 8;
+  static constexpr int WG = F64 ? 512 : 1024;
+  static constexpr int AHEAD = F64 ? 4 : 1;
+  static constexpr bool STAGE = F64;
+};
+
+// A task's data as loaded: its descriptor, 8 columns and 8 values per lane
+// (the values as raw 16-byte words).
+template <class VT>
+struct XlTaskData {
+  static constexpr int NV = sizeof(VT) / 2;  // 16-byte words of 8 values
+  int4 tk;
+  uint4 cw;
+  uint4 vw[NV];
+};
+
+template <class VT>
+__device__ __forceinline__ void xl_fetch(XlTaskData<VT>& d, const uint16_t* __restrict__ col,
+                                         const VT* __restrict__ val, const int4* __restrict__ task,
+                                         int t, int lane) {
+  d.tk = task[t];
+  d.cw = *reinterpret_cast<const uint4*>(col + d.tk.x + lane * 8);
+  // the task's values are stored word-interleaved: word i of lane l at
+  // 16-byte word i * 64 + l of the task
+  const uint4* vp = reinterpret_cast<const uint4*>(val + d.tk.x) + lane;
 #pragma unroll
-      for (int j = 0; j < XP; ++j) {
-        const int i = tid + j * kXlWG;
-        tmp[j] = i < wl ? x[c0 + i] : IO(0);
-      }
+  for (int i = 0; i < XlTaskData<VT>::NV; ++i) d.vw[i] = vp[i * 64];
+}
+
+template <class VT>
+__device__ __forceinline__ void xl_unpack(const XlTaskData<VT>& d, typename Val<VT>::acc (&v)[8]) {
+  if constexpr (sizeof(VT) == 8) {
 #pragma unroll
-      for (int j = 0; j < XP; ++j) xs[tid + j * kXlWG] = tmp[j];
-      loaded = p;
+    for (int i = 0; i < 4; ++i) {
+      v[2 * i] = __builtin_bit_cast(double, ((uint64_t)d.vw[i].y << 32) | d.vw[i].x);
+      v[2 * i + 1] = __builtin_bit_cast(double, ((uint64_t)d.vw[i].w << 32) | d.vw[i].z);
     }
-    if (tid == 0) next_tile = kXlWG / 64;  // tiles 0 .. waves-1 go one per wavefront
-    __syncthreads();
-    const int64_t ntiles = (g1 - g0 + kXlTile - 1) / kXlTile;
-    int64_t t = tid >> 6;
-    while (t < ntiles) {
-      int nt = 0;  // the next tile, claimed now
-      if (lane == 0) nt = atomicAdd(&next_tile, 1);
-      const int64_t gt = g0 + t * kXlTile;
-      const int np = (int)(gt + kXlTile < g1 ? kXlTile : g1 - gt);
-      const int e_beg = pst[gt], e_end = pst[gt + np];
-      for (int i = lane; i < kXlTile; i += 64) pacc[i] = 0.0;
-      int cur = -1;   // tile-local pair of the lane's running sum
-      A run = A(0);
-      int open = -1;  // tile-local pair open before the chunk
-      for (int cb = e_beg; cb < e_end; cb += 64 * kXlAhead) {
-        // loads without branches (an index past the tile reads its last
-        // entry and is masked), so no wait for all loads lands among them
+  } else if constexpr (sizeof(VT) == 4) {
 #pragma unroll
-        for (int k = 0; k < kXlAhead; ++k) {
-          const int e = cb + k * 64 + lane < e_end ? cb + k * 64 + lane : e_end - 1;
-          ca[k] = (uint32_t)col[e];
-          va[k] = Val<VT>::get(val, e);
-        }
-        A prod[kXlAhead];  // the group's x gathers issued together
+    for (int i = 0; i < 2; ++i) {
+      v[4 * i] = __uint_as_float(d.vw[i].x);
+      v[4 * i + 1] = __uint_as_float(d.vw[i].y);
+      v[4 * i + 2] = __uint_as_float(d.vw[i].z);
+      v[4 * i + 3] = __uint_as_float(d.vw[i].w);
+    }
+  } else {
+    const uint32_t w[4] = {d.vw[0].x, d.vw[0].y, d.vw[0].z, d.vw[0].w};
 #pragma unroll
-        for (int k = 0; k < kXlAhead; ++k) prod[k] = va[k] * (A)xs[ca[k] & 0x7FFFu];
-#pragma unroll
-        for (int k = 0; k < kXlAhead; ++k) {
-          const bool in = cb + k * 64 + lane < e_end;
-          const uint64_t H = __ballot(in && (ca[k] & 0x8000u));
-          const int below = (int)__builtin_amdgcn_mbcnt_hi(
-              (uint32_t)(H >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)H, 0u));
-          const int lp = open + below + (int)((H >> lane) & 1ull);
-          if (in) {
-            if (lp != cur) {
-              if (cur >= 0) atomicAdd(&pacc[cur], (double)run);
-              run = A(0);
-              cur = lp;
-            }
-            run += prod[k];
-          }
-          open += __popcll(H);
-        }
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (std::is_same<VT, bf16_t>::value) {
+        v[2 * i] = __uint_as_float(w[i] << 16);
+        v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+      } else {
+        v[2 * i] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w[i] & 0xFFFFu));
+        v[2 * i + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w[i] >> 16));
       }
-      if (cur >= 0) atomicAdd(&pacc[cur], (double)run);
-      for (int i = lane; i < np; i += 64) part[wpos[gt + i]] = (A)pacc[i];
-      t = __shfl(nt, 0, 64);
     }
   }
 }
 
-// Combine block b of the LDS-sliced image: rows r0 .. r1-1 (at most
-// xl_comb_rows, one per thread) with slots base .. end-1.  Each thread
-// loads its (at most 8) partials together and puts each at its row-major
-// place in LDS (lpos); then each row's partials are added in slice order
-// and alpha and beta applied.  A block of one row with more than
-// xl_comb_pairs partials (then already in slice order) is summed by the
-// whole workgroup.
-template <class A, class IO>
-__global__ __launch_bounds__(xl_comb_rows(sizeof(A))) void k_xl_combine(const int32_t* __restrict__ rowptr,
-                                                      const int32_t* __restrict__ cblk,
-                                                      const uint16_t* __restrict__ lpos,
-                                                      const A* __restrict__ part,
-                                                      IO* __restrict__ y, A alpha, A beta) {
-  constexpr int kXlCombWG = xl_comb_rows(sizeof(A));
-  constexpr int kXlCombPairs = xl_comb_pairs(sizeof(A));
-  constexpr int PT = kXlCombPairs / kXlCombWG;
-  __shared__ A prod[kXlCombPairs];
-  __shared__ A wsum[kXlCombWG / 64];
-  const int tid = threadIdx.x;
-  const int r0 = cblk[4 * blockIdx.x], r1 = cblk[4 * blockIdx.x + 1];
-  const int base = cblk[4 * blockIdx.x + 2], cnt = cblk[4 * blockIdx.x + 3];
-  if (cnt > kXlCombPairs) {
-    Acc<A, false> a;
-    for (int q = tid; q < cnt; q += kXlCombWG) a.add(part[base + q]);
-    a = group_reduce(a, 64);
-    if ((tid & 63) == 0) wsum[tid >> 6] = a.s;
+// Slots of a task staged in LDS (FP64): slot s of the task at s ^ ((s >> 5)
+// & 7), so that neither the lanes' scattered writes nor the contiguous read
+// back meet more than two-way bank conflicts.
+__device__ __forceinline__ int xl_swz(int s) { return s ^ ((s >> 5) & 7); }
+
+template <class VT, bool COMP, bool STAGE>
+__device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename Val<VT>::io* xs,
+                                        typename Val<VT>::acc* __restrict__ part,
+                                        typename Val<VT>::acc* stg, int lane) {
+  using A = typename Val<VT>::acc;
+  using IO = typename Val<VT>::io;
+  using Ac = Acc<A, COMP>;
+  constexpr int XW = xl_slice_cols(sizeof(IO));
+  const int ec = d.tk.y, pc0 = d.tk.z;
+  // a partial of the task: into its slot, or into the stage
+  auto emit = [&](int s, A v) {
+    if constexpr (STAGE) stg[xl_swz(s - pc0)] = v;
+    else part[s] = v;
+  };
+  A v[8];
+  xl_unpack<VT>(d, v);
+  const uint32_t cwd[4] = {d.cw.x, d.cw.y, d.cw.z, d.cw.w};
+  uint32_t c[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    c[2 * i] = cwd[i] & 0xFFFFu;
+    c[2 * i + 1] = cwd[i] >> 16;
+  }
+  const int nv = ec - lane * 8;  // valid entries of this lane (may be <= 0)
+  A prod[8];
+  bool fl[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const bool ok = k < nv;
+    fl[k] = ok && (c[k] & 0x8000u);
+    const A xv = (A)xs[c[k] & (XW - 1)];
+    prod[k] = ok ? v[k] * xv : A(0);
+  }
+  int fb = 0, tot = 0;  // marks in lower lanes, marks in the task
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const uint64_t B = __ballot(fl[k]);
+    fb += lanes_below(B);
+    tot += __popcll(B);
+  }
+  Ac run, head;
+  bool seen = false;
+  int slot = pc0 + fb - 1;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (fl[k]) {
+      if (seen) emit(slot, run.value());
+      else head = run;
+      seen = true;
+      run = Ac();
+      ++slot;
+    }
+    run.add(prod[k]);
+  }
+  // segmented inclusive scan of the open sums; lane 0 always holds a mark
+  const uint64_t H = __ballot(seen);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int st = 63 - __clzll((long long)(H & upto));
+  Ac S = run;
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    Ac o;
+    o.s = __shfl_up(S.s, dd, 64);
+    if constexpr (COMP) o.c = __shfl_up(S.c, dd, 64);
+    if (lane - dd >= st) S.merge(o);
+  }
+  Ac prev;
+  prev.s = __shfl_up(S.s, 1, 64);
+  if constexpr (COMP) prev.c = __shfl_up(S.c, 1, 64);
+  if (seen && lane > 0) {
+    prev.merge(head);
+    emit(pc0 + fb - 1, prev.value());
+  }
+  if (lane == 63) emit(pc0 + tot - 1, S.value());
+  if constexpr (STAGE) {
+    // the stage back in slot order: whole-line stores
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < tot; i += 64) part[pc0 + i] = stg[xl_swz(i)];
+  }
+}
+
+template <class VT, bool COMP>
+__global__ __launch_bounds__(XlShape<VT>::WG) void k_xl_slices(
+    const uint16_t* __restrict__ col, const VT* __restrict__ val, const int4* __restrict__ task,
+    const int32_t* __restrict__ units, const int32_t* __restrict__ wg,
+    const typename Val<VT>::io* __restrict__ x, int64_t n, typename Val<VT>::acc* __restrict__ part) {
+  using IO = typename Val<VT>::io;
+  using A = typename Val<VT>::acc;
+  constexpr int WGS = XlShape<VT>::WG;
+  constexpr int D = XlShape<VT>::AHEAD;
+  constexpr bool STAGE = XlShape<VT>::STAGE;
+  constexpr int XW = xl_slice_cols(sizeof(IO));
+  constexpr int XP = XW / WGS;
+  constexpr int NW = WGS / 64;
+  static_assert(XW % WGS == 0, "x slice per thread");
+  static_assert(!STAGE || XW * sizeof(IO) + NW * kXlTask * sizeof(A) <= 163840, "LDS");
+  __shared__ IO xs[XW];
+  __shared__ A stage[STAGE ? NW * kXlTask : 1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  A* stg = stage + (STAGE ? wave * kXlTask : 0);
+  const int u0 = wg[blockIdx.x], u1 = wg[blockIdx.x + 1];
+  for (int u = u0; u < u1; ++u) {
+    const int64_t p = units[3 * u];
+    const int t0 = units[3 * u + 1], t1 = units[3 * u + 2];
+    // the wavefront's first D tasks are fetched before the slice of x (a
+    // task index past the unit fetches its last task again, unused)
+    XlTaskData<VT> b[D];
+    const int tw = t0 + wave;
+#pragma unroll
+    for (int i = 0; i < D; ++i) xl_fetch<VT>(b[i], col, val, task, min(tw + i * NW, t1 - 1), lane);
+    __syncthreads();  // every wavefront is done with the previous slice
+    {
+      const int64_t c0 = p * XW;
+      const int wl = (int)(n - c0 < XW ? n - c0 : XW);
+      IO tmp[XP];
+#pragma unroll
+      for (int j = 0; j < XP; ++j) {
+        const int i = tid + j * WGS;
+        tmp[j] = i < wl ? x[c0 + i] : IO(0);
+      }
+#pragma unroll
+      for (int j = 0; j < XP; ++j) xs[tid + j * WGS] = tmp[j];
+    }
+    __syncthreads();
+    for (int t = tw; t < t1; t += D * NW) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        const int ti = t + i * NW;
+        if (ti < t1) xl_task<VT, COMP, STAGE>(b[i], xs, part, stg, lane);
+        // the ring slot refilled D tasks ahead
+        xl_fetch<VT>(b[i], col, val, task, min(ti + D * NW, t1 - 1), lane);
+      }
+    }
+  }
+}
+
+// Combine of the LDS-sliced image: block b (rows r0 .. r1-1) takes from
+// each slice p its run of slots off[b S + p] .. off[(b+1) S + p] - 1.  The
+// run lengths are scanned in LDS, numbering the block's partials 0 .. cnt-1
+// in slice order; thread t takes partials t, t + kXcWG, ... (each one's run
+// found by a two-level ballot search, all loads issued together) and puts
+// each at its row-major place lpos in LDS.  Then each row's
+// partials are added in slice order (a thread per row, a wavefront for a
+// row of more than 32 pieces) and alpha and beta applied.  A block of one
+// row with more pieces than LDS holds adds its runs in place.  Consecutive
+// blocks run on one XCD, so the lines their runs share stay in that XCD's
+// L2.
+template <class A, bool COMP>
+__device__ __forceinline__ void xc_finish(Acc<A, COMP> a, A* wsum, int lane, int wave) {
+  constexpr int NW = kXcWG / 64;
+  a = group_reduce(a, 64);
+  if (lane == 0) {
+    wsum[wave] = a.s;
+    wsum[NW + wave] = a.c;
+  }
+}
+
+template <class A, class IO, bool COMP>
+__global__ __launch_bounds__(kXcWG) void k_xl_combine(
+    const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cblk,
+    const int32_t* __restrict__ off, const uint16_t* __restrict__ lpos,
+    const A* __restrict__ part, IO* __restrict__ y, int64_t S, int nb, A alpha, A beta) {
+  using Ac = Acc<A, COMP>;
+  constexpr int CP = xl_comb_pieces(sizeof(A));
+  constexpr int NW = kXcWG / 64;
+  constexpr int PT = CP / kXcWG;
+  __shared__ A prod[CP];
+  __shared__ int pre[kXcMaxSlices + 1];
+  __shared__ int beg[kXcMaxSlices];
+  __shared__ A wsum[2 * NW];
+  __shared__ int wtot[NW];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int per = (nb + 7) / 8;
+  const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (b >= nb) return;
+  const int r0 = cblk[2 * b], r1 = cblk[2 * b + 1];
+  const int base = rowptr[r0], cnt = rowptr[r1] - base;
+  const int32_t* o0 = off + (int64_t)b * S;
+  const int32_t* o1 = o0 + S;
+  if (cnt > CP) {
+    Ac a;
+    for (int64_t p = wave; p < S; p += NW) {
+      const int a1 = o1[p];
+      for (int i = o0[p] + lane; i < a1; i += 64) a.add(part[i]);
+    }
+    xc_finish<A, COMP>(a, wsum, lane, wave);
     __syncthreads();
     if (tid == 0) {
-      A t = wsum[0];
-#pragma unroll
-      for (int w = 1; w < kXlCombWG / 64; ++w) t += wsum[w];
-      y[r0] = alpha * t + scaled_y(beta, y, r0);
+      Ac t;
+      t.s = wsum[0];
+      t.c = wsum[NW];
+      for (int w = 1; w < NW; ++w) {
+        Ac o;
+        o.s = wsum[w];
+        o.c = wsum[NW + w];
+        t.merge(o);
+      }
+      y[r0] = alpha * t.value() + scaled_y(beta, y, r0);
     }
     return;
   }
-  const int r = r0 + tid;  // this thread's row: its bounds fetched with the partials
-  int k0 = 0, k1 = 0;
-  if (r < r1) {
-    k0 = rowptr[r] - base;
-    k1 = rowptr[r + 1] - base;
-  }
-  A v[PT];
-  int at[PT];
+  if (S > kXcMaxSlices) {  // too many slices to scan in LDS: a wavefront per run
+    for (int64_t p = wave; p < S; p += NW) {
+      const int a0 = o0[p], a1 = o1[p];
+      for (int i = a0 + lane; i < a1; i += 64) prod[lpos[i]] = part[i];
+    }
+  } else {
+    // run starts and the scan of the run lengths (thread t: slice t)
+    int l0 = 0;
+    if (tid < S) {
+      const int a0 = o0[tid];
+      beg[tid] = a0;
+      l0 = o1[tid] - a0;
+    }
+    int inc = l0;  // inclusive scan across the workgroup
 #pragma unroll
-  for (int j = 0; j < PT; ++j) {
-    const int q = tid + j * kXlCombWG;
-    const int qq = q < cnt ? q : (cnt > 0 ? cnt - 1 : 0);  // in range, masked below
-    v[j] = cnt > 0 ? part[base + qq] : A(0);
-    at[j] = cnt > 0 ? (int)lpos[base + qq] : 0;
-  }
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    int wbase = 0;
+    for (int w = 0; w < wave; ++w) wbase += wtot[w];
+    if (tid < S) pre[tid] = wbase + inc - l0;
+    if (tid == 0) pre[S] = cnt;
+    __syncthreads();
+    // partial q = tid + j kXcWG: the run of the wavefront's first q by a
+    // two-level ballot search over the scan (16 runs, then 1), each lane's
+    // own run by walking on from there; all loads issued together
+    A v[PT];
+    int at[PT];
 #pragma unroll
-  for (int j = 0; j < PT; ++j)
-    if (tid + j * kXlCombWG < cnt) prod[at[j]] = v[j];
+    for (int j = 0; j < PT; ++j) {
+      const int qc = wave * 64 + j * kXcWG;  // uniform
+      v[j] = A(0);
+      at[j] = -1;
+      if (qc < cnt) {
+        const int i1 = lane * 16;
+        const uint64_t b1 = __ballot(i1 <= (int)S && pre[min(i1, (int)S)] <= qc);
+        const int L1 = 63 - __clzll((long long)b1);  // lane 0 always qualifies (pre[0] = 0)
+        const int i2 = L1 * 16 + (lane & 15);
+        const uint64_t b2 = __ballot(lane < 16 && i2 < (int)S && pre[min(i2, (int)S)] <= qc);
+        int lo = 63 - __clzll((long long)b2);
+        lo = L1 * 16 + lo;
+        const int q = qc + lane;
+        if (q < cnt) {
+          int nxt = pre[lo + 1];
+          while (q >= nxt) nxt = pre[++lo + 1];
+          const int slot = beg[lo] + (q - pre[lo]);
+          v[j] = part[slot];
+          at[j] = lpos[slot];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PT; ++j)
+      if (at[j] >= 0) prod[at[j]] = v[j];
+  }
   __syncthreads();
-  if (r < r1) {
-    A s = A(0);
-    for (int k = k0; k < k1; ++k) s += prod[k];
-    y[r] = alpha * s + scaled_y(beta, y, r);
+  for (int rb = r0; rb < r1; rb += kXcWG) {
+    const int r = rb + tid;
+    int k0 = 0, k1 = 0;
+    if (r < r1) {
+      k0 = rowptr[r] - base;
+      k1 = rowptr[r + 1] - base;
+    }
+    const bool is_long = r < r1 && k1 - k0 > 32;
+    if (r < r1 && !is_long) {
+      Ac s;
+      for (int k = k0; k < k1; ++k) s.add(prod[k]);
+      y[r] = alpha * s.value() + scaled_y(beta, y, r);
+    }
+    uint64_t lm = __ballot(is_long);
+    while (lm) {
+      const int l = __ffsll((unsigned long long)lm) - 1;
+      lm &= lm - 1;
+      const int rlo = __shfl(k0, l, 64), rhi = __shfl(k1, l, 64);
+      Ac a;
+      for (int k = rlo + lane; k < rhi; k += 64) a.add(prod[k]);
+      a = group_reduce(a, 64);
+      if (lane == 0) {
+        const int rr = rb + (tid - lane) + l;
+        y[rr] = alpha * a.value() + scaled_y(beta, y, rr);
+      }
+    }
   }
 }
 
@@ -371,15 +594,26 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
     const DevXLds& xl = d.xl;
     if (m == 0) return RVVLA_OK;
     if (xl.nunits > 0) {
-      hipLaunchKernelGGL((k_xl_slices<VT>), dim3((unsigned)xl.nwg), dim3(kXlWG), 0, s, xl.col,
-                         (const VT*)xl.val, xl.pst, xl.wpos, xl.units, xl.wg, (const IO*)x, xl.n,
-                         xl.W, (AC*)xl.part);
+      if (comp)
+        hipLaunchKernelGGL((k_xl_slices<VT, true>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s, xl.col,
+                           (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg, (const IO*)x,
+                           xl.n, (AC*)xl.part);
+      else
+        hipLaunchKernelGGL((k_xl_slices<VT, false>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s,
+                           xl.col, (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg,
+                           (const IO*)x, xl.n, (AC*)xl.part);
       if (int rc = hip_check(hipGetLastError(), "k_xl_slices launch")) return rc;
     }
-    hipLaunchKernelGGL((k_xl_combine<AC, IO>), dim3((unsigned)xl.nblocks),
-                       dim3(xl_comb_rows(sizeof(AC))), 0, s,
-                       xl.rowptr, xl.cblk, xl.lpos, (const AC*)xl.part, (IO*)y, (AC)alpha,
-                       (AC)beta);
+    const int nb = (int)xl.nblocks;
+    const unsigned grid = (unsigned)(8 * ((nb + 7) / 8));
+    if (comp)
+      hipLaunchKernelGGL((k_xl_combine<AC, IO, true>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
+                         xl.cblk, xl.off, xl.lpos, (const AC*)xl.part, (IO*)y, xl.S, nb, (AC)alpha,
+                         (AC)beta);
+    else
+      hipLaunchKernelGGL((k_xl_combine<AC, IO, false>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
+                         xl.cblk, xl.off, xl.lpos, (const AC*)xl.part, (IO*)y, xl.S, nb, (AC)alpha,
+                         (AC)beta);
     return hip_check(hipGetLastError(), "k_xl_combine launch");
   }
   if (d.xs.P > 0) {
