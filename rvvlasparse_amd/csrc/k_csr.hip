@@ -196,15 +196,14 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 }
 
 // Launch shape of the slice pass: FP64 (x slice of 16384 doubles) runs 8
-// wavefronts a CU, each with 4 tasks in flight, and stages its partials in
-// LDS so that they leave in whole lines; the 32-bit-x forms run 16
-// wavefronts with one task ahead and store their partials directly.
+// wavefronts a CU, each with 4 tasks in flight; the 32-bit-x forms run 16
+// wavefronts with one task ahead.  Both stage a task's partials in LDS, so
+// that they leave in whole lines.
 template <class VT>
 struct XlShape {
   static constexpr bool F64 = sizeof(VT) == 8;
   static constexpr int WG = F64 ? 512 : 1024;
   static constexpr int AHEAD = F64 ? 4 : 1;
-  static constexpr bool STAGE = F64;
 };
 
 // A task's data as loaded: its descriptor, 8 columns and 8 values per lane
@@ -261,12 +260,73 @@ __device__ __forceinline__ void xl_unpack(const XlTaskData<VT>& d, typename Val<
   }
 }
 
-// Slots of a task staged in LDS (FP64): slot s of the task at s ^ ((s >> 5)
-// & 7), so that neither the lanes' scattered writes nor the contiguous read
-// back meet more than two-way bank conflicts.
+// DPP move of a 32- or 64-bit value (CTRL: row_shr:n = 0x110 | n, wave_shr:1
+// = 0x138); lanes without a source lane get 0.
+template <int CTRL>
+__device__ __forceinline__ float xl_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double xl_dpp(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float xl_rl(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double xl_rl(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <class Ac>
+__device__ __forceinline__ Ac xl_readlane(const Ac& a, int l) {
+  Ac r;
+  r.s = xl_rl(a.s, l);
+  r.c = xl_rl(a.c, l);
+  return r;
+}
+// One in-row Hillis-Steele step of the segmented scan: lane l adds lane
+// l - D of its row when that lane is in its segment.
+template <int CTRL, int D, class Ac>
+__device__ __forceinline__ void xl_scan_step(Ac& S, int lr, int lane, int st) {
+  Ac o;
+  o.s = xl_dpp<CTRL>(S.s);
+  o.c = xl_dpp<CTRL>(S.c);
+  if (lr >= D && lane - D >= st) S.merge(o);
+}
+
+// Slots of a task staged in LDS: slot s of the task at s ^ ((s >> 5) & 7),
+// so that neither the lanes' scattered writes nor the contiguous read back
+// meet more than two-way bank conflicts.
 __device__ __forceinline__ int xl_swz(int s) { return s ^ ((s >> 5) & 7); }
 
-template <class VT, bool COMP, bool STAGE>
+// Exclusive prefix count of v over the lanes (DPP row shifts, then the row
+// totals), and the wavefront total.
+__device__ __forceinline__ int xl_lane_prefix(int v, int lane, int& total) {
+  int s = v;
+  s += __builtin_amdgcn_update_dpp(0, s, 0x111, 0xF, 0xF, false);
+  s += __builtin_amdgcn_update_dpp(0, s, 0x112, 0xF, 0xF, false);
+  s += __builtin_amdgcn_update_dpp(0, s, 0x114, 0xF, 0xF, false);
+  s += __builtin_amdgcn_update_dpp(0, s, 0x118, 0xF, 0xF, false);
+  const int r0 = __builtin_amdgcn_readlane(s, 15), r1 = __builtin_amdgcn_readlane(s, 31);
+  const int r2 = __builtin_amdgcn_readlane(s, 47), r3 = __builtin_amdgcn_readlane(s, 63);
+  const int row = lane >> 4;
+  s += (row >= 1 ? r0 : 0) + (row >= 2 ? r1 : 0) + (row >= 3 ? r2 : 0);
+  total = r0 + r1 + r2 + r3;
+  return s - v;
+}
+
+// One task.  The lane's walk has no branches: at every entry the running
+// sum of the lane's open piece is written to that piece's stage slot, so
+// the last write a lane makes to a slot is its complete share of the piece;
+// a piece that continues into higher lanes is written again, complete, by
+// the lane that closes it (after every walk write, so it wins).  The stage
+// then goes out in slot order.
+template <class VT, bool COMP>
 __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename Val<VT>::io* xs,
                                         typename Val<VT>::acc* __restrict__ part,
                                         typename Val<VT>::acc* stg, int lane) {
@@ -275,11 +335,6 @@ __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename 
   using Ac = Acc<A, COMP>;
   constexpr int XW = xl_slice_cols(sizeof(IO));
   const int ec = d.tk.y, pc0 = d.tk.z;
-  // a partial of the task: into its slot, or into the stage
-  auto emit = [&](int s, A v) {
-    if constexpr (STAGE) stg[xl_swz(s - pc0)] = v;
-    else part[s] = v;
-  };
   A v[8];
   xl_unpack<VT>(d, v);
   const uint32_t cwd[4] = {d.cw.x, d.cw.y, d.cw.z, d.cw.w};
@@ -290,63 +345,76 @@ __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename 
     c[2 * i + 1] = cwd[i] >> 16;
   }
   const int nv = ec - lane * 8;  // valid entries of this lane (may be <= 0)
+  // the eight x gathers from LDS issued together: every index is inside
+  // the slice, and the empty asm takes all eight values before any is
+  // masked, so that none is moved into a branch of its own
+  IO xv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) xv[k] = xs[c[k] & (XW - 1)];
+  __asm__ volatile("" ::"v"(xv[0]), "v"(xv[1]), "v"(xv[2]), "v"(xv[3]), "v"(xv[4]), "v"(xv[5]),
+                   "v"(xv[6]), "v"(xv[7]));
   A prod[8];
   bool fl[8];
+  int nfl = 0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const bool ok = k < nv;
     fl[k] = ok && (c[k] & 0x8000u);
-    const A xv = (A)xs[c[k] & (XW - 1)];
-    prod[k] = ok ? v[k] * xv : A(0);
+    nfl += fl[k] ? 1 : 0;
+    prod[k] = ok ? v[k] * (A)xv[k] : A(0);
   }
-  int fb = 0, tot = 0;  // marks in lower lanes, marks in the task
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const uint64_t B = __ballot(fl[k]);
-    fb += lanes_below(B);
-    tot += __popcll(B);
-  }
+  int tot;
+  const int fb = xl_lane_prefix(nfl, lane, tot);  // marks in lower lanes, in the task
   Ac run, head;
   bool seen = false;
-  int slot = pc0 + fb - 1;
+  int slot = fb - 1;  // task-relative slot of the lane's open piece
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    if (fl[k]) {
-      if (seen) emit(slot, run.value());
-      else head = run;
-      seen = true;
-      run = Ac();
-      ++slot;
-    }
+    const bool f = fl[k];
+    const bool first = f && !seen;
+    head.s = first ? run.s : head.s;
+    head.c = first ? run.c : head.c;
+    run.s = f ? A(0) : run.s;
+    run.c = f ? A(0) : run.c;
+    seen = seen || f;
+    slot += f ? 1 : 0;
     run.add(prod[k]);
+    stg[xl_swz(slot)] = run.value();
   }
-  // segmented inclusive scan of the open sums; lane 0 always holds a mark
+  // segmented inclusive scan of the open sums (lane 0 always holds a
+  // mark): Hillis-Steele steps inside each row of 16 lanes (DPP row
+  // shifts), then the open sum at the end of each row carried into the
+  // next rows' lanes whose segment started before them
   const uint64_t H = __ballot(seen);
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   const int st = 63 - __clzll((long long)(H & upto));
+  const int lr = lane & 15;
   Ac S = run;
-#pragma unroll
-  for (int dd = 1; dd < 64; dd <<= 1) {
-    Ac o;
-    o.s = __shfl_up(S.s, dd, 64);
-    if constexpr (COMP) o.c = __shfl_up(S.c, dd, 64);
-    if (lane - dd >= st) S.merge(o);
+  xl_scan_step<0x111, 1>(S, lr, lane, st);
+  xl_scan_step<0x112, 2>(S, lr, lane, st);
+  xl_scan_step<0x114, 4>(S, lr, lane, st);
+  xl_scan_step<0x118, 8>(S, lr, lane, st);
+  {
+    // row r's lanes take the open sum at the end of row r - 1, which
+    // already holds the carries of the rows before it
+    const Ac T = xl_readlane(S, 15);
+    if (lane >= 16 && lane < 32 && st < 16) S.merge(T);
+    const Ac E = xl_readlane(S, 31);
+    if (lane >= 32 && lane < 48 && st < 32) S.merge(E);
+    const Ac F = xl_readlane(S, 47);
+    if (lane >= 48 && st < 48) S.merge(F);
   }
   Ac prev;
-  prev.s = __shfl_up(S.s, 1, 64);
-  if constexpr (COMP) prev.c = __shfl_up(S.c, 1, 64);
-  if (seen && lane > 0) {
-    prev.merge(head);
-    emit(pc0 + fb - 1, prev.value());
-  }
-  if (lane == 63) emit(pc0 + tot - 1, S.value());
-  if constexpr (STAGE) {
-    // the stage back in slot order: whole-line stores
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int i = lane; i < tot; i += 64) part[pc0 + i] = stg[xl_swz(i)];
-  }
+  prev.s = xl_dpp<0x138>(S.s);  // wave_shr:1: the lane below
+  prev.c = xl_dpp<0x138>(S.c);
+  prev.merge(head);
+  if (seen && lane > 0) stg[xl_swz(fb - 1)] = prev.value();
+  if (lane == 63) stg[xl_swz(tot - 1)] = S.value();
+  // the stage back in slot order: whole-line stores
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int i = lane; i < tot; i += 64) part[pc0 + i] = stg[xl_swz(i)];
 }
 
 template <class VT, bool COMP>
@@ -358,17 +426,16 @@ __global__ __launch_bounds__(XlShape<VT>::WG) void k_xl_slices(
   using A = typename Val<VT>::acc;
   constexpr int WGS = XlShape<VT>::WG;
   constexpr int D = XlShape<VT>::AHEAD;
-  constexpr bool STAGE = XlShape<VT>::STAGE;
   constexpr int XW = xl_slice_cols(sizeof(IO));
   constexpr int XP = XW / WGS;
   constexpr int NW = WGS / 64;
   static_assert(XW % WGS == 0, "x slice per thread");
-  static_assert(!STAGE || XW * sizeof(IO) + NW * kXlTask * sizeof(A) <= 163840, "LDS");
+  static_assert(XW * sizeof(IO) + NW * kXlTask * sizeof(A) <= 163840, "LDS");
   __shared__ IO xs[XW];
-  __shared__ A stage[STAGE ? NW * kXlTask : 1];
+  __shared__ A stage[NW * kXlTask];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  A* stg = stage + (STAGE ? wave * kXlTask : 0);
+  A* stg = stage + wave * kXlTask;
   const int u0 = wg[blockIdx.x], u1 = wg[blockIdx.x + 1];
   for (int u = u0; u < u1; ++u) {
     const int64_t p = units[3 * u];
@@ -382,12 +449,17 @@ __global__ __launch_bounds__(XlShape<VT>::WG) void k_xl_slices(
     __syncthreads();  // every wavefront is done with the previous slice
     {
       const int64_t c0 = p * XW;
-      const int wl = (int)(n - c0 < XW ? n - c0 : XW);
       IO tmp[XP];
+      if (n - c0 >= XW) {  // a whole slice: unguarded loads
 #pragma unroll
-      for (int j = 0; j < XP; ++j) {
-        const int i = tid + j * WGS;
-        tmp[j] = i < wl ? x[c0 + i] : IO(0);
+        for (int j = 0; j < XP; ++j) tmp[j] = x[c0 + tid + j * WGS];
+      } else {
+        const int wl = (int)(n - c0);
+#pragma unroll
+        for (int j = 0; j < XP; ++j) {
+          const int i = tid + j * WGS;
+          tmp[j] = i < wl ? x[c0 + i] : IO(0);
+        }
       }
 #pragma unroll
       for (int j = 0; j < XP; ++j) xs[tid + j * WGS] = tmp[j];
@@ -397,7 +469,7 @@ __global__ __launch_bounds__(XlShape<VT>::WG) void k_xl_slices(
 #pragma unroll
       for (int i = 0; i < D; ++i) {
         const int ti = t + i * NW;
-        if (ti < t1) xl_task<VT, COMP, STAGE>(b[i], xs, part, stg, lane);
+        if (ti < t1) xl_task<VT, COMP>(b[i], xs, part, stg, lane);
         // the ring slot refilled D tasks ahead
         xl_fetch<VT>(b[i], col, val, task, min(ti + D * NW, t1 - 1), lane);
       }
