@@ -266,6 +266,17 @@ int rvvla_mat_info(rvvla_mat_t A, int64_t* m, int64_t* n, int64_t* nnz,
  * CSR kernels (for comparisons). */
 int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta,
                void* y, void* stream);
+/* rvvla_spmv without any allocation: no reference counterpart; for callers
+ * that cannot allocate device memory during a product (code under stream
+ * capture, tight serving loops, harnesses that forbid it).  The same kernels
+ * and results as rvvla_spmv, but the per-product partial-sum / hand-over
+ * buffers live in one 16 MiB device array that the library holds on each
+ * device until the process ends (its address is looked up on the first
+ * call on a device).  A product needing more than 16 MiB of such buffers
+ * returns RVVLA_ERR_LENGTH.  All calls on one device share the array:
+ * calls that overlap on one device race. */
+int rvvla_spmv_noalloc(rvvla_mat_t A, double alpha, const void* x, double beta,
+                       void* y, void* stream);
 /* The product with the operation of sparse_mv's type_op
  * (reference include/types.h:24-27; the reference accepts the flag but
  * always computes A x): op 0 = SPARSE_OPERATION_NON_TRANSPOSE (rvvla_spmv),

@@ -89,6 +89,7 @@ _sigs = {
     "rvvla_mat_info": (_i32, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
                               C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i64)]),
     "rvvla_spmv": (_i32, [_vp, _d, _vp, _d, _vp, _vp]),
+    "rvvla_spmv_noalloc": (_i32, [_vp, _d, _vp, _d, _vp, _vp]),
     "rvvla_spmv_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp, _vp]),
     "rvvla_spmv_host_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp]),
     "rvvla_spmm": (_i32, [_vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp]),
@@ -249,6 +250,12 @@ class DeviceMatrix:
         (a torch.cuda.Stream, a raw hipStream_t int, or None = torch's
         current stream when torch tensors are passed)."""
         check(lib.rvvla_spmv(self._h, alpha, _ptr(x), beta, _ptr(y), _stream(stream, y)))
+
+    def spmv_noalloc(self, x, y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
+        """The same product with no device allocation (rvvla_spmv_noalloc):
+        the per-product buffers live in the library's 16 MiB device array;
+        overlapping calls on one device race."""
+        check(lib.rvvla_spmv_noalloc(self._h, alpha, _ptr(x), beta, _ptr(y), _stream(stream, y)))
 
     def spmv_t(self, x, y, alpha: float = 1.0, beta: float = 0.0, stream=None) -> None:
         """y = alpha*A^T*x + beta*y (SPARSE_OPERATION_TRANSPOSE; CSR matrices)."""

@@ -1893,23 +1893,55 @@ static int need_device(rvvla_mat_t A) {
   return RVVLA_OK;
 }
 
-int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y, void* stream) {
-  if (!A || (A->n > 0 && !x) || (A->m > 0 && !y)) return RVVLA_ERR_INVALID_ARG;
-  if (int rc = need_device(A)) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  switch (A->format) {
+extern "C++" {
+namespace rvvla {
+
+int64_t spmv_scratch_bytes(const rvvla_mat_s& A) {
+  const int64_t acc = A.dtype == RVVLA_F64 ? 8 : 4;
+  auto sell_part = [&](const DevSELL& d) { return d.nsplit > 0 ? d.npieces * 64 * acc : 0; };
+  switch (A.format) {
     case RVVLA_FMT_CSR:
-      return A->csr.sell ? rvvla::launch_sell(*A, alpha, x, beta, y, s)
-                         : rvvla::launch_csr(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_HCSR: return rvvla::launch_hcsr(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_SELL: return rvvla::launch_sell(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_CSR5: return rvvla::launch_csr5(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_CVR: return rvvla::launch_cvr(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_VHCC: return rvvla::launch_vhcc(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_LAV: return rvvla::launch_lav(*A, alpha, x, beta, y, s);
-    case RVVLA_FMT_VNEC: return rvvla::launch_vnec(*A, alpha, x, beta, y, s);
+      if (A.csr.sell) return sell_part(A.sell);
+      if (A.csr.xl.S > 0) return std::max<int64_t>(A.csr.xl.npieces, 1) * acc;
+      if (A.csr.xs.P > 0) return A.csr.xs.pairs * acc;
+      return 0;
+    case RVVLA_FMT_SELL: return sell_part(A.sell);
+    case RVVLA_FMT_LAV: return A.lav.nsplit > 0 ? A.lav.npieces * 64 * acc : 0;
+    case RVVLA_FMT_CSR5: return A.csr5.ntiles * (2 * acc + 4) + 16;
+    case RVVLA_FMT_VHCC: return A.vhcc.nparts * 2 * acc + 16;
+    case RVVLA_FMT_VNEC: return A.vnec.nparts * 2 * acc + 16;
+  }
+  return 0;
+}
+
+int spmv_dispatch(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                  hipStream_t s, void* scratch) {
+  switch (A.format) {
+    case RVVLA_FMT_CSR:
+      return A.csr.sell ? launch_sell(A, alpha, x, beta, y, s, scratch)
+                        : launch_csr(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_FMT_HCSR: return launch_hcsr(A, alpha, x, beta, y, s);
+    case RVVLA_FMT_SELL: return launch_sell(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_FMT_CSR5: return launch_csr5(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_FMT_CVR: return launch_cvr(A, alpha, x, beta, y, s);
+    case RVVLA_FMT_VHCC: return launch_vhcc(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_FMT_LAV: return launch_lav(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_FMT_VNEC: return launch_vnec(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_UNKNOWN_FORMAT;
+}
+
+int spmv_check_args(rvvla_mat_t A, const void* x, void* y) {
+  if (!A || (A->n > 0 && !x) || (A->m > 0 && !y)) return RVVLA_ERR_INVALID_ARG;
+  return need_device(A);
+}
+
+}  // namespace rvvla
+}  // extern "C++"
+
+int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y, void* stream) {
+  if (int rc = rvvla::spmv_check_args(A, x, y)) return rc;
+  return rvvla::spmv_dispatch(*A, alpha, x, beta, y, (hipStream_t)stream, nullptr);
 }
 
 // CSR image of the transpose: counting sort of the entries by column (a

@@ -328,7 +328,7 @@ namespace rvvla {
 int hip_check(hipError_t e, const char* what);
 
 int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s);
+               hipStream_t s, void* scratch = nullptr);
 // Device-side CSR checks (RVVLA_ERR_INVALID_ARG on a bad CSR) and the
 // CSR -> SELL-64 build into M.sell (k_convert.hip); sigma <= 0 picks the
 // default rule on the device.
@@ -388,25 +388,36 @@ int build_hcsr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const i
 // Pieces of the SELL slices wider than kSellPiece steps, from the widths.
 void attach_sell_pieces(rvvla_mat_s& M, const std::vector<int>& cl);
 
-// The adaptive CSR product on any CSR image of m rows.
+// The adaptive CSR product on any CSR image of m rows.  In every product
+// launcher, `scratch` (optional) is device memory used in place of the
+// image's own partial-sum or hand-over buffer (spmv_scratch_bytes of the
+// matrix at least); every word of it that a product reads, the same product
+// has written first.
 int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,
-                     double beta, void* y, hipStream_t s);
+                     double beta, void* y, hipStream_t s, void* scratch = nullptr);
+// Bytes of partial-sum / hand-over buffer a product of A uses (0: none).
+int64_t spmv_scratch_bytes(const rvvla_mat_s& A);
+// y = alpha A x + beta y in A's format, with `scratch` (or A's own buffers
+// when null); spmv_check_args validates the arguments of an SpMV call.
+int spmv_dispatch(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
+                  hipStream_t s, void* scratch);
+int spmv_check_args(rvvla_mat_s* A, const void* x, void* y);
 int launch_csr_spmm(const rvvla_mat_s& A, int k, double alpha, const void* X, int64_t ldx,
                     double beta, void* Y, int64_t ldy, hipStream_t s);
 int launch_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s);
 int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s);
+                hipStream_t s, void* scratch = nullptr);
 int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s);
+                hipStream_t s, void* scratch = nullptr);
 int launch_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                hipStream_t s);
 int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s);
+                hipStream_t s, void* scratch = nullptr);
 int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s);
+               hipStream_t s, void* scratch = nullptr);
 int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s);
+                hipStream_t s, void* scratch = nullptr);
 int launch_spmm_values(const rvvla_mat_s& A, bool transpose, const void* vals, int k, double alpha,
                        const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
                        hipStream_t s);

@@ -673,59 +673,61 @@ __global__ __launch_bounds__(kWG) void k_csr_spmm(
 
 template <class VT>
 static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, double beta,
-                   void* y, hipStream_t s) {
+                   void* y, hipStream_t s, void* scratch) {
   const bool xs_nt = nt_enabled();
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const bool comp = sizeof(VT) == 2;
   if (d.xl.S > 0) {
     const DevXLds& xl = d.xl;
+    AC* part = (AC*)(scratch ? scratch : xl.part);
     if (m == 0) return RVVLA_OK;
     if (xl.nunits > 0) {
       if (comp)
         hipLaunchKernelGGL((k_xl_slices<VT, true>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s, xl.col,
                            (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg, (const IO*)x,
-                           xl.n, (AC*)xl.part);
+                           xl.n, part);
       else
         hipLaunchKernelGGL((k_xl_slices<VT, false>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s,
                            xl.col, (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg,
-                           (const IO*)x, xl.n, (AC*)xl.part);
+                           (const IO*)x, xl.n, part);
       if (int rc = hip_check(hipGetLastError(), "k_xl_slices launch")) return rc;
     }
     const int nb = (int)xl.nblocks;
     const unsigned grid = (unsigned)(8 * ((nb + 7) / 8));
     if (comp)
       hipLaunchKernelGGL((k_xl_combine<AC, IO, true>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
-                         xl.cblk, xl.off, xl.lpos, (const AC*)xl.part, (IO*)y, xl.S, nb, (AC)alpha,
+                         xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha,
                          (AC)beta);
     else
       hipLaunchKernelGGL((k_xl_combine<AC, IO, false>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
-                         xl.cblk, xl.off, xl.lpos, (const AC*)xl.part, (IO*)y, xl.S, nb, (AC)alpha,
+                         xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha,
                          (AC)beta);
     return hip_check(hipGetLastError(), "k_xl_combine launch");
   }
   if (d.xs.P > 0) {
     const DevXSlice& xs = d.xs;
+    IO* xpart = (IO*)(scratch ? scratch : xs.part);
     if (m == 0) return RVVLA_OK;
     if (xs.nblk > 0) {
       if (comp)
         hipLaunchKernelGGL((k_csr_adaptive<VT, true, true>), dim3((unsigned)xs.nblk), dim3(kWG), 0,
                            s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
-                           (IO*)xs.part, (AC)1, (AC)0);
+                           xpart, (AC)1, (AC)0);
       else if (xs_nt)
         hipLaunchKernelGGL((k_csr_adaptive<VT, false, true>), dim3((unsigned)xs.nblk), dim3(kWG),
                            0, s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
-                           (IO*)xs.part, (AC)1, (AC)0);
+                           xpart, (AC)1, (AC)0);
       else
         hipLaunchKernelGGL((k_csr_adaptive<VT, false, false>), dim3((unsigned)xs.nblk), dim3(kWG),
                            0, s, xs.rst, xs.col, (const VT*)xs.val, xs.blk, (const IO*)x,
-                           (IO*)xs.part, (AC)1, (AC)0);
+                           xpart, (AC)1, (AC)0);
       if (int rc = hip_check(hipGetLastError(), "k_csr_adaptive (sliced) launch")) return rc;
     }
     const int64_t per_wg = (int64_t)(kWG / 64) * kCombineGroups * 64;
     const unsigned g = (unsigned)((m + per_wg - 1) / per_wg);
     hipLaunchKernelGGL((k_xs_combine<IO, AC>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base, xs.pref,
-                       (const IO*)xs.part, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
+                       xpart, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_xs_combine launch");
   }
   if (d.nblocks == 0) return RVVLA_OK;
@@ -745,19 +747,19 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
 }
 
 int launch_csr_image(const DevCSR& d, int64_t m, int dtype, double alpha, const void* x,
-                     double beta, void* y, hipStream_t s) {
+                     double beta, void* y, hipStream_t s, void* scratch) {
   switch (dtype) {
-    case RVVLA_F64: return run_csr<double>(d, m, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_csr<float>(d, m, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_csr<bf16_t>(d, m, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_csr<_Float16>(d, m, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_csr<double>(d, m, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_csr<float>(d, m, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_csr<bf16_t>(d, m, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_csr<_Float16>(d, m, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }
 
 int launch_csr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s) {
-  return launch_csr_image(A.csr, A.m, A.dtype, alpha, x, beta, y, s);
+               hipStream_t s, void* scratch) {
+  return launch_csr_image(A.csr, A.m, A.dtype, alpha, x, beta, y, s, scratch);
 }
 
 template <class VT>

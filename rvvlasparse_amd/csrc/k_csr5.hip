@@ -262,14 +262,14 @@ __global__ __launch_bounds__(kWG) void k_csr5_long(
 
 template <class VT>
 static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                    hipStream_t s) {
+                    hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevCSR5& d = A.csr5;
   Csr5Args a{d.tile_ptr, d.desc,   d.eptr,   d.eoff,    d.rst,        d.col,
              d.empty_rows, d.chain, d.long_fix, d.ntiles, d.nempty, A.m, d.nlong,
              d.sigma, d.dl, d.bit_y, d.bit_seg, d.tail_start, d.first_tail_row, d.tail_base};
-  AC* fp = (AC*)d.scratch;
+  AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* lp = fp + d.ntiles;
   int32_t* lr = (int32_t*)(lp + d.ntiles);
   if (d.ntiles > 0) {
@@ -303,12 +303,12 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
 }
 
 int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s) {
+                hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_csr5<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_csr5<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_csr5<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_csr5<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_csr5<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_csr5<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_csr5<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_csr5<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }

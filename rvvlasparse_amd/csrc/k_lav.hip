@@ -148,10 +148,11 @@ __global__ __launch_bounds__(kWG) void k_lav_combine(
 
 template <class VT>
 static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                   hipStream_t s) {
+                   hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevLAV& d = A.lav;
+  AC* part = (AC*)(scratch ? scratch : d.part);
   // sparse_part with beta first, unless each row's sparse entries are added
   // by the dense kernels (fused)
   if (!d.fused)
@@ -173,15 +174,15 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
   if (d.nsplit > 0) {
     hipLaunchKernelGGL((k_lav_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
                        0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
-                       (AC*)d.part);
+                       part);
     if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
     if (d.fused)
       hipLaunchKernelGGL((k_lav_combine<VT, true>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
-                         d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order, (IO*)y, A.m,
+                         d.sp_chunk, d.sp_first, part, d.out_order, (IO*)y, A.m,
                          (AC)alpha, sp, (const IO*)x, (AC)beta);
     else
       hipLaunchKernelGGL((k_lav_combine<VT, false>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
-                         d.sp_chunk, d.sp_first, (const AC*)d.part, d.out_order, (IO*)y, A.m,
+                         d.sp_chunk, d.sp_first, part, d.out_order, (IO*)y, A.m,
                          (AC)alpha, sp, (const IO*)x, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_lav_combine launch")) return rc;
   }
@@ -189,12 +190,12 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
 }
 
 int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s) {
+               hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_lav<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_lav<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_lav<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_lav<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_lav<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_lav<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_lav<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_lav<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }

@@ -118,10 +118,11 @@ __global__ __launch_bounds__(kWG) void k_sell_combine(
 
 template <class VT>
 static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                    hipStream_t s) {
+                    hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevSELL& d = A.sell;
+  AC* part = (AC*)(scratch ? scratch : d.part);
   if (d.nslices == 0) return RVVLA_OK;
   const unsigned grid = (unsigned)((d.nslices + kWG / 64 - 1) / (kWG / 64));
   // nontemporal streaming pays for 8-byte values; narrower values run
@@ -144,22 +145,22 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
     constexpr int W = kWG / 64;
     hipLaunchKernelGGL((k_sell_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
                        0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs, d.col,
-                       (const VT*)d.val, (const IO*)x, (AC*)d.part);
+                       (const VT*)d.val, (const IO*)x, part);
     if (int rc = hip_check(hipGetLastError(), "k_sell_pieces launch")) return rc;
     hipLaunchKernelGGL((k_sell_combine<VT>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
                        d.sp_slice, d.sp_first, d.bs, d.sbs, d.perm,
-                       d.perm_identity, (const AC*)d.part, (IO*)y, (AC)alpha, (AC)beta);
+                       d.perm_identity, part, (IO*)y, (AC)alpha, (AC)beta);
   }
   return hip_check(hipGetLastError(), "k_sell64 launch");
 }
 
 int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s) {
+                hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_sell<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_sell<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_sell<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_sell<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_sell<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_sell<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_sell<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_sell<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }

@@ -146,13 +146,13 @@ __global__ __launch_bounds__(kWG) void k_vhcc_fixup(VhccArgs a,
 
 template <class VT>
 static int run_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                    hipStream_t s) {
+                    hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevVHCC& d = A.vhcc;
   VhccArgs a{d.start_vec, d.rbase, d.eor, d.row_arr, d.fwd_row, d.pass, d.col, d.empty_rows,
              d.nparts, d.nempty, A.m};
-  AC* fp = (AC*)d.scratch;
+  AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
   if (d.nparts > 0) {
     const unsigned g = (unsigned)((d.nparts + kWG / 64 - 1) / (kWG / 64));
@@ -170,12 +170,12 @@ static int run_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double be
 }
 
 int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s) {
+                hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_vhcc<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_vhcc<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_vhcc<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_vhcc<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_vhcc<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_vhcc<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_vhcc<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_vhcc<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }

@@ -132,12 +132,12 @@ __global__ __launch_bounds__(kWG) void k_vnec_fixup(
 
 template <class VT>
 static int run_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                    hipStream_t s) {
+                    hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
   const DevVNEC& d = A.vnec;
   if (d.nparts == 0) return RVVLA_OK;
-  AC* fp = (AC*)d.scratch;
+  AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
   hipLaunchKernelGGL((k_vnec<VT>), dim3((unsigned)d.nparts), dim3(kWG), 0, s, d.cx, d.cy,
                      d.nec_ptr, d.use_x, d.ecr, d.rst, (const VT*)d.val, (const IO*)x, (IO*)y, fp,
@@ -149,12 +149,12 @@ static int run_vnec(const rvvla_mat_s& A, double alpha, const void* x, double be
 }
 
 int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s) {
+                hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_vnec<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_vnec<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_vnec<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_vnec<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_vnec<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_vnec<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_vnec<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_vnec<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }
