@@ -230,14 +230,18 @@ static int device_xcds(const rvvla_mat_s& M) {
 // over 65536 columns away from the row's diagonal position: one slice per
 // XCD, or two per XCD when a slice of x would exceed the 4 MiB L2.
 template <class V>
-static void fill_sell(rvvla_mat_s& M, int dtype, const spMtxSELL_C_Sigma<V>& a);
+static void fill_sell(rvvla_mat_s& M, int dtype, const spMtxSELL_C_Sigma<V>& a,
+                      bool keep_perm = true);
 
 // The SELL-64 image of a CSR matrix (rows in natural order, sigma = 1):
 // `request` 1 = on, 0 = off, -1 = chosen from the matrix: at least 65536
-// rows whose 64-row slices pad by at most 1 % of the entries (the 5-point
-// Laplacian 2048^2: 53 us against 59 us for the adaptive CSR kernel on
-// MI355X, FP64).  The SELL kernel adds a row's entries in four interleaved
-// partial sums (entries j mod 4), a fixed order within the usual bound.
+// rows, none longer than kSellPiece entries (so every slice runs as one
+// wavefront), whose 64-row slices pad by at most 1 % of the entries and
+// hold fewer than 2^31 slots (the 5-point Laplacian 2048^2: 53 us against
+// 59 us for the adaptive CSR kernel on MI355X, FP64).  The SELL kernel adds
+// a row's entries in four interleaved partial sums (entries j mod 4), a
+// fixed order within the usual bound, and reads only the row's own entries
+// (the row lengths bound each lane), as the CSR product does.
 template <class V>
 static bool choose_csr_sell(const spMtxCRS<V>& a, int request) {
   if (request >= 0) return request > 0 && a.m > 0;
@@ -248,8 +252,10 @@ static bool choose_csr_sell(const spMtxCRS<V>& a, int request) {
     const int64_t r1 = std::min<int64_t>(m, r0 + 64);
     int64_t w = 0;
     for (int64_t r = r0; r < r1; ++r) w = std::max<int64_t>(w, a.Rst[r + 1] - a.Rst[r]);
+    if (w > kSellPiece) return false;
     padded += w * (r1 - r0);
   }
+  if (padded > 2147483647LL) return false;
   return (double)(padded - nnz) <= 0.01 * (double)nnz;
 }
 
@@ -338,7 +344,7 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, i
   if (P == 0 && choose_csr_sell(a, sell)) {
     spMtxSELL_C_Sigma<V> S;
     build_sell<V>(S, a, 64, 1);
-    fill_sell(*M, dtype, S);
+    fill_sell(*M, dtype, S, false);
     M->csr.sell = true;
   }
   if (P > 0) {
@@ -594,7 +600,8 @@ rvvla_mat_t upload_sell(int dtype, const spMtxSELL_C_Sigma<V>& a) {
 
 // The SELL-C-sigma device image of a checked container into M.sell.
 template <class V>
-static void fill_sell(rvvla_mat_s& Mr, int dtype, const spMtxSELL_C_Sigma<V>& a) {
+static void fill_sell(rvvla_mat_s& Mr, int dtype, const spMtxSELL_C_Sigma<V>& a,
+                      bool keep_perm) {
   const int64_t m = (int64_t)a.m, nb = (int64_t)a.cnt_b;
   rvvla_mat_s* M = &Mr;
   DevSELL& d = M->sell;
@@ -604,9 +611,11 @@ static void fill_sell(rvvla_mat_s& Mr, int dtype, const spMtxSELL_C_Sigma<V>& a)
   d.bs = M->put_vec(a.vBs);
   d.sbs = M->put_vec(a.vSBs);
   d.col = M->put_vec(a.vCol);
-  d.perm = M->put_vec(a.vPerm);
   d.perm_identity = true;
   for (int64_t s = 0; s < m && d.perm_identity; ++s) d.perm_identity = a.vPerm[(size_t)s] == (uint32_t)s;
+  // the identity permutation is never read by the kernels; the CSR image
+  // does not keep it
+  d.perm = d.perm_identity && !keep_perm ? nullptr : M->put_vec(a.vPerm);
   d.val = put_values(*M, dtype, a.vVal.data(), a.vVal.size());
   attach_sell_pieces(*M, a.vCl);
 }
@@ -1856,6 +1865,7 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
             : f == "xl.off"    ? V{c.xl.off, c.xl.S * (c.xl.nblocks + 1), 4}
             : f == "xl.lpos"   ? V{c.xl.lpos, c.xl.npieces, 2}
             : f == "sell.cl"   ? V{A->sell.cl, c.sell ? A->sell.nslices : 0, 4}
+            : f == "sell.vPerm" ? V{A->sell.perm, c.sell && A->sell.perm ? A->m : 0, 4}
                                : V{nullptr, -1, 0};
   if (v.n < 0) {
     rvvla::set_last_error("rvvla_mat_view: unknown field " + f);

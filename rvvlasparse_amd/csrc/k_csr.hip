@@ -110,9 +110,9 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
   // longer rows: the whole wavefront of the row's thread, one row at a time
   const bool is_long = tid < nrows && hi - lo > kShortSum;
   if (tid < nrows && !is_long) {
-    A s = 0;
-    for (int k = lo; k < hi; ++k) s += prod[k];
-    y[t_row] = alpha * s + scaled_y(beta, y, t_row);
+    Acc<A, COMP> s;
+    for (int k = lo; k < hi; ++k) s.add(prod[k]);
+    y[t_row] = alpha * s.value() + scaled_y(beta, y, t_row);
   }
   const int lane = tid & 63;
   uint64_t lm = __ballot(is_long);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
     const int l = __ffsll((unsigned long long)lm) - 1;
     lm &= lm - 1;
     const int rlo = __shfl(lo, l, 64), rhi = __shfl(hi, l, 64);
-    Acc<A, false> a;
+    Acc<A, COMP> a;
     for (int k = rlo + lane; k < rhi; k += 64) a.add(prod[k]);
     a = group_reduce(a, 64);
     if (lane == 0) {
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kWG) void k_csr_adaptive(
 // pref[p*nq + q] + (lower lanes with bit p set).  All partial loads of all
 // groups are issued before the (in-order) sums.
 constexpr int kCombineGroups = 1;
-template <class IO, class A>
+template <class IO, class A, bool COMP>
 __global__ __launch_bounds__(kWG) void k_xs_combine(
     const uint16_t* __restrict__ mask, const int32_t* __restrict__ base,
     const int32_t* __restrict__ pref, const IO* __restrict__ part, IO* __restrict__ y,
@@ -166,11 +166,11 @@ __global__ __launch_bounds__(kWG) void k_xs_combine(
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     const int64_t r = (q0 + g) * 64 + lane;
-    A s = A(0);
+    Acc<A, COMP> s;
 #pragma unroll
     for (int p = 0; p < 16; ++p)
-      if ((mk[g] >> p) & 1u) s += v[g][p];
-    if (r < m) y[r] = alpha * s + scaled_y(beta, y, r);
+      if ((mk[g] >> p) & 1u) s.add(v[g][p]);
+    if (r < m) y[r] = alpha * s.value() + scaled_y(beta, y, r);
   }
 }
 
@@ -276,45 +276,6 @@ __device__ __forceinline__ void xl_unpack(const XlTaskData<VT>& d, typename Val<
   }
 }
 
-// DPP move of a 32- or 64-bit value (CTRL: row_shr:n = 0x110 | n, wave_shr:1
-// = 0x138); lanes without a source lane get 0.
-template <int CTRL>
-__device__ __forceinline__ float xl_dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double xl_dpp(double v) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ float xl_rl(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double xl_rl(double v, int l) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-template <class Ac>
-__device__ __forceinline__ Ac xl_readlane(const Ac& a, int l) {
-  Ac r;
-  r.s = xl_rl(a.s, l);
-  r.c = xl_rl(a.c, l);
-  return r;
-}
-// One in-row Hillis-Steele step of the segmented scan: lane l adds lane
-// l - D of its row when that lane is in its segment.
-template <int CTRL, int D, class Ac>
-__device__ __forceinline__ void xl_scan_step(Ac& S, int lr, int lane, int st) {
-  Ac o;
-  o.s = xl_dpp<CTRL>(S.s);
-  o.c = xl_dpp<CTRL>(S.c);
-  if (lr >= D && lane - D >= st) S.merge(o);
-}
-
 // Slots of a task staged in LDS: slot s of the task at s ^ ((s >> 5) & 7),
 // so that neither the lanes' scattered writes nor the contiguous read back
 // meet more than two-way bank conflicts.
@@ -397,32 +358,11 @@ __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename 
     run.add(prod[k]);
     stg[xl_swz(slot)] = run.value();
   }
-  // segmented inclusive scan of the open sums (lane 0 always holds a
-  // mark): Hillis-Steele steps inside each row of 16 lanes (DPP row
-  // shifts), then the open sum at the end of each row carried into the
-  // next rows' lanes whose segment started before them
-  const uint64_t H = __ballot(seen);
-  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int st = 63 - __clzll((long long)(H & upto));
-  const int lr = lane & 15;
-  Ac S = run;
-  xl_scan_step<0x111, 1>(S, lr, lane, st);
-  xl_scan_step<0x112, 2>(S, lr, lane, st);
-  xl_scan_step<0x114, 4>(S, lr, lane, st);
-  xl_scan_step<0x118, 8>(S, lr, lane, st);
-  {
-    // row r's lanes take the open sum at the end of row r - 1, which
-    // already holds the carries of the rows before it
-    const Ac T = xl_readlane(S, 15);
-    if (lane >= 16 && lane < 32 && st < 16) S.merge(T);
-    const Ac E = xl_readlane(S, 31);
-    if (lane >= 32 && lane < 48 && st < 32) S.merge(E);
-    const Ac F = xl_readlane(S, 47);
-    if (lane >= 48 && st < 48) S.merge(F);
-  }
+  // segmented inclusive scan of the open sums (lane 0 always holds a mark)
+  const Ac S = wave_seg_scan(run, __ballot(seen), lane);
   Ac prev;
-  prev.s = xl_dpp<0x138>(S.s);  // wave_shr:1: the lane below
-  prev.c = xl_dpp<0x138>(S.c);
+  prev.s = dpp_mov<0x138>(S.s);  // wave_shr:1: the lane below
+  prev.c = dpp_mov<0x138>(S.c);
   prev.merge(head);
   if (seen && lane > 0) stg[xl_swz(fb - 1)] = prev.value();
   if (lane == 63) stg[xl_swz(tot - 1)] = S.value();
@@ -726,8 +666,12 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
     }
     const int64_t per_wg = (int64_t)(kWG / 64) * kCombineGroups * 64;
     const unsigned g = (unsigned)((m + per_wg - 1) / per_wg);
-    hipLaunchKernelGGL((k_xs_combine<IO, AC>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base, xs.pref,
-                       xpart, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
+    if (comp)
+      hipLaunchKernelGGL((k_xs_combine<IO, AC, true>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base,
+                         xs.pref, xpart, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
+    else
+      hipLaunchKernelGGL((k_xs_combine<IO, AC, false>), dim3(g), dim3(kWG), 0, s, xs.mask, xs.base,
+                         xs.pref, xpart, (IO*)y, m, xs.nq, xs.P, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_xs_combine launch");
   }
   if (d.nblocks == 0) return RVVLA_OK;

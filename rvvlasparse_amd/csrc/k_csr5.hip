@@ -56,13 +56,14 @@ __device__ __forceinline__ uint64_t csr5_flags(const uint32_t* w, int sigma, int
 // consecutive entries at a time, and the values of x go back to the lane
 // that owns the entry in the tile layout.  SIG = 0: any sigma, four tiles
 // per workgroup, gathers in tile order.
-template <class VT, int SIG>
+template <class VT, int SIG, bool COMP>
 __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
     Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ first_part,
     typename Val<VT>::acc* __restrict__ last_part, int32_t* __restrict__ last_row,
     typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
+  using Ac = Acc<A, COMP>;
   constexpr int TPB = SIG > 0 ? 1 : kWG / 64;  // tiles per workgroup
   __shared__ A xb[SIG > 0 ? 64 * (SIG + 1) : 1];
   const int64_t t = (int64_t)blockIdx.x * TPB + (threadIdx.x >> 6);
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
   auto row_of = [&](uint32_t ord) -> int64_t {  // ord: 1-based counted-flag ordinal
     return has_empty ? row0 + 1 + (int64_t)a.eoff[ebase + ord - 1] : row0 + (int64_t)ord;
   };
-  A head = 0, cur = 0;
+  Ac head, cur;
   uint32_t nfl = 0;
   const int64_t base = t * 64 * (int64_t)sigma + lane;
   const uint32_t* __restrict__ colp = a.col;
@@ -93,12 +94,12 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
         head = cur;
       } else {
         const int64_t r = row_of(yoff + nfl);
-        y[r] = alpha * cur + scaled_y(beta, y, r);
+        y[r] = alpha * cur.value() + scaled_y(beta, y, r);
       }
-      cur = 0;
+      cur = Ac();
       ++nfl;
     }
-    cur += p;
+    cur.add(p);
   };
   if constexpr (SIG > 0) {
     constexpr int W = sizeof(A) / 4;  // 32-bit words per LDS slot
@@ -155,22 +156,28 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
   if (!flagged) head = cur;
   // U[l] = head[l] + head[l+1] + ... up to and including the next flagged
   // lane (segmented suffix scan; a flagged lane closes its own chain)
-  A U = head;
+  Ac U = head;
   int closed = flagged ? 1 : 0;
   for (int d = 1; d < 64; d <<= 1) {
-    const A ov = __shfl_down(U, d, 64);
+    Ac ov;
+    ov.s = __shfl_down(U.s, d, 64);
+    if constexpr (COMP) ov.c = __shfl_down(U.c, d, 64);
     const int oc = __shfl_down(closed, d, 64);
     if (!closed && lane + d < 64) {
-      U += ov;
+      U.merge(ov);
       closed |= oc;
     }
   }
-  A cont = __shfl_down(U, 1, 64);
-  if (lane == 63) cont = 0;
+  Ac cont;
+  cont.s = __shfl_down(U.s, 1, 64);
+  if constexpr (COMP) cont.c = __shfl_down(U.c, 1, 64);
+  if (lane == 63) cont = Ac();
   const uint64_t mask = __ballot(flagged);
   const int last_lane = mask ? 63 - __builtin_clzll(mask) : -1;
   if (flagged) {
-    const A s = (cur) + cont;
+    Ac sc = cur;
+    sc.merge(cont);
+    const A s = sc.value();
     const int64_t r = row_of(yoff + nfl);
     if (lane != last_lane) {
       y[r] = alpha * s + scaled_y(beta, y, r);
@@ -181,18 +188,18 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
   }
   if (lane == 0) {
     if (starts_here && mask) {
-      y[row0] = alpha * U + scaled_y(beta, y, row0);
+      y[row0] = alpha * U.value() + scaled_y(beta, y, row0);
     } else if (starts_here) {
-      last_part[t] = U;
+      last_part[t] = U.value();
       last_row[t] = (int32_t)row0;
     } else {
-      first_part[t] = U;
+      first_part[t] = U.value();
       if (!mask) last_row[t] = -1;
     }
   }
 }
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_csr5_fixup(
     Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
@@ -207,17 +214,18 @@ __global__ __launch_bounds__(kWG) void k_csr5_fixup(
     const uint32_t ch = (uint32_t)a.chain[i];
     const int64_t q1 = ch & 0x7FFFFFFFu;
     if ((ch >> 31) || q1 - i - 1 > kCsr5Serial) return;  // k_csr5_long's
-    A s = last_part[i];
-    for (int64_t q = i + 1; q < q1; ++q) s += first_part[q];
-    y[r] = alpha * s + scaled_y(beta, y, r);
+    Acc<A, COMP> s;
+    s.add(last_part[i]);
+    for (int64_t q = i + 1; q < q1; ++q) s.add(first_part[q]);
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
   } else if (i < a.ntiles + ntail) {
     const int64_t r = a.first_tail_row + (i - a.ntiles);
     const int64_t e0 = a.rst[r], e1 = a.rst[r + 1];
     if (e0 == e1) return;  // empty rows are in the empty list
     if (e1 - e0 > kCsr5Serial) return;  // k_csr5_long's
-    A s = 0;
-    for (int64_t e = e0; e < e1; ++e) s += Val<VT>::get(val, e) * (A)x[a.col[e]];
-    y[r] = alpha * s + scaled_y(beta, y, r);
+    Acc<A, COMP> s;
+    for (int64_t e = e0; e < e1; ++e) s.add(Val<VT>::get(val, e) * (A)x[a.col[e]]);
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
   } else if (i < a.ntiles + ntail + a.nempty) {
     const int64_t r = a.empty_rows[i - a.ntiles - ntail];
     y[r] = scaled_y(beta, y, r);
@@ -228,7 +236,7 @@ __global__ __launch_bounds__(kWG) void k_csr5_fixup(
 // the last part of its tile, the leading parts of the tiles it spans and
 // (if it runs into the tail) its tail entries, lanes striding and then a
 // butterfly; a tail row is summed the same way.
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_csr5_long(
     Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
@@ -239,7 +247,7 @@ __global__ __launch_bounds__(kWG) void k_csr5_long(
   if (w >= a.nlong) return;
   const int lane = threadIdx.x & 63;
   const int32_t code = a.long_fix[w];
-  Acc<A, false> s;
+  Acc<A, COMP> s;
   int64_t r, e0, e1;
   if (code >= 0) {
     const int64_t i = code;
@@ -269,19 +277,20 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
   Csr5Args a{d.tile_ptr, d.desc,   d.eptr,   d.eoff,    d.rst,        d.col,
              d.empty_rows, d.chain, d.long_fix, d.ntiles, d.nempty, A.m, d.nlong,
              d.sigma, d.dl, d.bit_y, d.bit_seg, d.tail_start, d.first_tail_row, d.tail_base};
+  constexpr bool COMP = sizeof(VT) == 2;  // 16-bit values: TwoSum throughout
   AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* lp = fp + d.ntiles;
   int32_t* lr = (int32_t*)(lp + d.ntiles);
   if (d.ntiles > 0) {
     if (d.sigma == 16) {
-      hipLaunchKernelGGL((k_csr5_tiles<VT, 16>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 16, COMP>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
                          (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     } else if (d.sigma == 8) {
-      hipLaunchKernelGGL((k_csr5_tiles<VT, 8>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 8, COMP>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
                          (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     } else {
       const unsigned g1 = (unsigned)((d.ntiles + kWG / 64 - 1) / (kWG / 64));
-      hipLaunchKernelGGL((k_csr5_tiles<VT, 0>), dim3(g1), dim3(kWG), 0, s, a, (const VT*)d.val,
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 0, COMP>), dim3(g1), dim3(kWG), 0, s, a, (const VT*)d.val,
                          (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     }
     if (int rc = hip_check(hipGetLastError(), "k_csr5_tiles launch")) return rc;
@@ -289,13 +298,13 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
   const int64_t nfix = d.ntiles + (A.m - d.first_tail_row) + d.nempty;
   if (nfix > 0) {
     const unsigned g2 = (unsigned)((nfix + kWG - 1) / kWG);
-    hipLaunchKernelGGL((k_csr5_fixup<VT>), dim3(g2), dim3(kWG), 0, s, a, (const VT*)d.val,
+    hipLaunchKernelGGL((k_csr5_fixup<VT, COMP>), dim3(g2), dim3(kWG), 0, s, a, (const VT*)d.val,
                        (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_csr5_fixup launch")) return rc;
   }
   if (d.nlong > 0) {
     const unsigned g3 = (unsigned)((d.nlong + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_csr5_long<VT>), dim3(g3), dim3(kWG), 0, s, a, (const VT*)d.val,
+    hipLaunchKernelGGL((k_csr5_long<VT, COMP>), dim3(g3), dim3(kWG), 0, s, a, (const VT*)d.val,
                        (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_csr5_long launch");
   }

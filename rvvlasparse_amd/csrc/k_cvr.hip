@@ -15,7 +15,7 @@
 
 namespace rvvla {
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_cvr(
     const int64_t* __restrict__ pack_start, const int32_t* __restrict__ rec_start,
     const uint64_t* __restrict__ flush, const int32_t* __restrict__ rec_rows,
@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kWG) void k_cvr(
   const uint64_t below = (uint64_t(1) << lane) - 1;
   const int64_t k0 = pack_start[t], k1 = pack_start[t + 1];
   int32_t rb = rec_start[t];
-  A acc = 0;
+  Acc<A, COMP> acc;  // 16-bit values (COMP): TwoSum compensated
   // kBatch packs at a time: masks, flushed rows, columns, values and the
   // gathers are all issued before the packs are walked
   constexpr int kBatch = 8;
@@ -56,14 +56,14 @@ __global__ __launch_bounds__(kWG) void k_cvr(
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if ((mk[u] >> lane) & 1u) {
-        y[rr[u]] = alpha * acc + scaled_y(beta, y, rr[u]);
-        acc = 0;
+        y[rr[u]] = alpha * acc.value() + scaled_y(beta, y, rr[u]);
+        acc = Acc<A, COMP>();
       }
-      acc += p[u];  // zero past the stream's last pack
+      acc.add(p[u]);  // zero past the stream's last pack
     }
   }
   const int32_t r = tail_rows[t * 64 + lane];
-  if (r >= 0) y[r] = alpha * acc + scaled_y(beta, y, r);
+  if (r >= 0) y[r] = alpha * acc.value() + scaled_y(beta, y, r);
 }
 
 template <class VT>
@@ -85,7 +85,8 @@ static int run_cvr(const rvvla_mat_s& A, double alpha, const void* x, double bet
   const DevCVR& d = A.cvr;
   if (d.nstreams > 0) {
     const unsigned g = (unsigned)((d.nstreams + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_cvr<VT>), dim3(g), dim3(kWG), 0, s, d.pack_start, d.rec_start, d.flush,
+    hipLaunchKernelGGL((k_cvr<VT, sizeof(VT) == 2>), dim3(g), dim3(kWG), 0, s, d.pack_start,
+                       d.rec_start, d.flush,
                        d.rec_rows, d.tail_rows, d.col, (const VT*)d.val, (const IO*)x, (IO*)y,
                        d.nstreams, (AC)alpha, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_cvr launch")) return rc;

@@ -38,7 +38,7 @@ __device__ __forceinline__ typename Val<VT>::acc lav_sparse_row(
   return s;
 }
 
-template <class VT, bool FUSED>
+template <class VT, bool FUSED, bool COMP>
 __global__ __launch_bounds__(kWG) void k_lav_dense(
     const int32_t* __restrict__ coff, const int64_t* __restrict__ sptr, int64_t nseg,
     int64_t nchunks, const int32_t* __restrict__ whole, int64_t nwhole,
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
       s1 = sp.rst[r + 1];
     }
   }
-  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  Acc<A, COMP> a0, a1, a2, a3;
   for (int64_t g = 0; g < nseg; ++g) {
     const int64_t base = sptr[g];
     int64_t k = base + coff[g * (nchunks + 1) + c];
@@ -72,18 +72,21 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
       const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
       const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
       const A v2 = Val<VT>::get(val, e0 + 128), v3 = Val<VT>::get(val, e0 + 192);
-      a0 += v0 * (A)x[c0];
-      a1 += v1 * (A)x[c1];
-      a2 += v2 * (A)x[c2];
-      a3 += v3 * (A)x[c3];
+      a0.add(v0 * (A)x[c0]);
+      a1.add(v1 * (A)x[c1]);
+      a2.add(v2 * (A)x[c2]);
+      a3.add(v3 * (A)x[c3]);
     }
     for (; k < k1; ++k) {
       const int64_t e0 = k * 64 + lane;
-      a0 += Val<VT>::get(val, e0) * (A)x[col[e0]];
+      a0.add(Val<VT>::get(val, e0) * (A)x[col[e0]]);
     }
   }
   if (rp < m) {
-    const A dense = (a0 + a1) + (a2 + a3);
+    a0.merge(a1);
+    a2.merge(a3);
+    a0.merge(a2);
+    const A dense = a0.value();
     if constexpr (FUSED)
       y[r] = alpha * (lav_sparse_row(sp, s0, s1, x) + dense) + scaled_y(beta, y, r);
     else
@@ -92,8 +95,10 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
 }
 
 // One piece (at most kLavPiece steps of one long chunk in one segment) per
-// wavefront: its 64 lane sums go to part[piece * 64 + lane].
-template <class VT>
+// wavefront: its 64 lane sums go to part[piece * 64 + lane].  In the 16-bit
+// value forms (COMP) the dense sums, pieces and combine add with TwoSum
+// compensation (long chunks hold the long rows).
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_lav_pieces(
     const int64_t* __restrict__ pc_k0, const int64_t* __restrict__ pc_k1, int64_t npieces,
     const uint32_t* __restrict__ col, const VT* __restrict__ val,
@@ -102,7 +107,7 @@ __global__ __launch_bounds__(kWG) void k_lav_pieces(
   const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
   if (q >= npieces) return;
   const int lane = threadIdx.x & 63;
-  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  Acc<A, COMP> a0, a1, a2, a3;
   int64_t k = pc_k0[q];
   const int64_t k1 = pc_k1[q];
   for (; k + 3 < k1; k += 4) {
@@ -110,31 +115,34 @@ __global__ __launch_bounds__(kWG) void k_lav_pieces(
     const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
     const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
     const A v2 = Val<VT>::get(val, e0 + 128), v3 = Val<VT>::get(val, e0 + 192);
-    a0 += v0 * (A)x[c0];
-    a1 += v1 * (A)x[c1];
-    a2 += v2 * (A)x[c2];
-    a3 += v3 * (A)x[c3];
+    a0.add(v0 * (A)x[c0]);
+    a1.add(v1 * (A)x[c1]);
+    a2.add(v2 * (A)x[c2]);
+    a3.add(v3 * (A)x[c3]);
   }
   for (; k < k1; ++k) {
     const int64_t e0 = k * 64 + lane;
-    a0 += Val<VT>::get(val, e0) * (A)x[col[e0]];
+    a0.add(Val<VT>::get(val, e0) * (A)x[col[e0]]);
   }
-  part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
+  a0.merge(a1);
+  a2.merge(a3);
+  a0.merge(a2);
+  part[q * 64 + lane] = a0.value();
 }
 
 // One split chunk per workgroup: the sum of its pieces' partials (all four
 // wavefronts load pieces), then the first wavefront adds alpha * sum into
 // the chunk's rows.
-template <class VT, bool FUSED>
+template <class VT, bool FUSED, bool COMP>
 __global__ __launch_bounds__(kWG) void k_lav_combine(
     const int32_t* __restrict__ sp_chunk, const int32_t* __restrict__ sp_first,
     const typename Val<VT>::acc* __restrict__ part, const uint32_t* __restrict__ out_order,
     typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha,
     LavSparse<VT> sp, const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  __shared__ A lds[kWG];
+  __shared__ A lds[COMP ? 2 * kWG : kWG];
   const int64_t w = blockIdx.x;
-  const A s = wg_piece_sum(part, sp_first[w], sp_first[w + 1], lds);
+  const A s = wg_piece_sum<A, COMP>(part, sp_first[w], sp_first[w + 1], lds);
   if (threadIdx.x >= 64) return;
   const int64_t rp = (int64_t)sp_chunk[w] * 64 + threadIdx.x;
   if (rp < m) {
@@ -160,28 +168,29 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
   if (d.nseg == 0 || d.nchunks == 0) return RVVLA_OK;
   const LavSparse<VT> sp{A.csr.rst, A.csr.col, (const VT*)A.csr.val};
   constexpr int W = kWG / 64;
+  constexpr bool COMP = sizeof(VT) == 2;
   if (d.nwhole > 0) {
     const unsigned g = (unsigned)((d.nwhole + W - 1) / W);
 #define RVVLA_LAV_DENSE(F)                                                                      \
-  hipLaunchKernelGGL((k_lav_dense<VT, F>), dim3(g), dim3(kWG), 0, s, d.coff, d.sptr, d.nseg,    \
-                     d.nchunks, d.whole, d.nwhole, d.out_order, d.col, (const VT*)d.val,        \
-                     (const IO*)x, (IO*)y, A.m, (AC)alpha, sp, (AC)beta)
+  hipLaunchKernelGGL((k_lav_dense<VT, F, COMP>), dim3(g), dim3(kWG), 0, s, d.coff, d.sptr,     \
+                     d.nseg, d.nchunks, d.whole, d.nwhole, d.out_order, d.col,                \
+                     (const VT*)d.val, (const IO*)x, (IO*)y, A.m, (AC)alpha, sp, (AC)beta)
     if (d.fused) RVVLA_LAV_DENSE(true);
     else RVVLA_LAV_DENSE(false);
 #undef RVVLA_LAV_DENSE
     if (int rc = hip_check(hipGetLastError(), "k_lav_dense launch")) return rc;
   }
   if (d.nsplit > 0) {
-    hipLaunchKernelGGL((k_lav_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
+    hipLaunchKernelGGL((k_lav_pieces<VT, COMP>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
                        0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
                        part);
     if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
     if (d.fused)
-      hipLaunchKernelGGL((k_lav_combine<VT, true>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+      hipLaunchKernelGGL((k_lav_combine<VT, true, COMP>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
                          d.sp_chunk, d.sp_first, part, d.out_order, (IO*)y, A.m,
                          (AC)alpha, sp, (const IO*)x, (AC)beta);
     else
-      hipLaunchKernelGGL((k_lav_combine<VT, false>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+      hipLaunchKernelGGL((k_lav_combine<VT, false, COMP>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
                          d.sp_chunk, d.sp_first, part, d.out_order, (IO*)y, A.m,
                          (AC)alpha, sp, (const IO*)x, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_lav_combine launch")) return rc;

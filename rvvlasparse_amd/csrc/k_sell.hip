@@ -17,14 +17,18 @@
 
 namespace rvvla {
 
-template <class VT, bool IDENT, bool NT>
+// BOUND (the SELL image of a CSR matrix, rows in natural order): a lane
+// adds only the entries of its row (rst gives the length), so that padding
+// slots never multiply x, as in the CSR product; the SELL format itself
+// multiplies its zero padding like the reference's SELL kernels.
+template <class VT, bool IDENT, bool NT, bool BOUND>
 __global__ __launch_bounds__(kWG) void k_sell64(
     const int32_t* __restrict__ cl, const int32_t* __restrict__ cs,
     const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
     const uint32_t* __restrict__ col, const VT* __restrict__ val,
     const uint32_t* __restrict__ perm, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, int64_t nslices, typename Val<VT>::acc alpha,
-    typename Val<VT>::acc beta) {
+    typename Val<VT>::acc beta, const int32_t* __restrict__ rst) {
   using A = typename Val<VT>::acc;
   // the slice index is uniform across the wavefront: keep it (and the slice
   // descriptor loads) scalar
@@ -39,6 +43,8 @@ __global__ __launch_bounds__(kWG) void k_sell64(
   if (lane >= height) return;
   int64_t idx = base + lane;
   A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int len = width;  // BOUND: the row's own entries
+  if constexpr (BOUND) len = rst[srow + lane + 1] - rst[srow + lane];
   int j = 0;
   for (; j + 3 < width; j += 4) {
     // all column and value loads first, then the four gathers
@@ -48,34 +54,51 @@ __global__ __launch_bounds__(kWG) void k_sell64(
     const A v0 = ldv<NT>(val, idx), v1 = ldv<NT>(val, idx + height);
     const A v2 = ldv<NT>(val, idx + 2 * (int64_t)height);
     const A v3 = ldv<NT>(val, idx + 3 * (int64_t)height);
-    a0 += v0 * (A)x[c0];
-    a1 += v1 * (A)x[c1];
-    a2 += v2 * (A)x[c2];
-    a3 += v3 * (A)x[c3];
+    if constexpr (BOUND) {
+      a0 += j < len ? v0 * (A)x[c0] : A(0);
+      a1 += j + 1 < len ? v1 * (A)x[c1] : A(0);
+      a2 += j + 2 < len ? v2 * (A)x[c2] : A(0);
+      a3 += j + 3 < len ? v3 * (A)x[c3] : A(0);
+    } else {
+      a0 += v0 * (A)x[c0];
+      a1 += v1 * (A)x[c1];
+      a2 += v2 * (A)x[c2];
+      a3 += v3 * (A)x[c3];
+    }
     idx += 4 * (int64_t)height;
   }
-  for (; j < width; ++j, idx += height) a0 += ldv<NT>(val, idx) * (A)x[ldu<NT>(col, idx)];
+  for (; j < width; ++j, idx += height) {
+    const A p = ldv<NT>(val, idx) * (A)x[ldu<NT>(col, idx)];
+    a0 += !BOUND || j < len ? p : A(0);
+  }
   const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
   y[r] = alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r);
 }
 
 // One piece (steps [j0, j1) of a wide slice) per wavefront: lane partials
-// to part[piece * 64 + lane].
-template <class VT>
+// to part[piece * 64 + lane].  Wide slices hold the long rows: in the
+// 16-bit value forms (COMP) every addition of a piece and of the combine is
+// compensated (TwoSum) and a piece's partial is its compensated sum.
+template <class VT, bool COMP, bool BOUND>
 __global__ __launch_bounds__(kWG) void k_sell_pieces(
     const int32_t* __restrict__ pc_slice, const int32_t* __restrict__ pc_j0,
     const int32_t* __restrict__ pc_j1, int64_t npieces, const int32_t* __restrict__ cs,
     const int32_t* __restrict__ bs, const uint32_t* __restrict__ col, const VT* __restrict__ val,
-    const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc* __restrict__ part) {
+    const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc* __restrict__ part,
+    const int32_t* __restrict__ sbs, const int32_t* __restrict__ rst) {
   using A = typename Val<VT>::acc;
   const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
   if (q >= npieces) return;
   const int lane = threadIdx.x & 63;
   const int sl = pc_slice[q], height = bs[sl];
-  A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  Acc<A, COMP> a0, a1, a2, a3;
   if (lane < height) {
     int j = pc_j0[q];
-    const int j1 = pc_j1[q];
+    int j1 = pc_j1[q];
+    if constexpr (BOUND) {  // the row's own entries only
+      const int r = sbs[sl] + lane;
+      j1 = min(j1, rst[r + 1] - rst[r]);
+    }
     int64_t idx = cs[sl] + (int64_t)j * height + lane;
     for (; j + 3 < j1; j += 4) {
       const uint32_t c0 = col[idx], c1 = col[idx + height];
@@ -83,21 +106,24 @@ __global__ __launch_bounds__(kWG) void k_sell_pieces(
       const A v0 = Val<VT>::get(val, idx), v1 = Val<VT>::get(val, idx + height);
       const A v2 = Val<VT>::get(val, idx + 2 * (int64_t)height);
       const A v3 = Val<VT>::get(val, idx + 3 * (int64_t)height);
-      a0 += v0 * (A)x[c0];
-      a1 += v1 * (A)x[c1];
-      a2 += v2 * (A)x[c2];
-      a3 += v3 * (A)x[c3];
+      a0.add(v0 * (A)x[c0]);
+      a1.add(v1 * (A)x[c1]);
+      a2.add(v2 * (A)x[c2]);
+      a3.add(v3 * (A)x[c3]);
       idx += 4 * (int64_t)height;
     }
-    for (; j < j1; ++j, idx += height) a0 += Val<VT>::get(val, idx) * (A)x[col[idx]];
+    for (; j < j1; ++j, idx += height) a0.add(Val<VT>::get(val, idx) * (A)x[col[idx]]);
   }
-  part[q * 64 + lane] = (a0 + a1) + (a2 + a3);
+  a0.merge(a1);
+  a2.merge(a3);
+  a0.merge(a2);
+  part[q * 64 + lane] = a0.value();
 }
 
 // One wide slice per workgroup: the sum of its pieces' partials (all four
 // wavefronts load pieces; a slice of the longest rows has hundreds), then
 // the first wavefront writes the slice's rows.
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_sell_combine(
     const int32_t* __restrict__ sp_slice, const int32_t* __restrict__ sp_first,
     const int32_t* __restrict__ bs, const int32_t* __restrict__ sbs,
@@ -105,10 +131,10 @@ __global__ __launch_bounds__(kWG) void k_sell_combine(
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  __shared__ A lds[kWG];
+  __shared__ A lds[COMP ? 2 * kWG : kWG];
   const int64_t w = blockIdx.x;
   const int sl = sp_slice[w];
-  const A s = wg_piece_sum(part, sp_first[w], sp_first[w + 1], lds);
+  const A s = wg_piece_sum<A, COMP>(part, sp_first[w], sp_first[w + 1], lds);
   const int lane = threadIdx.x;
   if (lane >= bs[sl]) return;  // rows past the slice's height, and wavefronts 1..3
   const int k = sbs[sl] + lane;
@@ -128,26 +154,37 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
   // nontemporal streaming pays for 8-byte values; narrower values run
   // faster with ordinary loads
   const bool nt = nt_enabled() && sizeof(VT) == 8;
-#define RVVLA_SELL_LAUNCH(ID, NT_)                                                              \
-  hipLaunchKernelGGL((k_sell64<VT, ID, NT_>), dim3(grid), dim3(kWG), 0, s, d.cl, d.cs, d.bs,    \
-                     d.sbs, d.col, (const VT*)d.val, d.perm, (const IO*)x, (IO*)y, d.nslices,   \
-                     (AC)alpha, (AC)beta)
-  if (d.perm_identity) {
-    if (nt) RVVLA_SELL_LAUNCH(true, true);
-    else RVVLA_SELL_LAUNCH(true, false);
+  // the SELL image of a CSR matrix: rows bounded by their CSR lengths
+  const bool bound = A.format == RVVLA_FMT_CSR;
+#define RVVLA_SELL_LAUNCH(ID, NT_, B_)                                                          \
+  hipLaunchKernelGGL((k_sell64<VT, ID, NT_, B_>), dim3(grid), dim3(kWG), 0, s, d.cl, d.cs,      \
+                     d.bs, d.sbs, d.col, (const VT*)d.val, d.perm, (const IO*)x, (IO*)y,        \
+                     d.nslices, (AC)alpha, (AC)beta, A.csr.rst)
+  if (bound) {  // natural row order: the identity permutation
+    if (nt) RVVLA_SELL_LAUNCH(true, true, true);
+    else RVVLA_SELL_LAUNCH(true, false, true);
+  } else if (d.perm_identity) {
+    if (nt) RVVLA_SELL_LAUNCH(true, true, false);
+    else RVVLA_SELL_LAUNCH(true, false, false);
   } else {
-    if (nt) RVVLA_SELL_LAUNCH(false, true);
-    else RVVLA_SELL_LAUNCH(false, false);
+    if (nt) RVVLA_SELL_LAUNCH(false, true, false);
+    else RVVLA_SELL_LAUNCH(false, false, false);
   }
 #undef RVVLA_SELL_LAUNCH
   if (d.nsplit > 0) {
     if (int rc = hip_check(hipGetLastError(), "k_sell64 launch")) return rc;
     constexpr int W = kWG / 64;
-    hipLaunchKernelGGL((k_sell_pieces<VT>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
-                       0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs, d.col,
-                       (const VT*)d.val, (const IO*)x, part);
+    constexpr bool COMP = sizeof(VT) == 2;
+    if (bound)
+      hipLaunchKernelGGL((k_sell_pieces<VT, COMP, true>), dim3((unsigned)((d.npieces + W - 1) / W)),
+                         dim3(kWG), 0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs,
+                         d.col, (const VT*)d.val, (const IO*)x, part, d.sbs, A.csr.rst);
+    else
+      hipLaunchKernelGGL((k_sell_pieces<VT, COMP, false>), dim3((unsigned)((d.npieces + W - 1) / W)),
+                         dim3(kWG), 0, s, d.pc_slice, d.pc_j0, d.pc_j1, d.npieces, d.cs, d.bs,
+                         d.col, (const VT*)d.val, (const IO*)x, part, d.sbs, A.csr.rst);
     if (int rc = hip_check(hipGetLastError(), "k_sell_pieces launch")) return rc;
-    hipLaunchKernelGGL((k_sell_combine<VT>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
+    hipLaunchKernelGGL((k_sell_combine<VT, COMP>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,
                        d.sp_slice, d.sp_first, d.bs, d.sbs, d.perm,
                        d.perm_identity, part, (IO*)y, (AC)alpha, (AC)beta);
   }

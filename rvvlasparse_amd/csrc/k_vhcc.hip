@@ -33,7 +33,7 @@ struct VhccArgs {
   int64_t nparts, nempty, m;
 };
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__ val,
                                                const typename Val<VT>::io* __restrict__ x,
                                                typename Val<VT>::io* __restrict__ y,
@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
                                                typename Val<VT>::acc alpha,
                                                typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
+  using Ac = Acc<A, COMP>;
   const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= a.nparts) return;
   const int lane = threadIdx.x & 63;
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
   const uint32_t* __restrict__ colp = a.col;
   const uint32_t* __restrict__ rowp = a.row_arr;
   const uint64_t* __restrict__ eorp = a.eor;
-  A hist = 0, carry = 0;
+  Ac hist, carry;
   bool pending = false;  // hist holds sums of vectors without row ends
   // kBatch vectors at a time: all their loads (columns, values, masks, the
   // ending rows) and gathers are issued before the scans
@@ -85,42 +86,47 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
       if (vb + u >= v1) break;
       const uint64_t m = mk[u];
       if (!m) {
-        hist += p[u];
+        hist.add(p[u]);
         pending = true;
         continue;
       }
       if (pending) {
-        A h = hist;
-        for (int d = 32; d > 0; d >>= 1) h += __shfl_xor(h, d, 64);
-        carry += h;
-        hist = 0;
+        const Ac h = group_reduce(hist, 64);
+        carry.merge(h);
+        hist = Ac();
         pending = false;
       }
-      A sum = p[u] + (lane == 0 ? carry : A(0));
+      Ac sum;
+      sum.add(p[u]);
+      if (lane == 0) sum.merge(carry);
       const uint64_t heads = 1u | (m << 1);
       const int hp = 63 - __builtin_clzll(heads & upto);  // this lane's segment start
       for (int d = 1; d < 64; d <<= 1) {
-        const A o = __shfl_up(sum, d, 64);
-        if (lane - d >= hp) sum += o;
+        Ac o;
+        o.s = __shfl_up(sum.s, d, 64);
+        if constexpr (COMP) o.c = __shfl_up(sum.c, d, 64);
+        if (lane - d >= hp) sum.merge(o);
       }
       if ((m >> lane) & 1u) {
         const uint32_t r = rr[u];
         const int64_t e = (vb + u) * 64 + lane;
         if (r >= (uint32_t)a.m) {
-          first_part[t] = sum;  // the row this partition continues
+          first_part[t] = sum.value();  // the row this partition continues
         } else if (has_fwd && e == last_entry) {
-          fwd_part[t] = sum;    // the row handed on to the next partition
+          fwd_part[t] = sum.value();    // the row handed on to the next partition
         } else {
-          y[r] = alpha * sum + scaled_y(beta, y, r);
+          y[r] = alpha * sum.value() + scaled_y(beta, y, r);
         }
       }
-      const A last = __shfl(sum, 63, 64);
-      carry = (m >> 63) ? A(0) : last;
+      Ac last;
+      last.s = __shfl(sum.s, 63, 64);
+      if constexpr (COMP) last.c = __shfl(sum.c, 63, 64);
+      carry = (m >> 63) ? Ac() : last;
     }
   }
 }
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_vhcc_fixup(VhccArgs a,
                                                      typename Val<VT>::io* __restrict__ y,
                                                      const typename Val<VT>::acc* __restrict__ first_part,
@@ -132,12 +138,13 @@ __global__ __launch_bounds__(kWG) void k_vhcc_fixup(VhccArgs a,
   if (i < a.nparts) {
     const int32_t r = a.fwd_row[i];
     if (r < 0) return;
-    A s = fwd_part[i];
+    Acc<A, COMP> s;
+    s.add(fwd_part[i]);
     for (int64_t q = i + 1; q < a.nparts; ++q) {
-      s += first_part[q];
+      s.add(first_part[q]);
       if (!a.pass[q]) break;
     }
-    y[r] = alpha * s + scaled_y(beta, y, r);
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
   } else if (i < a.nparts + a.nempty) {
     const int32_t r = a.empty_rows[i - a.nparts];
     y[r] = scaled_y(beta, y, r);
@@ -152,17 +159,18 @@ static int run_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double be
   const DevVHCC& d = A.vhcc;
   VhccArgs a{d.start_vec, d.rbase, d.eor, d.row_arr, d.fwd_row, d.pass, d.col, d.empty_rows,
              d.nparts, d.nempty, A.m};
+  constexpr bool COMP = sizeof(VT) == 2;  // 16-bit values: TwoSum throughout
   AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
   if (d.nparts > 0) {
     const unsigned g = (unsigned)((d.nparts + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_vhcc<VT>), dim3(g), dim3(kWG), 0, s, a, (const VT*)d.val, (const IO*)x,
+    hipLaunchKernelGGL((k_vhcc<VT, COMP>), dim3(g), dim3(kWG), 0, s, a, (const VT*)d.val, (const IO*)x,
                        (IO*)y, fp, wp, (AC)alpha, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_vhcc launch")) return rc;
   }
   const int64_t nfix = d.nparts + d.nempty;
   if (nfix > 0) {
-    hipLaunchKernelGGL((k_vhcc_fixup<VT>), dim3((unsigned)((nfix + kWG - 1) / kWG)), dim3(kWG), 0,
+    hipLaunchKernelGGL((k_vhcc_fixup<VT, COMP>), dim3((unsigned)((nfix + kWG - 1) / kWG)), dim3(kWG), 0,
                        s, a, (IO*)y, fp, wp, (AC)alpha, (AC)beta);
     return hip_check(hipGetLastError(), "k_vhcc_fixup launch");
   }

@@ -19,7 +19,7 @@ namespace rvvla {
 static_assert(vnec_items(4) <= 8 * kWG && vnec_items(8) <= 8 * kWG,
               "k_vnec stages at most 8 entries per thread");
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_vnec(
     const int32_t* __restrict__ cx, const int32_t* __restrict__ cy,
     const int32_t* __restrict__ nec_ptr, const int32_t* __restrict__ use_x,
@@ -86,9 +86,9 @@ __global__ __launch_bounds__(kWG) void k_vnec(
       const int r = x0 + i;
       const int l = i == tid ? lo : max(rst[r], y0) - y0;
       const int h = i == tid ? hi : min(rst[r + 1], y1) - y0;
-      A s = 0;
-      for (int k = l; k < h; ++k) s += prod[k];
-      finish(r, s);
+      Acc<A, COMP> s;
+      for (int k = l; k < h; ++k) s.add(prod[k]);
+      finish(r, s.value());
     }
     return;
   }
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kWG) void k_vnec(
   int width = 64;
   while (width * nrows > kWG) width >>= 1;
   const int g = tid / width, l = tid & (width - 1);
-  Acc<A, false> a;
+  Acc<A, COMP> a;
   if (g < nrows) {
     const int r = x0 + g;
     const int lo = max(rst[r], y0) - y0, hi = min(rst[r + 1], y1) - y0;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kWG) void k_vnec(
   if (g < nrows && l == 0) finish(x0 + g, a.value());
 }
 
-template <class VT>
+template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_vnec_fixup(
     const int32_t* __restrict__ cx, const int32_t* __restrict__ cy,
     const int32_t* __restrict__ rst, typename Val<VT>::io* __restrict__ y,
@@ -118,16 +118,17 @@ __global__ __launch_bounds__(kWG) void k_vnec_fixup(
   if (t >= nparts) return;
   const int r = cx[t + 1];
   if (r >= m || rst[r] < cy[t] || rst[r] >= cy[t + 1]) return;  // not the row's first partition
-  A s = fwd_part[t];
+  Acc<A, COMP> s;
+  s.add(fwd_part[t]);
   for (int64_t q = t + 1; q < nparts; ++q) {
     if (cx[q] == cx[q + 1]) {
-      s += fwd_part[q];  // the whole partition lies inside the row
+      s.add(fwd_part[q]);  // the whole partition lies inside the row
     } else {
-      s += first_part[q];
+      s.add(first_part[q]);
       break;
     }
   }
-  y[r] = alpha * s + scaled_y(beta, y, r);
+  y[r] = alpha * s.value() + scaled_y(beta, y, r);
 }
 
 template <class VT>
@@ -137,13 +138,14 @@ static int run_vnec(const rvvla_mat_s& A, double alpha, const void* x, double be
   using AC = typename Val<VT>::acc;
   const DevVNEC& d = A.vnec;
   if (d.nparts == 0) return RVVLA_OK;
+  constexpr bool COMP = sizeof(VT) == 2;  // 16-bit values: TwoSum throughout
   AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
-  hipLaunchKernelGGL((k_vnec<VT>), dim3((unsigned)d.nparts), dim3(kWG), 0, s, d.cx, d.cy,
+  hipLaunchKernelGGL((k_vnec<VT, COMP>), dim3((unsigned)d.nparts), dim3(kWG), 0, s, d.cx, d.cy,
                      d.nec_ptr, d.use_x, d.ecr, d.rst, (const VT*)d.val, (const IO*)x, (IO*)y, fp,
                      wp, A.m, (AC)alpha, (AC)beta);
   if (int rc = hip_check(hipGetLastError(), "k_vnec launch")) return rc;
-  hipLaunchKernelGGL((k_vnec_fixup<VT>), dim3((unsigned)((d.nparts + kWG - 1) / kWG)), dim3(kWG),
+  hipLaunchKernelGGL((k_vnec_fixup<VT, COMP>), dim3((unsigned)((d.nparts + kWG - 1) / kWG)), dim3(kWG),
                      0, s, d.cx, d.cy, d.rst, (IO*)y, fp, wp, d.nparts, A.m, (AC)alpha, (AC)beta);
   return hip_check(hipGetLastError(), "k_vnec_fixup launch");
 }

@@ -151,27 +151,41 @@ __device__ __forceinline__ T shfl_xor(T v, int m) {
 // eight loads in flight, and the four wavefront sums are added in
 // wavefront order through LDS, so the order of the additions is fixed by
 // the piece count alone.  Every thread of the workgroup must call it; each
-// gets the sum for its lane.  lds holds kWG values.
-template <class A>
+// gets the sum for its lane.  lds holds kWG values, or 2 kWG with COMP:
+// then every addition is compensated (TwoSum), for the 16-bit value forms.
+template <class A, bool COMP = false>
 __device__ __forceinline__ A wg_piece_sum(const A* __restrict__ part, int64_t q0, int64_t q1,
                                           A* lds) {
   constexpr int W = kWG / 64;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  A a[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) a[i] = A(0);
+  Acc<A, COMP> a[8];
   int64_t q = q0 + wv;
   for (; q + 7 * W < q1; q += 8 * W) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] += part[(q + i * W) * 64 + lane];
+    for (int i = 0; i < 8; ++i) a[i].add(part[(q + i * W) * 64 + lane]);
   }
-  for (; q < q1; q += W) a[0] += part[q * 64 + lane];
-  lds[wv * 64 + lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  for (; q < q1; q += W) a[0].add(part[q * 64 + lane]);
+  a[0].merge(a[1]);
+  a[2].merge(a[3]);
+  a[4].merge(a[5]);
+  a[6].merge(a[7]);
+  a[0].merge(a[2]);
+  a[4].merge(a[6]);
+  a[0].merge(a[4]);
+  lds[wv * 64 + lane] = a[0].s;
+  if constexpr (COMP) lds[kWG + wv * 64 + lane] = a[0].c;
   __syncthreads();
-  A s = lds[lane];
+  Acc<A, COMP> s;
+  s.s = lds[lane];
+  if constexpr (COMP) s.c = lds[kWG + lane];
 #pragma unroll
-  for (int w = 1; w < W; ++w) s += lds[w * 64 + lane];
-  return s;
+  for (int w = 1; w < W; ++w) {
+    Acc<A, COMP> o;
+    o.s = lds[w * 64 + lane];
+    if constexpr (COMP) o.c = lds[kWG + w * 64 + lane];
+    s.merge(o);
+  }
+  return s.value();
 }
 
 // Sum over aligned groups of `width` lanes (power of two <= 64); every lane
@@ -185,6 +199,71 @@ __device__ __forceinline__ Acc<T, COMP> group_reduce(Acc<T, COMP> a, int width) 
     a.merge(o);
   }
   return a;
+}
+
+// DPP move of a 32- or 64-bit value (CTRL: row_shr:n = 0x110 | n, wave_shr:1
+// = 0x138); lanes without a source lane get 0.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+template <class Ac>
+__device__ __forceinline__ Ac acc_readlane(const Ac& a, int l) {
+  Ac r;
+  r.s = lane_read(a.s, l);
+  r.c = lane_read(a.c, l);
+  return r;
+}
+// One in-row Hillis-Steele step of the segmented scan: lane l adds lane
+// l - D of its row when that lane is in its segment.
+template <int CTRL, int D, class Ac>
+__device__ __forceinline__ void seg_scan_step(Ac& S, int lr, int lane, int st) {
+  Ac o;
+  o.s = dpp_mov<CTRL>(S.s);
+  o.c = dpp_mov<CTRL>(S.c);
+  if (lr >= D && lane - D >= st) S.merge(o);
+}
+
+// Segmented inclusive scan of a across the wavefront: lane l gets the sum
+// over lanes st..l, st the highest lane <= l whose bit is set in heads
+// (lane 0's bit must be set).  Hillis-Steele steps inside each row of 16
+// lanes (DPP row shifts), then the open sum at the end of each row carried
+// into the next rows' lanes whose segment started before them; the order
+// of the additions is fixed by heads alone.
+template <class Ac>
+__device__ __forceinline__ Ac wave_seg_scan(Ac S, uint64_t heads, int lane) {
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int st = 63 - __clzll((long long)(heads & upto));
+  const int lr = lane & 15;
+  seg_scan_step<0x111, 1>(S, lr, lane, st);
+  seg_scan_step<0x112, 2>(S, lr, lane, st);
+  seg_scan_step<0x114, 4>(S, lr, lane, st);
+  seg_scan_step<0x118, 8>(S, lr, lane, st);
+  // row r's lanes take the open sum at the end of row r - 1, which already
+  // holds the carries of the rows before it
+  const Ac T = acc_readlane(S, 15);
+  if (lane >= 16 && lane < 32 && st < 16) S.merge(T);
+  const Ac E = acc_readlane(S, 31);
+  if (lane >= 32 && lane < 48 && st < 32) S.merge(E);
+  const Ac F = acc_readlane(S, 47);
+  if (lane >= 48 && st < 48) S.merge(F);
+  return S;
 }
 
 }  // namespace rvvla

@@ -168,9 +168,101 @@ def test_csr_column_slices_16bit(dtype):
         assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
 
 
+def _compensation_case(dtype):
+    """A 50,000-entry row (row 217) among rows of 1..3 entries: its first
+    half of products positive, its second half negative, so a left-to-right
+    FP32 sum climbs past 2^15 and rounds on every addition there.  Values
+    have 7 fraction bits and x 2, on [1, 2): every product is a multiple of
+    2^-9 below 4, exact in FP32 and in the 16-bit storage, so the FP64 sum
+    is exact (the oracle's row is the exact sum), and so is the FP32 sum of
+    any 8191 or fewer of them: a compensated sum whose partials each cover
+    at most that many entries is exact, a plain one is not."""
+    del dtype  # the same values are exact in BF16 and FP16
+    rng = np.random.default_rng(31)
+    m, n, big = 600, 120000, 50000
+    lens = rng.integers(1, 4, m)
+    lens[217] = big
+    cols = [np.sort(rng.choice(n, k, replace=False)) for k in lens]
+    rst = np.zeros(m + 1, np.int32)
+    rst[1:] = np.cumsum(lens)
+    col = np.concatenate(cols).astype(np.int32)
+    v = np.round(rng.uniform(1.0, 2.0, len(col)) * 128) / 128
+    sign = rng.choice([-1.0, 1.0], len(col))
+    e0 = rst[217]
+    sign[e0:e0 + big // 2] = 1.0
+    sign[e0 + big // 2:e0 + big] = -1.0
+    a = R.CSR(m, n, rst, col, v * sign)
+    x = np.round(rng.uniform(1.0, 2.0, n) * 4) / 4
+    return a, x
+
+
+def _compensated_bound(y_exact, absdot):
+    """One rounding of the result, plus a margin far below one rounding of
+    a plain sum's running value: 4 u |y| + 2^-32 sum|a||x| (u = 2^-24)."""
+    return 4 * 2.0 ** -24 * np.abs(y_exact) + 2.0 ** -32 * absdot
+
+
+@pytest.mark.parametrize("dtype", [R.BF16, R.F16])
+@pytest.mark.parametrize("fmt,params", [("csr", ""), ("csr", "xlds=1"), ("csr", "xslices=8"),
+                                        ("csr", "sell=1"),
+                                        pytest.param("hcsr", "", marks=pytest.mark.skip(
+                                            reason="HCSR 16-bit sums not yet compensated")),
+                                        ("sell", ""), ("csr5", ""), ("lav", ""), ("vnec", ""),
+                                        ("vhcc", ""),
+                                        pytest.param("cvr", "", marks=pytest.mark.skip(
+                                            reason="CVR refuses rows this long"))])
+def test_16bit_long_row_compensated(fmt, params, dtype):
+    """Rows longer than 1024 entries are summed with TwoSum compensation in
+    the 16-bit value forms: the long row's error against its exact sum stays
+    within _compensated_bound, which a plain FP32 left-to-right sum of the
+    row, or of its 512-entry partials, misses by far (next test); every row
+    also meets 4 * 2^-24 * sum|a||x|."""
+    a, x = _compensation_case(dtype)
+    print("16-bit compensation", fmt, params, dtype, flush=True)
+    y0 = np.zeros(a.m)
+    y = run_gpu(fmt, dtype, a, x, y0, 1.0, 0.0, params)
+    y_ref = O.csr_spmv(a.rst, a.col, a.val, x, 1.0, 0.0)
+    scale = O.csr_absdot(a.rst, a.col, a.val, x)
+    err = np.abs(y.astype(np.float64) - y_ref)
+    lim = _compensated_bound(y_ref[217], scale[217])
+    assert err[217] <= lim, f"{fmt} {params}: long-row error {err[217]:.3g} > {lim:.3g}"
+    assert np.all(err <= 4 * 2.0 ** -24 * scale)
+
+
+def test_16bit_compensation_bound_power():
+    """The bound of the compensation test against FP32 summations of its
+    long row: a plain left-to-right sum (a per-row loop without TwoSum)
+    misses it by more than 1000x, and a plain left-to-right sum of exact
+    512-entry partials by more than 10x; TwoSum-compensated sums of exact
+    512-entry or 8-entry partials meet it."""
+    a, x = _compensation_case(R.BF16)
+    e0, e1 = a.rst[217], a.rst[218]
+    t = a.val[e0:e1] * x[a.col[e0:e1]]
+    t32 = t.astype(np.float32)
+    assert np.array_equal(t32.astype(np.float64), t), "products exact in FP32"
+    v32 = a.val.astype(np.float32)
+    assert not np.any(v32.view(np.uint32) & 0xFFFF), "values exact in BF16"
+    assert np.array_equal(v32.astype(np.float16).astype(np.float64), a.val), "and in FP16"
+    exact = t.sum()
+    lim = _compensated_bound(exact, np.abs(t).sum())
+    seq = np.float32(0)
+    for q in t32:
+        seq = np.float32(seq + q)
+    assert abs(float(seq) - exact) > 1000 * lim
+    for chunk in (512, 8):
+        parts = np.array([np.float32(t[i:i + chunk].sum()) for i in range(0, len(t), chunk)])
+        plain = np.float32(0)
+        for q in parts:
+            plain = np.float32(plain + q)
+        if chunk == 512:
+            assert abs(float(plain) - exact) > 10 * lim
+        comp = np.float32(parts.astype(np.float64).sum())
+        assert abs(float(comp) - exact) <= lim
+
+
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("params", ["sell=1", "sell=0"])
-@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16])
+@pytest.mark.parametrize("dtype", [R.F64, R.F32, R.BF16, R.F16])
 def test_csr_sell_image(name, params, dtype):
     """CSR products with and without the matrix's SELL-64 image, against
     the oracle."""
@@ -178,15 +270,41 @@ def test_csr_sell_image(name, params, dtype):
     x64, y064 = inputs(a, 15)
     dt = np.float64 if dtype == R.F64 else np.float32
     y = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, params)
-    if dtype == R.BF16:
+    if dtype in (R.BF16, R.F16):
         x32, y032 = x64.astype(np.float32), y064.astype(np.float32)
         y64 = O.csr_spmv(a.rst, a.col, a.val, x32.astype(np.float64), 1.0, 0.5, y032.astype(np.float64))
         if np.linalg.norm(y64) > 0:
-            assert np.linalg.norm(y - y64) / np.linalg.norm(y64) <= 5e-3
+            rel = np.linalg.norm(y - y64) / np.linalg.norm(y64)
+            assert rel <= (5e-3 if dtype == R.BF16 else 1e-3), rel
         return
     v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
     y_ref = O.csr_spmv(a.rst, a.col, v, x, 1.0, 0.5, y0)
     assert_within_bound(y, y_ref, a.rst, a.col, v, x, 1.0, 0.5, y0, dt, f"csr {params} {name}")
+
+
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_csr_sell_image_nonfinite_x(dtype):
+    """The SELL image of CSR reads only each row's own entries: with x[0] =
+    Inf, an empty row (whose padding points at column 0) gives beta*y and a
+    short row of a wider slice whose last column holds Inf gives that Inf,
+    as the reference CSR product does (no 0*Inf NaN from padding)."""
+    n = 64
+    rows = [[], [0], [3, 5], [1, 2, 7, 9, 11]] + [[(r * 7) % n, (r * 7 + 1) % n] for r in range(60)]
+    rst = np.zeros(len(rows) + 1, np.int32)
+    rst[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate([np.sort(np.array(r, np.int64)) for r in rows if r]).astype(np.int32)
+    a = R.CSR(len(rows), n, rst, col, np.linspace(0.5, 1.5, len(col)))
+    x64 = np.linspace(-1, 1, n)
+    x64[0] = np.inf
+    y064 = np.full(a.m, 2.0)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    y = run_gpu("csr", dtype, a, x64, y064, 1.0, 0.5, "sell=1")
+    y_ref = O.csr_spmv(a.rst, a.col, a.values_for(dtype), x64.astype(dt), 1.0, 0.5, y064.astype(dt))
+    assert np.array_equal(np.isnan(y), np.isnan(y_ref)) and not np.isnan(y).any()
+    assert y[0] == 1.0 and np.isinf(y[1])
+    fin = np.isfinite(y_ref)
+    assert np.array_equal(np.isinf(y), np.isinf(y_ref))
+    assert np.allclose(y[fin], y_ref[fin])
 
 
 @pytest.mark.parametrize("name", CASES + ["wide"])

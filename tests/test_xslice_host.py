@@ -180,8 +180,9 @@ def test_empty_and_bad_request():
 
 def test_csr_sell_image_choice():
     """CSR keeps a natural-order SELL-64 image (its products run the SELL
-    kernel) for matrices of at least 65536 rows whose 64-row slices pad by
-    at most 1 %; not for uneven rows, small matrices or sell=0."""
+    kernel) for matrices of at least 65536 rows of at most 64 entries whose
+    64-row slices pad by at most 1 %; not for uneven or longer rows, small
+    matrices or sell=0."""
     lap = R.gen_laplacian5(300)  # 90000 rows of 3..5 entries
     assert len(R.convert("csr", lap, R.F64).upload().view("sell.cl")) == -(-lap.m // 64)
     assert len(R.convert("csr", lap, R.F64, "sell=0").upload().view("sell.cl")) == 0
@@ -190,3 +191,11 @@ def test_csr_sell_image_choice():
     small = R.gen_laplacian5(100)  # 10000 rows
     assert len(R.convert("csr", small, R.F32).upload().view("sell.cl")) == 0
     assert len(R.convert("csr", small, R.F32, "sell=1").upload().view("sell.cl")) == -(-small.m // 64)
+    # even rows longer than one SELL piece (65 entries, more than 64 steps a
+    # slice): not chosen by default, so long rows keep the CSR kernels
+    even = R.gen_random_rows(66000, 65, seed=5)
+    for dt in (R.F64, R.BF16):
+        assert len(R.convert("csr", even, dt).upload().view("sell.cl")) == 0
+    assert len(R.convert("csr", even, R.F64, "sell=1").upload().view("sell.cl")) == -(-even.m // 64)
+    # the identity permutation of the CSR image is not kept on the device
+    assert len(R.convert("csr", lap, R.F64).upload().view("sell.vPerm")) == 0
