@@ -104,6 +104,19 @@ uint16_t f32_to_f16(float f) {
   return u;
 }
 
+float bf16_to_f32(uint16_t b) {
+  const uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+float f16_to_f32(uint16_t b) {
+  _Float16 h;
+  std::memcpy(&h, &b, 2);
+  return (float)h;
+}
+
 }  // namespace rvvla
 
 using rvvla::Error;
@@ -368,15 +381,69 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, i
   return M.release();
 }
 
+// Uploads the image of hcsr_image.cpp (from the validated layout) into
+// M->hcsr; values as V (double for F64, float otherwise) narrowed for
+// BF16/F16.
+template <class V>
+void put_hcsr_image(rvvla_mat_s& M, int dtype, const HcsrImage<V>& im) {
+  DevHCSR& d = M.hcsr;
+  d.nunits = im.nunits;
+  d.ndrows = (int64_t)im.dcol.size() / 64;
+  d.ntail = (int64_t)im.tcol.size();
+  d.nsplit = im.nsplit;
+  d.nparts = im.nparts;
+  d.unit = M.put_vec(im.unit);
+  d.dxoff = M.put_vec(im.dxoff);
+  d.dcol = M.put_vec(im.dcol);
+  d.dval = put_values(M, dtype, im.dval.data(), im.dval.size());
+  d.tcol = M.put_vec(im.tcol);
+  d.trow = M.put_vec(im.trow);
+  d.tval = put_values(M, dtype, im.tval.data(), im.tval.size());
+  d.split = M.put_vec(im.split);
+  const size_t pb = (size_t)im.nparts * 64 * (dtype == RVVLA_F64 ? 8 : 4);
+  d.part = pb ? M.put(nullptr, pb) : nullptr;
+}
+
+// The image of a layout built on the device (rvvla_hcsr_from_device): the
+// layout is copied to the host (16-bit values widened exactly) and laid
+// out by build_hcsr_image.
+template <class V>
+void hcsr_image_from_device(rvvla_mat_s& M, hipStream_t s) {
+  DevHCSR& d = M.hcsr;
+  const int64_t nb = d.b_m * d.b_n + 1, nz = M.nnz;
+  std::vector<int32_t> bp((size_t)nb);
+  std::vector<uint32_t> ro((size_t)nb), rp((size_t)std::max<int64_t>(d.nref_row_ptr, 1));
+  std::vector<uint16_t> ci((size_t)std::max<int64_t>(nz, 1));
+  std::vector<V> val((size_t)std::max<int64_t>(nz, 1));
+  const int vb = M.dtype == RVVLA_F64 ? 8 : M.dtype == RVVLA_F32 ? 4 : 2;
+  std::vector<uint16_t> v16(vb == 2 ? (size_t)std::max<int64_t>(nz, 1) : 0);
+  auto get = [&](void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    if (int rc = hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "copy back"))
+      throw Error(rc, last_error());
+  };
+  get(bp.data(), d.ref_block_ptr, (size_t)nb * 4);
+  get(ro.data(), d.ref_row_offset, (size_t)nb * 4);
+  get(rp.data(), d.ref_row_ptr, (size_t)d.nref_row_ptr * 4);
+  get(ci.data(), d.ref_col_idx, (size_t)nz * 2);
+  get(vb == 2 ? (void*)v16.data() : (void*)val.data(), d.ref_values, (size_t)nz * vb);
+  if (int rc = hip_check(hipStreamSynchronize(s), "copy back")) throw Error(rc, last_error());
+  if (vb == 2)
+    for (int64_t i = 0; i < nz; ++i)
+      val[(size_t)i] = (V)(M.dtype == RVVLA_BF16 ? bf16_to_f32(v16[(size_t)i]) : f16_to_f32(v16[(size_t)i]));
+  HcsrImage<V> im;
+  build_hcsr_image(im, M.m, M.n, d.R, d.C, bp.data(), ro.data(), rp.data(), ci.data(), val.data());
+  put_hcsr_image(M, M.dtype, im);
+}
+template void hcsr_image_from_device<double>(rvvla_mat_s&, hipStream_t);
+template void hcsr_image_from_device<float>(rvvla_mat_s&, hipStream_t);
+
 template <class V>
 rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   const int64_t m = (int64_t)a.m, n = (int64_t)a.n, nz = (int64_t)a.nz;
   const int64_t R = (int64_t)a.R, C = (int64_t)a.C;
   if (R <= 0 || C <= 0 || C > 65536) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: bad R or C");
-  const size_t acc_bytes = dtype == RVVLA_F64 ? 8 : 4;
-  if ((size_t)R * acc_bytes > 65536)
-    throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "HCSR: R * accumulator size must fit 64 KiB of LDS");
   if (nz > 2147483647LL || m > 2147483647LL) throw Error(RVVLA_ERR_LENGTH, "HCSR: too large");
   const int64_t bm = (m + R - 1) / R, bn = (n + C - 1) / C;
   if ((int64_t)a.b_m != bm || (int64_t)a.b_n != bn)
@@ -387,9 +454,6 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
     throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: array sizes do not match the block grid");
   if (a.block_ptr[0] != 0 || a.block_ptr[nblocks] != nz)
     throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: block_ptr must run from 0 to nz");
-  std::vector<int32_t> brow_ptr((size_t)bm + 1, 0), nb_col, nb_ptr;
-  std::vector<uint32_t> nb_roff, rp32;
-  std::vector<uint16_t> coo16((size_t)nz, 0);
   for (int64_t br = 0; br < bm; ++br) {
     const int64_t rows = std::min<int64_t>(R, m - br * R);
     for (int64_t bc = 0; bc < bn; ++bc) {
@@ -403,11 +467,7 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
           throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: col_idx outside its block");
       if (kind == 0) {
         if (e1 != e0) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: empty-kind block holds entries");
-        continue;
-      }
-      nb_col.push_back((int32_t)bc);
-      nb_ptr.push_back((int32_t)e0);
-      if (kind == 1) {
+      } else if (kind == 1) {
         if ((size_t)off + (size_t)rows + 1 > a.row_ptr.size())
           throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: row_ptr too short");
         const uint32_t* rp = a.row_ptr.data() + off;
@@ -415,126 +475,27 @@ rvvla_mat_t upload_hcsr(int dtype, const spMtxHCSR<V>& a) {
           throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: local row_ptr does not span the block");
         for (int64_t r = 0; r < rows; ++r)
           if (rp[r + 1] < rp[r]) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: local row_ptr");
-        nb_roff.push_back((1u << 30) | (uint32_t)rp32.size());
-        rp32.insert(rp32.end(), rp, rp + rows + 1);
-        if (rp32.size() > 0x3FFFFFFFu) throw Error(RVVLA_ERR_LENGTH, "HCSR: rp32 overflow");
       } else if (kind == 2) {
         if ((size_t)off + (size_t)(e1 - e0) > a.row_ptr.size())
           throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: row_ptr too short");
-        for (int64_t e = e0; e < e1; ++e) {
-          const uint32_t lr = a.row_ptr[off + (size_t)(e - e0)];
-          if ((int64_t)lr >= rows) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: COO row out of block");
-          coo16[(size_t)e] = (uint16_t)lr;
-        }
-        nb_roff.push_back(2u << 30);
+        for (int64_t e = e0; e < e1; ++e)
+          if ((int64_t)a.row_ptr[off + (size_t)(e - e0)] >= rows)
+            throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: COO row out of block");
       } else {
         throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: unknown block kind");
       }
     }
-    brow_ptr[(size_t)br + 1] = (int32_t)nb_col.size();
   }
-  nb_ptr.push_back((int32_t)nz);
-  // block-rows with many blocks: per-entry local rows and block columns
-  std::vector<uint8_t> flat((size_t)bm, 0);
-  std::vector<uint16_t> ebc16;
-  bool any_flat = false;
-  if (bn <= 65536) {
-    for (int64_t br = 0; br < bm; ++br) {
-      const int32_t k0 = brow_ptr[(size_t)br], k1 = brow_ptr[(size_t)br + 1];
-      if (k1 - k0 <= kHcsrFlat) continue;
-      flat[(size_t)br] = 1;
-      if (!any_flat) ebc16.assign((size_t)nz, 0);
-      any_flat = true;
-      for (int32_t k = k0; k < k1; ++k) {
-        const int32_t e0 = nb_ptr[(size_t)k], e1 = nb_ptr[(size_t)k + 1];
-        for (int32_t e = e0; e < e1; ++e) ebc16[(size_t)e] = (uint16_t)nb_col[(size_t)k];
-        if ((nb_roff[(size_t)k] >> 30) == 1) {  // CSR kind: expand the local row pointers
-          const uint32_t* rp = rp32.data() + (nb_roff[(size_t)k] & 0x3FFFFFFFu);
-          const int64_t rows = std::min<int64_t>(R, m - br * R);
-          for (int64_t r = 0; r < rows; ++r)
-            for (uint32_t e = rp[r]; e < rp[r + 1]; ++e) coo16[(size_t)(e0 + e)] = (uint16_t)r;
-        }
-      }
-    }
-  }
-  // the other block-rows: units of kWG local rows; COO-kind blocks there
-  // get their entries ordered by local row (a stable reorder of the block,
-  // only where the caller's order differs) and the entry offset of every
-  // unit boundary
-  const int64_t S = (R + kWG - 1) / kWG;
-  std::vector<int32_t> u_br, u_lr0, csplit, flat_rows;
-  std::vector<uint16_t> col16;  // reordered copies, made on first need
-  std::vector<V> vals;
-  for (int64_t br = 0; br < bm; ++br) {
-    if (flat[(size_t)br]) {
-      flat_rows.push_back((int32_t)br);
-      continue;
-    }
-    const int64_t rows = std::min<int64_t>(R, m - br * R);
-    for (int64_t lr = 0; lr < rows; lr += kWG) {
-      u_br.push_back((int32_t)br);
-      u_lr0.push_back((int32_t)lr);
-    }
-    for (int32_t k = brow_ptr[(size_t)br]; k < brow_ptr[(size_t)br + 1]; ++k) {
-      if ((nb_roff[(size_t)k] >> 30) != 2) continue;
-      const int32_t e0 = nb_ptr[(size_t)k], e1 = nb_ptr[(size_t)k + 1];
-      if (!std::is_sorted(coo16.begin() + e0, coo16.begin() + e1)) {
-        if (col16.empty()) {
-          col16.assign(a.col_idx.begin(), a.col_idx.end());
-          vals.assign(a.values.begin(), a.values.end());
-        }
-        std::vector<int32_t> ord((size_t)(e1 - e0));
-        for (int32_t i = 0; i < e1 - e0; ++i) ord[(size_t)i] = e0 + i;
-        std::stable_sort(ord.begin(), ord.end(),
-                         [&](int32_t p, int32_t q) { return coo16[(size_t)p] < coo16[(size_t)q]; });
-        std::vector<uint16_t> r2, c2;
-        std::vector<V> v2;
-        for (int32_t e : ord) {
-          r2.push_back(coo16[(size_t)e]);
-          c2.push_back(col16[(size_t)e]);
-          v2.push_back(vals[(size_t)e]);
-        }
-        std::copy(r2.begin(), r2.end(), coo16.begin() + e0);
-        std::copy(c2.begin(), c2.end(), col16.begin() + e0);
-        std::copy(v2.begin(), v2.end(), vals.begin() + e0);
-      }
-      if (csplit.size() + (size_t)S + 1 > 0x3FFFFFFFu) throw Error(RVVLA_ERR_LENGTH, "HCSR: csplit overflow");
-      nb_roff[(size_t)k] = (2u << 30) | (uint32_t)csplit.size();
-      int32_t e = e0;
-      for (int64_t s = 0; s <= S; ++s) {
-        while (e < e1 && (int64_t)coo16[(size_t)e] < s * kWG) ++e;
-        csplit.push_back(e - e0);
-      }
-    }
-  }
+  HcsrImage<V> im;
+  build_hcsr_image(im, m, n, R, C, a.block_ptr.data(), a.row_offset.data(), a.row_ptr.data(),
+                   a.col_idx.data(), a.values.data());
   auto M = new_mat(RVVLA_FMT_HCSR, dtype, m, n, nz);
   DevHCSR& d = M->hcsr;
-  if (any_flat) {
-    d.flat = M->put_vec(flat);
-    d.ebc16 = M->put_vec(ebc16);
-    d.nflat = (int64_t)flat_rows.size();
-    d.flat_rows = M->put_vec(flat_rows);
-  }
-  d.nunits = (int64_t)u_br.size();
-  d.nblk = (int64_t)nb_col.size();
-  d.nrp32 = (int64_t)rp32.size();
-  d.ncsplit = (int64_t)csplit.size();
-  d.u_br = M->put_vec(u_br);
-  d.u_lr0 = M->put_vec(u_lr0);
-  d.csplit = M->put_vec(csplit);
   d.R = R;
   d.C = C;
   d.b_m = bm;
   d.b_n = bn;
-  d.brow_ptr = M->put_vec(brow_ptr);
-  d.nb_col = M->put_vec(nb_col);
-  d.nb_ptr = M->put_vec(nb_ptr);
-  d.nb_roff = M->put_vec(nb_roff);
-  d.rp32 = M->put_vec(rp32);
-  d.coo16 = M->put_vec(coo16);
-  d.col16 = M->put_vec(col16.empty() ? a.col_idx : col16);
-  d.val = vals.empty() ? put_values(*M, dtype, a.values.data(), a.values.size())
-                       : put_values(*M, dtype, vals.data(), vals.size());
+  put_hcsr_image(*M, dtype, im);
   return M.release();
 }
 
@@ -1683,8 +1644,6 @@ int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
     if (C <= 0) C = dflt;
     if (!(p > 0.0)) p = 2.0;
     if (C > 65536) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: C must not exceed 65536 (16-bit col_idx)");
-    if (R * (dtype == RVVLA_F64 ? 8 : 4) > 65536)
-      throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "HCSR: R * accumulator size must fit 64 KiB of LDS");
     const int64_t bm = (m + R - 1) / R, bn = (n + C - 1) / C;
     if (bn != 0 && bm > (int64_t(2147483646) / bn))
       throw Error(RVVLA_ERR_LENGTH, "HCSR: more than 2^31 - 2 blocks");
@@ -1697,6 +1656,10 @@ int rvvla_hcsr_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const i
         ? rvvla::build_hcsr_device<double>(*M, m, n, nnz, d_rst, d_col, (const double*)d_val, R, C, p, s)
         : rvvla::build_hcsr_device<float>(*M, m, n, nnz, d_rst, d_col, (const float*)d_val, R, C, p, s);
     if (rc) throw Error(rc, rvvla::last_error());
+    // the kernels' image: the layout copied back and laid out by the host
+    // image builder
+    if (dtype == RVVLA_F64) rvvla::hcsr_image_from_device<double>(*M, s);
+    else rvvla::hcsr_image_from_device<float>(*M, s);
     *out = M.release();
     return RVVLA_OK;
   });
@@ -1787,8 +1750,8 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
     const rvvla::DevHCSR& h = A->hcsr;
     const int64_t nb = h.b_m * h.b_n + 1;
     v = f == "block_ptr" ? V{h.ref_block_ptr, nb, 4} : f == "row_offset" ? V{h.ref_row_offset, nb, 4}
-      : f == "row_ptr" ? V{h.rp32, h.nrp32, 4} : f == "col_idx" ? V{h.col16, A->nnz, 2}
-      : f == "values" ? V{h.val, A->nnz, vb} : V{nullptr, -1, 0};
+      : f == "row_ptr" ? V{h.ref_row_ptr, h.nref_row_ptr, 4} : f == "col_idx" ? V{h.ref_col_idx, A->nnz, 2}
+      : f == "values" ? V{h.ref_values, A->nnz, vb} : V{nullptr, -1, 0};
   }
   if (v.n < 0) {
     rvvla::set_last_error("rvvla_mat_read: unknown field " + f + " for this format");
@@ -1831,20 +1794,15 @@ int rvvla_mat_view(rvvla_mat_t A, const char* field, const void** data, int64_t*
   const std::string f = field;
   struct V { const void* p; int64_t n; int e; };
   const bool hc = A->format == RVVLA_FMT_HCSR;
-  const V v = hc ? (f == "brow_ptr"    ? V{h.brow_ptr, h.b_m + 1, 4}
-                  : f == "nb_col"    ? V{h.nb_col, h.nblk, 4}
-                  : f == "nb_ptr"    ? V{h.nb_ptr, h.nblk + 1, 4}
-                  : f == "nb_roff"   ? V{h.nb_roff, h.nblk, 4}
-                  : f == "rp32"      ? V{h.rp32, h.nrp32, 4}
-                  : f == "coo16"     ? V{h.coo16, A->nnz, 2}
-                  : f == "col16"     ? V{h.col16, A->nnz, 2}
-                  : f == "val"       ? V{h.val, A->nnz, vb}
-                  : f == "ebc16"     ? V{h.ebc16, h.nflat ? A->nnz : 0, 2}
-                  : f == "flat_rows" ? V{h.flat_rows, h.nflat, 4}
-                  : f == "u_br"      ? V{h.u_br, h.nunits, 4}
-                  : f == "u_lr0"     ? V{h.u_lr0, h.nunits, 4}
-                  : f == "csplit"    ? V{h.csplit, h.ncsplit, 4}
-                                     : V{nullptr, -1, 0})
+  const V v = hc ? (f == "hc.unit"    ? V{h.unit, h.nunits * rvvla::kHcUnitInts, 4}
+                  : f == "hc.dxoff" ? V{h.dxoff, h.ndrows, 4}
+                  : f == "hc.dcol"  ? V{h.dcol, h.ndrows * 64, 2}
+                  : f == "hc.dval"  ? V{h.dval, h.ndrows * 64, vb}
+                  : f == "hc.tcol"  ? V{h.tcol, h.ntail, 4}
+                  : f == "hc.trow"  ? V{h.trow, h.ntail, 1}
+                  : f == "hc.tval"  ? V{h.tval, h.ntail, vb}
+                  : f == "hc.split" ? V{h.split, h.nsplit * 4, 4}
+                                    : V{nullptr, -1, 0})
             : f == "rst"     ? V{c.rst, A->m + 1, 4}
             : f == "col"     ? V{c.col, A->nnz, 4}
             : f == "blocks"  ? V{c.blocks, nblk, 4}
@@ -1916,6 +1874,7 @@ int64_t spmv_scratch_bytes(const rvvla_mat_s& A) {
       if (A.csr.xs.P > 0) return A.csr.xs.pairs * acc;
       return 0;
     case RVVLA_FMT_SELL: return sell_part(A.sell);
+    case RVVLA_FMT_HCSR: return A.hcsr.nparts * 64 * acc;
     case RVVLA_FMT_LAV: return A.lav.nsplit > 0 ? A.lav.npieces * 64 * acc : 0;
     case RVVLA_FMT_CSR5: return A.csr5.ntiles * (2 * acc + 4) + 16;
     case RVVLA_FMT_VHCC: return A.vhcc.nparts * 2 * acc + 16;
@@ -1930,7 +1889,7 @@ int spmv_dispatch(const rvvla_mat_s& A, double alpha, const void* x, double beta
     case RVVLA_FMT_CSR:
       return A.csr.sell ? launch_sell(A, alpha, x, beta, y, s, scratch)
                         : launch_csr(A, alpha, x, beta, y, s, scratch);
-    case RVVLA_FMT_HCSR: return launch_hcsr(A, alpha, x, beta, y, s);
+    case RVVLA_FMT_HCSR: return launch_hcsr(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_SELL: return launch_sell(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_CSR5: return launch_csr5(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_CVR: return launch_cvr(A, alpha, x, beta, y, s);

@@ -23,4 +23,10 @@ constexpr int xl_slice_cols(int io_bytes) { return kXlLdsBytes / io_bytes; }
 constexpr int kXlCombRows = 2048;
 constexpr int xl_comb_pieces(int acc_bytes) { return 65536 / acc_bytes; }
 
+// HCSR image (hcsr_image.cpp): ints per unit record (row0, nrows, first
+// and end slot row, first and end tail entry, partial slot or -1, 0) and
+// the most work rows (64 slots each) one unit takes.
+constexpr int kHcUnitInts = 8;
+constexpr int kHcUnitRows = 64;
+
 }  // namespace rvvla

@@ -63,40 +63,30 @@ struct DevCSR {
   bool sell = false;  // products run on the matrix's SELL-64 image (rvvla_mat_s::sell)
 };
 
-// HCSR: non-empty blocks only, grouped by block-row.
+// HCSR: one wavefront per unit of work of a 64-row slice of a block-row
+// (hcsr_image.cpp): dense runs of 16-bit in-block columns laid 64 lanes
+// wide, and a row-ordered tail of (column, lane, value) entries.
 struct DevHCSR {
   int64_t R = 0, C = 0, b_m = 0, b_n = 0;
-  const int32_t* brow_ptr = nullptr;  // b_m+1 offsets into the non-empty block list
-  const int32_t* nb_col = nullptr;    // block column of each non-empty block
-  const int32_t* nb_ptr = nullptr;    // first entry of each non-empty block (+ sentinel nnz)
-  const uint32_t* nb_roff = nullptr;  // kind << 30 | offset into rp32 (CSR kind)
-  const uint32_t* rp32 = nullptr;     // local row offsets of CSR-kind blocks
-  const uint16_t* coo16 = nullptr;    // local row of each COO-kind entry (nnz-sized)
-  const uint16_t* col16 = nullptr;
-  const void* val = nullptr;
-  // block-rows with more than kHcsrFlat non-empty blocks are walked entry
-  // by entry: coo16 then also holds the local row of their CSR-kind
-  // entries and ebc16 the block column of every entry
-  const uint8_t* flat = nullptr;      // b_m flags (nullptr: none)
-  const uint16_t* ebc16 = nullptr;
-  int64_t nflat = 0;
-  const int32_t* flat_rows = nullptr;  // the flat block-rows
-  // the other block-rows run as units of kWG local rows (one per thread):
-  // unit u is local rows u_lr0[u] .. +kWG of block-row u_br[u]; the entries
-  // of a COO-kind block there are ordered by local row, and csplit (at the
-  // block's offset field) holds the entry offset of each unit boundary
-  int64_t nunits = 0;
-  int64_t nblk = 0, nrp32 = 0, ncsplit = 0;  // sizes of nb_*, rp32, csplit
-  // built on the device (rvvla_hcsr_from_device): the reference layout's
-  // block_ptr and row_offset over all b_m * b_n blocks (rp32 is then the
-  // reference row_ptr, col16 its col_idx, val its values)
+  int64_t nunits = 0, ndrows = 0, ntail = 0, nsplit = 0, nparts = 0;
+  const int32_t* unit = nullptr;   // kHcUnitInts per unit
+  const int32_t* dxoff = nullptr;  // first column of the block of each slot row
+  const uint16_t* dcol = nullptr;  // 64 * ndrows in-block columns (0xFFFF: empty slot)
+  const void* dval = nullptr;
+  const int32_t* tcol = nullptr;   // ntail tail columns
+  const uint8_t* trow = nullptr;   // their lanes (rows inside the slice)
+  const void* tval = nullptr;
+  const int32_t* split = nullptr;  // per slice cut into units: row0, nrows, first, end part
+  void* part = nullptr;            // 64 * nparts partial sums
+  // built on the device (rvvla_hcsr_from_device): the reference layout,
+  // kept for reading back
   const int32_t* ref_block_ptr = nullptr;
   const uint32_t* ref_row_offset = nullptr;
-  const int32_t* u_br = nullptr;
-  const int32_t* u_lr0 = nullptr;
-  const int32_t* csplit = nullptr;
+  const uint32_t* ref_row_ptr = nullptr;
+  const uint16_t* ref_col_idx = nullptr;
+  const void* ref_values = nullptr;
+  int64_t nref_row_ptr = 0;
 };
-constexpr int kHcsrFlat = 32;
 
 // SELL-C-sigma with C = 64: one wavefront per slice.
 struct DevSELL {
@@ -405,7 +395,7 @@ int spmv_check_args(rvvla_mat_s* A, const void* x, void* y);
 int launch_csr_spmm(const rvvla_mat_s& A, int k, double alpha, const void* X, int64_t ldx,
                     double beta, void* Y, int64_t ldy, hipStream_t s);
 int launch_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                hipStream_t s);
+                hipStream_t s, void* scratch = nullptr);
 int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s, void* scratch = nullptr);
 int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,

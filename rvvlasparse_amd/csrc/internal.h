@@ -73,6 +73,21 @@ constexpr int64_t kVhccPartEntries = 1024;
 // Rows [b[t], b[t+1]) of part t: contiguous, about nnz/parts entries each.
 std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts);
 
+// The device image of an HCSR matrix (hcsr_image.cpp) from the validated
+// reference layout.
+template <class V>
+struct HcsrImage {
+  std::vector<int32_t> unit, split, dxoff, tcol;
+  std::vector<uint16_t> dcol;
+  std::vector<uint8_t> trow;
+  std::vector<V> dval, tval;
+  int64_t nunits = 0, nsplit = 0, nparts = 0;
+};
+template <class V>
+void build_hcsr_image(HcsrImage<V>& im, int64_t m, int64_t n, int64_t R, int64_t C,
+                      const int32_t* block_ptr, const uint32_t* row_offset,
+                      const uint32_t* row_ptr, const uint16_t* col_idx, const V* values);
+
 // ---- device matrices ----------------------------------------------------
 // Upload of each format's host container; value arrays are given as V
 // (double for F64, float otherwise) and narrowed for BF16/F16.
@@ -149,5 +164,7 @@ double far_fraction(int64_t m, int64_t n, const int* rst, const int* col, int64_
 // 16-bit value encodings (round to nearest even).
 uint16_t f32_to_bf16(float f);
 uint16_t f32_to_f16(float f);
+float bf16_to_f32(uint16_t b);
+float f16_to_f32(uint16_t b);
 
 }  // namespace rvvla

@@ -17,11 +17,8 @@
 //     (COO kind) its row_ptr slot; one thread per row_ptr slot of a CSR-kind
 //     block writes the count of the block's entries in earlier local rows
 //     (binary search over the block's local rows, which ascend);
-//  4. the kernels' image from that layout: the non-empty block list and its
-//     per-block-row offsets (a scan of non-empty flags), the entry-by-entry
-//     block-rows (more than kHcsrFlat non-empty blocks) and the 256-row
-//     units of the others, with the entry offset of every unit boundary in
-//     their COO-kind blocks (binary search again).
+//  4. the kernels' image is then built from that layout by the host image
+//     builder (hcsr_image.cpp) after a copy back (rvvla_hcsr_from_device).
 // The reference layout (block_ptr, row_offset, row_ptr, col_idx, values)
 // is bit-identical to the host converter's for the same R, C, p.
 #include <rocprim/device/device_radix_sort.hpp>
@@ -97,8 +94,7 @@ __global__ void k_hc_row_offset(const uint8_t* __restrict__ kind, const int64_t*
   ro[b] = b < nblocks ? ((uint32_t)kind[b] << 30) | (uint32_t)off[b] : (uint32_t)total;
 }
 
-// sorted entry i: col_idx, value, local row, (COO kind) its row_ptr slot,
-// and (when some block-row is walked entry by entry) its block column
+// sorted entry i: col_idx, value, local row and (COO kind) its row_ptr slot
 template <class V, class S, int NARROW>
 __global__ void k_hc_scatter(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sidx,
                              const int32_t* __restrict__ erow, const int32_t* __restrict__ col,
@@ -106,7 +102,7 @@ __global__ void k_hc_scatter(const uint32_t* __restrict__ skey, const uint32_t* 
                              const uint8_t* __restrict__ kind, const int64_t* __restrict__ off,
                              int64_t nnz, int64_t bn, int64_t R, int64_t C,
                              uint16_t* __restrict__ col16, S* __restrict__ sval,
-                             uint16_t* __restrict__ lrow, uint16_t* __restrict__ ebc16,
+                             uint16_t* __restrict__ lrow,
                              uint32_t* __restrict__ row_ptr) {
   const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
   if (i >= nnz) return;
@@ -119,7 +115,6 @@ __global__ void k_hc_scatter(const uint32_t* __restrict__ skey, const uint32_t* 
   else sval[i] = v;
   const uint32_t lr = (uint32_t)(erow[src] - br * R);
   lrow[i] = (uint16_t)lr;
-  if (ebc16) ebc16[i] = (uint16_t)bc;
   if (kind[b] == 2) row_ptr[off[b] + (i - bp[b])] = lr;
 }
 
@@ -134,81 +129,6 @@ __global__ void k_hc_csr_rowptr(const int32_t* __restrict__ bp, const uint8_t* _
   if (kind[b] != 1) return;
   const int64_t lr = q - off[b];
   row_ptr[q] = (uint32_t)(dev_lower_bound(lrow, bp[b], bp[b + 1], (uint64_t)lr) - bp[b]);
-}
-
-__global__ void k_hc_nonempty(const int32_t* __restrict__ bp, int64_t nblocks,
-                              int64_t* __restrict__ flag) {
-  const int64_t b = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (b < nblocks) flag[b] = bp[b + 1] > bp[b] ? 1 : 0;
-}
-
-// block-row offsets into the non-empty block list, and which block-rows
-// are walked entry by entry (more than kHcsrFlat blocks, b_n <= 65536)
-__global__ void k_hc_brow(const int64_t* __restrict__ nbidx, const int64_t* __restrict__ flag,
-                          int64_t bm, int64_t bn, int64_t nblk, int64_t m, int64_t R,
-                          int32_t* __restrict__ brow_ptr, uint8_t* __restrict__ flat,
-                          int64_t* __restrict__ ucnt, int64_t* __restrict__ fcnt) {
-  const int64_t br = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (br > bm) return;
-  const int64_t first = br == bm ? nblk : bn > 0 ? nbidx[br * bn] : 0;
-  brow_ptr[br] = (int32_t)first;
-  if (br == bm) return;
-  const int64_t last = bn > 0 ? nbidx[br * bn + bn - 1] + flag[br * bn + bn - 1] : first;
-  const bool f = bn <= 65536 && last - first > kHcsrFlat;
-  flat[br] = f ? 1 : 0;
-  const int64_t rows = (m - br * R) < R ? (m - br * R) : R;
-  ucnt[br] = f ? 0 : (rows + kWG - 1) / kWG;
-  fcnt[br] = f ? 1 : 0;
-}
-
-// the non-empty block list; COO-kind blocks of unit block-rows ask for
-// S + 1 unit-boundary offsets
-__global__ void k_hc_list(const int32_t* __restrict__ bp, const uint8_t* __restrict__ kind,
-                          const int64_t* __restrict__ off, const int64_t* __restrict__ nbidx,
-                          const uint8_t* __restrict__ flat, int64_t nblocks, int64_t bn,
-                          int64_t S, int32_t* __restrict__ nb_col, int32_t* __restrict__ nb_ptr,
-                          uint32_t* __restrict__ nb_roff, int64_t* __restrict__ ccnt) {
-  const int64_t b = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (b >= nblocks || bp[b + 1] == bp[b]) return;
-  const int64_t k = nbidx[b];
-  nb_col[k] = (int32_t)(b % bn);
-  nb_ptr[k] = bp[b];
-  const bool coo_unit = kind[b] == 2 && !flat[b / bn];
-  nb_roff[k] = kind[b] == 1 ? ((1u << 30) | (uint32_t)off[b]) : (2u << 30);
-  ccnt[k] = coo_unit ? S + 1 : 0;
-}
-
-__global__ void k_hc_csplit(const int32_t* __restrict__ bp, const uint8_t* __restrict__ kind,
-                            const int64_t* __restrict__ nbidx, const uint8_t* __restrict__ flat,
-                            const int64_t* __restrict__ coff, const uint16_t* __restrict__ lrow,
-                            int64_t nblocks, int64_t bn, int64_t S, uint32_t* __restrict__ nb_roff,
-                            int32_t* __restrict__ csplit) {
-  const int64_t b = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (b >= nblocks || bp[b + 1] == bp[b] || kind[b] != 2 || flat[b / bn]) return;
-  const int64_t k = nbidx[b], o = coff[k];
-  nb_roff[k] = (2u << 30) | (uint32_t)o;
-  int64_t e = bp[b];
-  for (int64_t s = 0; s <= S; ++s) {
-    e = dev_lower_bound(lrow, e, (int64_t)bp[b + 1], (uint64_t)(s * kWG));
-    csplit[o + s] = (int32_t)(e - bp[b]);
-  }
-}
-
-__global__ void k_hc_units(const int64_t* __restrict__ ucnt, const int64_t* __restrict__ ubase,
-                           const int64_t* __restrict__ fcnt, const int64_t* __restrict__ fbase,
-                           int64_t bm, int32_t* __restrict__ u_br, int32_t* __restrict__ u_lr0,
-                           int32_t* __restrict__ flat_rows) {
-  const int64_t br = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  if (br >= bm) return;
-  for (int64_t u = 0; u < ucnt[br]; ++u) {
-    u_br[ubase[br] + u] = (int32_t)br;
-    u_lr0[ubase[br] + u] = (int32_t)(u * kWG);
-  }
-  if (fcnt[br]) flat_rows[fbase[br]] = (int32_t)br;
-}
-
-__global__ void k_hc_sentinel(int32_t* __restrict__ nb_ptr, int64_t nblk, int64_t nnz) {
-  if (threadIdx.x == 0) nb_ptr[nblk] = (int32_t)nnz;
 }
 
 namespace {
@@ -248,7 +168,6 @@ int build_hcsr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const i
                       hipStream_t s) {
   DevHCSR& d = M.hcsr;
   const int64_t bm = (m + R - 1) / R, bn = (n + C - 1) / C, nblocks = bm * bn;
-  const int64_t S = (R + kWG - 1) / kWG;
   d.R = R;
   d.C = C;
   d.b_m = bm;
@@ -294,41 +213,25 @@ int build_hcsr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const i
   hipLaunchKernelGGL(k_hc_row_offset, dim3(grid_of(nblocks + 1)), dim3(kWG), 0, s, kind, off,
                      nblocks, total, ro);
   launched("k_hc_row_offset launch");
-  // the non-empty blocks, the entry-by-entry block-rows, the units
-  int64_t* flag = (int64_t*)M.temp((size_t)nblocks * 16 + 8);
-  int64_t* nbidx = flag + nblocks;
-  hipLaunchKernelGGL(k_hc_nonempty, dim3(grid_of(nblocks)), dim3(kWG), 0, s, bp, nblocks, flag);
-  launched("k_hc_nonempty launch");
-  const int64_t nblk = scan_total(M, flag, nbidx, nblocks, s);
-  int32_t* brow_ptr = (int32_t*)M.put(nullptr, (size_t)(bm + 1) * 4);
-  uint8_t* flat = (uint8_t*)M.temp((size_t)bm + 1);
-  int64_t* bcnt = (int64_t*)M.temp((size_t)bm * 32 + 8);
-  int64_t *ucnt = bcnt, *ubase = bcnt + bm, *fcnt = bcnt + 2 * bm, *fbase = bcnt + 3 * bm;
-  hipLaunchKernelGGL(k_hc_brow, dim3(grid_of(bm + 1)), dim3(kWG), 0, s, nbidx, flag, bm, bn, nblk,
-                     m, R, brow_ptr, flat, ucnt, fcnt);
-  launched("k_hc_brow launch");
-  const int64_t nunits = scan_total(M, ucnt, ubase, bm, s);
-  const int64_t nflat = scan_total(M, fcnt, fbase, bm, s);
   // 3. the entries
   const size_t vbytes = M.dtype == RVVLA_F64 ? 8 : M.dtype == RVVLA_F32 ? 4 : 2;
   uint16_t* col16 = (uint16_t*)M.put(nullptr, (size_t)nnz * 2);
   void* sval = M.put(nullptr, (size_t)nnz * vbytes);
-  uint16_t* lrow = (uint16_t*)M.put(nullptr, (size_t)nnz * 2);
-  uint16_t* ebc16 = nflat > 0 ? (uint16_t*)M.put(nullptr, (size_t)nnz * 2) : nullptr;
+  uint16_t* lrow = (uint16_t*)M.temp((size_t)nnz * 2 + 2);
   uint32_t* row_ptr = (uint32_t*)M.put(nullptr, (size_t)total * 4);
   if (nnz > 0) {
     const unsigned g = grid_of(nnz);
     if (M.dtype == RVVLA_BF16)
       hipLaunchKernelGGL((k_hc_scatter<V, uint16_t, 1>), dim3(g), dim3(kWG), 0, s, skey, sidx,
                          erow, col, val, bp, kind, off, nnz, bn, R, C, col16, (uint16_t*)sval,
-                         lrow, ebc16, row_ptr);
+                         lrow, row_ptr);
     else if (M.dtype == RVVLA_F16)
       hipLaunchKernelGGL((k_hc_scatter<V, uint16_t, 2>), dim3(g), dim3(kWG), 0, s, skey, sidx,
                          erow, col, val, bp, kind, off, nnz, bn, R, C, col16, (uint16_t*)sval,
-                         lrow, ebc16, row_ptr);
+                         lrow, row_ptr);
     else
       hipLaunchKernelGGL((k_hc_scatter<V, V, 0>), dim3(g), dim3(kWG), 0, s, skey, sidx, erow, col,
-                         val, bp, kind, off, nnz, bn, R, C, col16, (V*)sval, lrow, ebc16, row_ptr);
+                         val, bp, kind, off, nnz, bn, R, C, col16, (V*)sval, lrow, row_ptr);
     launched("k_hc_scatter launch");
   }
   if (total > 0) {
@@ -336,62 +239,14 @@ int build_hcsr_device(rvvla_mat_s& M, int64_t m, int64_t n, int64_t nnz, const i
                        nblocks, total, lrow, row_ptr);
     launched("k_hc_csr_rowptr launch");
   }
-  // 4. the block list, unit boundaries, units
-  int32_t* nb_col = (int32_t*)M.put(nullptr, (size_t)nblk * 4);
-  int32_t* nb_ptr = (int32_t*)M.put(nullptr, (size_t)(nblk + 1) * 4);
-  uint32_t* nb_roff = (uint32_t*)M.put(nullptr, (size_t)nblk * 4);
-  int64_t* ccnt = (int64_t*)M.temp((size_t)nblk * 16 + 8);
-  int64_t* coff = ccnt + nblk;
-  hipLaunchKernelGGL(k_hc_list, dim3(grid_of(nblocks)), dim3(kWG), 0, s, bp, kind, off, nbidx,
-                     flat, nblocks, bn, S, nb_col, nb_ptr, nb_roff, ccnt);
-  launched("k_hc_list launch");
-  hipLaunchKernelGGL(k_hc_sentinel, dim3(1), dim3(64), 0, s, nb_ptr, nblk, nnz);
-  launched("k_hc_sentinel launch");
-  const int64_t ncsplit = scan_total(M, ccnt, coff, nblk, s);
-  if (ncsplit > 0x3FFFFFFF) {
-    set_last_error("HCSR: csplit overflow");
-    return RVVLA_ERR_LENGTH;
-  }
-  int32_t* csplit = (int32_t*)M.put(nullptr, (size_t)ncsplit * 4);
-  hipLaunchKernelGGL(k_hc_csplit, dim3(grid_of(nblocks)), dim3(kWG), 0, s, bp, kind, nbidx, flat,
-                     coff, lrow, nblocks, bn, S, nb_roff, csplit);
-  launched("k_hc_csplit launch");
-  int32_t* u_br = (int32_t*)M.put(nullptr, (size_t)nunits * 4);
-  int32_t* u_lr0 = (int32_t*)M.put(nullptr, (size_t)nunits * 4);
-  int32_t* flat_rows = (int32_t*)M.put(nullptr, (size_t)nflat * 4);
-  hipLaunchKernelGGL(k_hc_units, dim3(grid_of(bm)), dim3(kWG), 0, s, ucnt, ubase, fcnt, fbase, bm,
-                     u_br, u_lr0, flat_rows);
-  launched("k_hc_units launch");
-  uint8_t* flat_keep = nullptr;
-  if (nflat > 0) {
-    flat_keep = (uint8_t*)M.put(nullptr, (size_t)bm);
-    if (int rc = hip_check(hipMemcpyAsync(flat_keep, flat, (size_t)bm, hipMemcpyDeviceToDevice, s),
-                           "copy"))
-      return rc;
-  }
   if (int rc = hip_check(hipStreamSynchronize(s), "device HCSR build")) return rc;
   M.free_temps();
-  d.brow_ptr = brow_ptr;
-  d.nb_col = nb_col;
-  d.nb_ptr = nb_ptr;
-  d.nb_roff = nb_roff;
-  d.rp32 = row_ptr;
-  d.coo16 = lrow;
-  d.col16 = col16;
-  d.val = sval;
-  d.flat = flat_keep;
-  d.ebc16 = ebc16;
-  d.nflat = nflat;
-  d.flat_rows = flat_rows;
-  d.nunits = nunits;
-  d.nblk = nblk;
-  d.nrp32 = total;
-  d.ncsplit = ncsplit;
-  d.u_br = u_br;
-  d.u_lr0 = u_lr0;
-  d.csplit = csplit;
   d.ref_block_ptr = bp;
   d.ref_row_offset = ro;
+  d.ref_row_ptr = row_ptr;
+  d.nref_row_ptr = total;
+  d.ref_col_idx = col16;
+  d.ref_values = sval;
   return RVVLA_OK;
 }
 

@@ -54,17 +54,13 @@ __global__ __launch_bounds__(kWG) void k_sell64(
     const A v0 = ldv<NT>(val, idx), v1 = ldv<NT>(val, idx + height);
     const A v2 = ldv<NT>(val, idx + 2 * (int64_t)height);
     const A v3 = ldv<NT>(val, idx + 3 * (int64_t)height);
-    if constexpr (BOUND) {
-      a0 += j < len ? v0 * (A)x[c0] : A(0);
-      a1 += j + 1 < len ? v1 * (A)x[c1] : A(0);
-      a2 += j + 2 < len ? v2 * (A)x[c2] : A(0);
-      a3 += j + 3 < len ? v3 * (A)x[c3] : A(0);
-    } else {
-      a0 += v0 * (A)x[c0];
-      a1 += v1 * (A)x[c1];
-      a2 += v2 * (A)x[c2];
-      a3 += v3 * (A)x[c3];
-    }
+    // the gathers run for every slot (padding holds a valid column); BOUND
+    // then drops a padding product (possibly 0 * Inf) by a select
+    const A p0 = v0 * (A)x[c0], p1 = v1 * (A)x[c1], p2 = v2 * (A)x[c2], p3 = v3 * (A)x[c3];
+    a0 += !BOUND || j < len ? p0 : A(0);
+    a1 += !BOUND || j + 1 < len ? p1 : A(0);
+    a2 += !BOUND || j + 2 < len ? p2 : A(0);
+    a3 += !BOUND || j + 3 < len ? p3 : A(0);
     idx += 4 * (int64_t)height;
   }
   for (; j < width; ++j, idx += height) {

@@ -141,6 +141,15 @@ struct Acc {
   __device__ __forceinline__ T value() const { return COMP ? s + c : s; }
 };
 
+// Marks a loaded value as used here, unconditionally, so that the
+// compiler cannot move its load under a later select or branch (a load
+// moved into a conditional block is waited for on its own, which
+// serialises a batch of gathers).  Emits no instruction.
+template <class T>
+__device__ __forceinline__ void keep_live(T& v) {
+  asm volatile("" : "+v"(v));
+}
+
 template <class T>
 __device__ __forceinline__ T shfl_xor(T v, int m) {
   return __shfl_xor(v, m, 64);

@@ -205,8 +205,7 @@ def _compensated_bound(y_exact, absdot):
 @pytest.mark.parametrize("dtype", [R.BF16, R.F16])
 @pytest.mark.parametrize("fmt,params", [("csr", ""), ("csr", "xlds=1"), ("csr", "xslices=8"),
                                         ("csr", "sell=1"),
-                                        pytest.param("hcsr", "", marks=pytest.mark.skip(
-                                            reason="HCSR 16-bit sums not yet compensated")),
+                                        ("hcsr", ""),
                                         ("sell", ""), ("csr5", ""), ("lav", ""), ("vnec", ""),
                                         ("vhcc", ""),
                                         pytest.param("cvr", "", marks=pytest.mark.skip(
