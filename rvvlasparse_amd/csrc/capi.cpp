@@ -695,46 +695,73 @@ rvvla_mat_t upload_cvr(int dtype, const spMtxCVR<V>& a, int64_t nnz) {
   std::vector<int64_t> pstart((size_t)S + 1);
   std::vector<int32_t> rstart((size_t)S), rec_rows, tails((size_t)(S * 64), -1);
   std::vector<uint64_t> flush((size_t)npacks, 0);
-  std::vector<char> done((size_t)m, 0);
+  std::vector<int32_t> times((size_t)m, 0);  // records of each row
   const int* wb = a.writeback_pos_record.data();
   for (int64_t t = 0; t <= S; ++t) pstart[(size_t)t] = t < S ? wb[t * (L + 2)] : npacks;
+  auto rec_range = [&](int64_t t) {
+    return std::make_pair((int64_t)wb[t * (L + 2) + 1], t + 1 < S ? (int64_t)wb[(t + 1) * (L + 2) + 1] : nrec);
+  };
   for (int64_t t = 0; t < S; ++t) {
     if (pstart[(size_t)t] < 0 || pstart[(size_t)t] > pstart[(size_t)t + 1])
       throw Error(RVVLA_ERR_INVALID_ARG, "CVR: stream pack ranges");
-    const int64_t r0 = wb[t * (L + 2) + 1], r1 = t + 1 < S ? wb[(t + 1) * (L + 2) + 1] : nrec;
+    const auto [r0, r1] = rec_range(t);
     if (r0 < 0 || r0 > r1 || r1 > nrec) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: record ranges");
+    int64_t prev = -1;
+    for (int64_t q = r0; q < r1; ++q) {
+      const int64_t pos = a.vPack_vec_record[(size_t)(2 * q)];
+      const int64_t row = a.vPack_vec_record[(size_t)(2 * q + 1)];
+      const int64_t k = pos / L, lane = pos % L;
+      if (row < 0 || row >= m || k < pstart[(size_t)t] || k > pstart[(size_t)t + 1])
+        throw Error(RVVLA_ERR_INVALID_ARG, "CVR: record out of range");
+      // records must come in (pack, lane) order inside a stream for the
+      // rank lookup
+      if (pos <= prev) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: records not in position order");
+      prev = pos;
+      ++times[(size_t)row];
+    }
+  }
+  // rows recorded more than once: their slots, row by row (each row's in
+  // record order); listed for the combine with the rows of more than 64
+  // slots first, then the others, each part in row order
+  std::vector<int32_t> split_rows, split_ptr, sbase((size_t)m, -1);
+  int64_t nslots = 0;
+  for (int64_t r = 0; r < m; ++r)
+    if (times[(size_t)r] > 1) {
+      sbase[(size_t)r] = (int32_t)nslots;
+      nslots += times[(size_t)r];
+    }
+  if (nslots > 2147483647LL) throw Error(RVVLA_ERR_LENGTH, "CVR: too many partial sums");
+  for (int pass = 0; pass < 2; ++pass)
+    for (int64_t r = 0; r < m; ++r)
+      if (times[(size_t)r] > 1 && (times[(size_t)r] > 64) == (pass == 0)) {
+        split_rows.push_back((int32_t)r);
+        split_ptr.push_back(sbase[(size_t)r]);
+        split_ptr.push_back(sbase[(size_t)r] + times[(size_t)r]);
+      }
+  int64_t nsplit_long = 0;
+  for (int64_t r = 0; r < m; ++r) nsplit_long += times[(size_t)r] > 64;
+  std::vector<int32_t> seen((size_t)m, 0);
+  for (int64_t t = 0; t < S; ++t) {
+    const auto [r0, r1] = rec_range(t);
     rstart[(size_t)t] = (int32_t)rec_rows.size();
     for (int64_t q = r0; q < r1; ++q) {
       const int64_t pos = a.vPack_vec_record[(size_t)(2 * q)];
       const int64_t row = a.vPack_vec_record[(size_t)(2 * q + 1)];
       const int64_t k = pos / L, lane = pos % L;
-      if (row < 0 || row >= m || done[(size_t)row] || k < pstart[(size_t)t] || k > pstart[(size_t)t + 1])
-        throw Error(RVVLA_ERR_INVALID_ARG, "CVR: record out of range or row flushed twice");
-      done[(size_t)row] = 1;
+      const int32_t target = sbase[(size_t)row] < 0
+          ? (int32_t)row : cvr_slot_target(sbase[(size_t)row] + seen[(size_t)row]++);
       if (k == pstart[(size_t)t + 1]) {
-        tails[(size_t)(t * 64 + lane)] = (int32_t)row;  // flushed after the stream's last pack
+        tails[(size_t)(t * 64 + lane)] = target;  // flushed after the stream's last pack
       } else {
         if ((flush[(size_t)k] >> lane) & 1u) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: duplicate record");
         flush[(size_t)k] |= uint64_t(1) << lane;
-        rec_rows.push_back((int32_t)row);
-      }
-    }
-  }
-  // records must come in (pack, lane) order inside a stream for the rank lookup
-  {
-    for (int64_t t = 0; t < S; ++t) {
-      const int64_t r0 = wb[t * (L + 2) + 1], r1 = t + 1 < S ? wb[(t + 1) * (L + 2) + 1] : nrec;
-      int64_t prev = -1;
-      for (int64_t q = r0; q < r1; ++q) {
-        const int64_t pos = a.vPack_vec_record[(size_t)(2 * q)];
-        if (pos <= prev) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: records not in position order");
-        prev = pos;
+        rec_rows.push_back(target);
       }
     }
   }
   std::vector<int32_t> empty;
   for (int64_t r = 0; r < m; ++r)
-    if (!done[(size_t)r]) empty.push_back((int32_t)r);
+    if (!times[(size_t)r]) empty.push_back((int32_t)r);
   auto M = new_mat(RVVLA_FMT_CVR, dtype, m, n, nnz);
   DevCVR& d = M->cvr;
   d.nstreams = S;
@@ -749,6 +776,14 @@ rvvla_mat_t upload_cvr(int dtype, const spMtxCVR<V>& a, int64_t nnz) {
   d.col = M->put_vec(a.vPack_vec_cols);
   d.val = put_values(*M, dtype, a.vPack_vec_vals.data(), a.vPack_vec_vals.size());
   d.empty_rows = M->put_vec(empty);
+  d.nsplit = (int64_t)split_rows.size();
+  d.nsplit_long = nsplit_long;
+  d.nslots = nslots;
+  if (d.nsplit > 0) {
+    d.split_rows = M->put_vec(split_rows);
+    d.split_ptr = M->put_vec(split_ptr);
+    d.part = M->put(nullptr, (size_t)d.nslots * (dtype == RVVLA_F64 ? 8 : 4));
+  }
   return M.release();
 }
 
@@ -1875,6 +1910,7 @@ int64_t spmv_scratch_bytes(const rvvla_mat_s& A) {
       return 0;
     case RVVLA_FMT_SELL: return sell_part(A.sell);
     case RVVLA_FMT_HCSR: return A.hcsr.nparts * 64 * acc;
+    case RVVLA_FMT_CVR: return A.cvr.nslots * acc;
     case RVVLA_FMT_LAV: return A.lav.nsplit > 0 ? A.lav.npieces * 64 * acc : 0;
     case RVVLA_FMT_CSR5: return A.csr5.ntiles * (2 * acc + 4) + 16;
     case RVVLA_FMT_VHCC: return A.vhcc.nparts * 2 * acc + 16;
@@ -1892,7 +1928,7 @@ int spmv_dispatch(const rvvla_mat_s& A, double alpha, const void* x, double beta
     case RVVLA_FMT_HCSR: return launch_hcsr(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_SELL: return launch_sell(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_CSR5: return launch_csr5(A, alpha, x, beta, y, s, scratch);
-    case RVVLA_FMT_CVR: return launch_cvr(A, alpha, x, beta, y, s);
+    case RVVLA_FMT_CVR: return launch_cvr(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_VHCC: return launch_vhcc(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_LAV: return launch_lav(A, alpha, x, beta, y, s, scratch);
     case RVVLA_FMT_VNEC: return launch_vnec(A, alpha, x, beta, y, s, scratch);

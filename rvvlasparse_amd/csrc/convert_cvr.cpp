@@ -5,7 +5,9 @@
 // method of Xie et al. (CGO 2018): rows are split into Nthreads streams of
 // contiguous rows with balanced entry counts; inside a stream SIMD_LEN lanes
 // each take the next non-empty row whenever their current row is used up,
-// so pack k holds one entry per lane.  A lane that switches rows before
+// so pack k holds one entry per lane.  Rows longer than cvr_segment_cap
+// entries are taken as several segments of at most that many entries, each
+// recorded under the row (the records add into y).  A lane that switches rows before
 // pack k produces the record (k*SIMD_LEN + lane, finished row); a lane with
 // nothing left holds a zero-valued pad entry (column of the stream's last
 // entry).  Stream t's packs start at writeback_pos_record[t*(SIMD_LEN+2)],
@@ -41,27 +43,47 @@ std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts
   return b;
 }
 
+int64_t cvr_segment_cap(int64_t nnz, int64_t nstreams) {
+  const int64_t per_lane = (nnz + nstreams * 64 - 1) / std::max<int64_t>(1, nstreams * 64);
+  return std::max<int64_t>(64, per_lane);
+}
+
 template <class V>
 void build_cvr(spMtxCVR<V>& d, const spMtxCRS<V>& s, int simd_len, int nstreams) {
   if (simd_len < 1 || simd_len > 64) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: SIMD_LEN in [1, 64]");
   if (nstreams < 1) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: at least one stream");
   const int64_t m = (int64_t)s.m, L = simd_len;
-  const std::vector<int64_t> rb = balanced_row_parts(m, s.Rst, nstreams);
-  // A stream lasts at least as many packs as its longest row: with rows far
-  // longer than a stream's share, the lanes idle on padding.  Refuse layouts
-  // whose lower bound on packed slots exceeds 16x the entries (plus 2^20).
+  // Rows longer than the cap (about one stream's share of entries per
+  // lane) are cut into segments of cap entries, which the lanes take like
+  // rows and whose records all name the row: the reference kernel adds
+  // every record into y, so a row may be recorded once per segment.
+  // Without the cut one lane would run a whole long row while the others
+  // idle on padding.
+  const int64_t cap = cvr_segment_cap((int64_t)s.nz, nstreams);
+  std::vector<int> vrst{0};
+  std::vector<int32_t> vrow;
+  for (int64_t r = 0; r < m; ++r)
+    for (int64_t e = s.Rst[r]; e < s.Rst[r + 1]; e += cap) {
+      vrst.push_back((int)std::min<int64_t>(e + cap, s.Rst[r + 1]));
+      vrow.push_back((int32_t)r);
+    }
+  const int64_t nv = (int64_t)vrow.size();
+  const std::vector<int64_t> rb = balanced_row_parts(nv, vrst.data(), nstreams);
+  // A stream lasts at least as many packs as its longest segment: refuse
+  // layouts whose lower bound on packed slots exceeds 16x the entries (plus
+  // 2^20) -- many more streams than entries
   {
     int64_t bound = 0;
     for (int t = 0; t < nstreams; ++t) {
       int64_t longest = 0;
-      for (int64_t r = rb[(size_t)t]; r < rb[(size_t)t + 1]; ++r)
-        longest = std::max<int64_t>(longest, s.Rst[r + 1] - s.Rst[r]);
-      const int64_t nz_t = s.Rst[rb[(size_t)t + 1]] - s.Rst[rb[(size_t)t]];
+      for (int64_t v = rb[(size_t)t]; v < rb[(size_t)t + 1]; ++v)
+        longest = std::max<int64_t>(longest, vrst[(size_t)v + 1] - vrst[(size_t)v]);
+      const int64_t nz_t = vrst[(size_t)rb[(size_t)t + 1]] - vrst[(size_t)rb[(size_t)t]];
       bound += std::max<int64_t>((nz_t + L - 1) / L, longest) * L;
     }
     if (bound > 16 * (int64_t)s.nz + (int64_t(1) << 20) || bound > 2147483647LL)
       throw Error(RVVLA_ERR_UNSUPPORTED_SPMV,
-                  "CVR: rows are too long for the stream layout (at least " + std::to_string(bound) +
+                  "CVR: too many streams for the entries (at least " + std::to_string(bound) +
                       " packed slots for " + std::to_string(s.nz) + " entries)");
   }
   // pass 1 (parallel): each stream packed on its own
@@ -77,34 +99,33 @@ void build_cvr(spMtxCVR<V>& d, const spMtxCRS<V>& s, int simd_len, int nstreams)
     Stream& S = st[(size_t)t];
     int64_t next = rb[(size_t)t];
     const int64_t end = rb[(size_t)t + 1];
-    std::vector<int64_t> row((size_t)L, -1), pos((size_t)L, 0), stop((size_t)L, 0);
+    std::vector<int64_t> seg((size_t)L, -1), pos((size_t)L, 0), stop((size_t)L, 0);
     uint32_t pad_col = 0;
-    if (end > rb[(size_t)t] && s.Rst[end] > s.Rst[rb[(size_t)t]]) pad_col = (uint32_t)s.Col[s.Rst[end] - 1];
+    if (end > rb[(size_t)t]) pad_col = (uint32_t)s.Col[vrst[(size_t)end] - 1];
     auto take = [&](int64_t lane) {
-      while (next < end && s.Rst[next + 1] == s.Rst[next]) ++next;  // skip empty rows
       if (next >= end) {
-        row[(size_t)lane] = -1;
+        seg[(size_t)lane] = -1;
         return;
       }
-      row[(size_t)lane] = next;
-      pos[(size_t)lane] = s.Rst[next];
-      stop[(size_t)lane] = s.Rst[next + 1];
+      seg[(size_t)lane] = next;
+      pos[(size_t)lane] = vrst[(size_t)next];
+      stop[(size_t)lane] = vrst[(size_t)next + 1];
       ++next;
     };
     for (int64_t l = 0; l < L; ++l) take(l);
     for (int64_t k = 0;; ++k) {
       bool any = false;
       for (int64_t l = 0; l < L; ++l) {
-        if (row[(size_t)l] >= 0 && pos[(size_t)l] == stop[(size_t)l]) {
+        if (seg[(size_t)l] >= 0 && pos[(size_t)l] == stop[(size_t)l]) {
           S.rec.push_back((int)(k * L + l));
-          S.rec.push_back((int)row[(size_t)l]);
+          S.rec.push_back(vrow[(size_t)seg[(size_t)l]]);
           take(l);
         }
-        any |= row[(size_t)l] >= 0;
+        any |= seg[(size_t)l] >= 0;
       }
       if (!any) break;
       for (int64_t l = 0; l < L; ++l) {
-        if (row[(size_t)l] >= 0) {
+        if (seg[(size_t)l] >= 0) {
           S.vals.push_back(s.Val[pos[(size_t)l]]);
           S.cols.push_back((uint32_t)s.Col[pos[(size_t)l]]);
           ++pos[(size_t)l];

@@ -146,12 +146,25 @@ struct DevCVR {
   const int64_t* pack_start = nullptr;  // nstreams + 1
   const int32_t* rec_start = nullptr;   // nstreams
   const uint64_t* flush = nullptr;      // one mask per pack
+  // flush targets: a row >= 0 written directly, or -(slot + 2) for a
+  // flush of a row recorded more than once (a cut long row): its partial
+  // sum goes to part[slot]
   const int32_t* rec_rows = nullptr;
   const int32_t* tail_rows = nullptr;   // 64 per stream, -1 = none
   const uint32_t* col = nullptr;
   const void* val = nullptr;
   const int32_t* empty_rows = nullptr;
+  // rows recorded more than once: split_rows[i] sums part[split_ptr[2i] ..
+  // split_ptr[2i+1]) in slot order (its flushes in record order; slots are
+  // given row by row); the first nsplit_long of them have more than 64
+  // slots
+  int64_t nsplit = 0, nsplit_long = 0, nslots = 0;
+  const int32_t* split_rows = nullptr;
+  const int32_t* split_ptr = nullptr;
+  void* part = nullptr;
 };
+// CVR flush target of slot q (rows recorded more than once).
+constexpr int32_t cvr_slot_target(int64_t q) { return (int32_t)(-q - 2); }
 
 // VHCC (vec_size = 64): one partition per wavefront; the row-end mask of
 // every vector, the target rows, and the partition hand-over description.
@@ -401,7 +414,7 @@ int launch_sell(const rvvla_mat_s& A, double alpha, const void* x, double beta, 
 int launch_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s, void* scratch = nullptr);
 int launch_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s);
+               hipStream_t s, void* scratch = nullptr);
 int launch_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s, void* scratch = nullptr);
 int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,

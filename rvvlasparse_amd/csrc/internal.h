@@ -72,6 +72,9 @@ constexpr int64_t kCvrStreamEntries = 4096;
 constexpr int64_t kVhccPartEntries = 1024;
 // Rows [b[t], b[t+1]) of part t: contiguous, about nnz/parts entries each.
 std::vector<int64_t> balanced_row_parts(int64_t m, const int* rst, int64_t parts);
+// CVR: the most entries a lane takes as one row; longer rows are cut into
+// segments of this many (at least 64, about one stream's share per lane).
+int64_t cvr_segment_cap(int64_t nnz, int64_t nstreams);
 
 // The device image of an HCSR matrix (hcsr_image.cpp) from the validated
 // reference layout.

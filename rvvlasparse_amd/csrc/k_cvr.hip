@@ -5,12 +5,14 @@
 // src/scalar/spmv_mtx_cvr.cpp:32-66): every lane accumulates the entries of
 // its current row in pack order; when a record says the lane switches rows
 // before pack k, the lane's sum is flushed into that row; at the stream's
-// end the remaining lanes flush into their open rows.  Each non-empty row
-// is flushed exactly once, so the flush writes y = alpha*acc + beta*y
-// directly (the reference scales y by beta first); empty rows come from a
-// separate list.  The records are re-laid at upload as one 64-bit flush
-// mask per pack (a uniform scalar load) plus the flushed rows in record
-// order; a lane finds its row by counting the flushing lanes below it.
+// end the remaining lanes flush into their open rows.  A row flushed once
+// is written directly, y = alpha*acc + beta*y (the reference scales y by
+// beta first); a long row cut into segments is flushed once per segment,
+// each flush into its own slot of a partial-sum buffer, and k_cvr_combine
+// adds a row's slots in order; empty rows come from a separate list.  The
+// records are re-laid at upload as one 64-bit flush mask per pack (a
+// uniform scalar load) plus the flush targets in record order; a lane finds
+// its target by counting the flushing lanes below it.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -21,8 +23,8 @@ __global__ __launch_bounds__(kWG) void k_cvr(
     const uint64_t* __restrict__ flush, const int32_t* __restrict__ rec_rows,
     const int32_t* __restrict__ tail_rows, const uint32_t* __restrict__ col,
     const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
-    typename Val<VT>::io* __restrict__ y, int64_t nstreams, typename Val<VT>::acc alpha,
-    typename Val<VT>::acc beta) {
+    typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ part,
+    int64_t nstreams, typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= nstreams) return;
@@ -31,6 +33,10 @@ __global__ __launch_bounds__(kWG) void k_cvr(
   const int64_t k0 = pack_start[t], k1 = pack_start[t + 1];
   int32_t rb = rec_start[t];
   Acc<A, COMP> acc;  // 16-bit values (COMP): TwoSum compensated
+  auto write = [&](int32_t r) {
+    if (r >= 0) y[r] = alpha * acc.value() + scaled_y(beta, y, r);
+    else part[-(int64_t)r - 2] = acc.value();
+  };
   // kBatch packs at a time: masks, flushed rows, columns, values and the
   // gathers are all issued before the packs are walked
   constexpr int kBatch = 8;
@@ -56,14 +62,50 @@ __global__ __launch_bounds__(kWG) void k_cvr(
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if ((mk[u] >> lane) & 1u) {
-        y[rr[u]] = alpha * acc.value() + scaled_y(beta, y, rr[u]);
+        write(rr[u]);
         acc = Acc<A, COMP>();
       }
       acc.add(p[u]);  // zero past the stream's last pack
     }
   }
   const int32_t r = tail_rows[t * 64 + lane];
-  if (r >= 0) y[r] = alpha * acc.value() + scaled_y(beta, y, r);
+  if (r != -1) write(r);
+}
+
+// Rows flushed more than once: y = alpha * (their slots' sum, in slot
+// order) + beta * y.  Rows with more than 64 slots take a wavefront each
+// (lane l adds slots l, l + 64, ...; the lane sums are then added in a
+// fixed tree), the others a thread each.
+template <class VT, bool COMP>
+__global__ __launch_bounds__(kWG) void k_cvr_combine_long(
+    const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr, int64_t n,
+    const typename Val<VT>::acc* __restrict__ part, typename Val<VT>::io* __restrict__ y,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t i = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
+  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
+  Acc<A, COMP> s;
+  for (int64_t q = ptr[2 * i] + lane; q < ptr[2 * i + 1]; q += 64) s.add(part[q]);
+  s = group_reduce(s, 64);
+  if (lane == 0) {
+    const int32_t r = rows[i];
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
+  }
+}
+
+template <class VT, bool COMP>
+__global__ __launch_bounds__(kWG) void k_cvr_combine(
+    const int32_t* __restrict__ rows, const int32_t* __restrict__ ptr, int64_t i0, int64_t n,
+    const typename Val<VT>::acc* __restrict__ part, typename Val<VT>::io* __restrict__ y,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+  using A = typename Val<VT>::acc;
+  const int64_t i = i0 + (int64_t)blockIdx.x * kWG + threadIdx.x;
+  if (i >= n) return;
+  Acc<A, COMP> s;
+  for (int64_t q = ptr[2 * i]; q < ptr[2 * i + 1]; ++q) s.add(part[q]);
+  const int32_t r = rows[i];
+  y[r] = alpha * s.value() + scaled_y(beta, y, r);
 }
 
 template <class VT>
@@ -79,17 +121,31 @@ __global__ __launch_bounds__(kWG) void k_scale_rows(const int32_t* __restrict__ 
 
 template <class VT>
 static int run_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-                   hipStream_t s) {
+                   hipStream_t s, void* scratch) {
   using IO = typename Val<VT>::io;
   using AC = typename Val<VT>::acc;
+  constexpr bool COMP = sizeof(VT) == 2;
   const DevCVR& d = A.cvr;
+  AC* part = (AC*)(scratch ? scratch : d.part);
   if (d.nstreams > 0) {
     const unsigned g = (unsigned)((d.nstreams + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_cvr<VT, sizeof(VT) == 2>), dim3(g), dim3(kWG), 0, s, d.pack_start,
-                       d.rec_start, d.flush,
-                       d.rec_rows, d.tail_rows, d.col, (const VT*)d.val, (const IO*)x, (IO*)y,
-                       d.nstreams, (AC)alpha, (AC)beta);
+    hipLaunchKernelGGL((k_cvr<VT, COMP>), dim3(g), dim3(kWG), 0, s, d.pack_start, d.rec_start,
+                       d.flush, d.rec_rows, d.tail_rows, d.col, (const VT*)d.val, (const IO*)x,
+                       (IO*)y, part, d.nstreams, (AC)alpha, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_cvr launch")) return rc;
+  }
+  if (d.nsplit_long > 0) {
+    const unsigned g = (unsigned)((d.nsplit_long + kWG / 64 - 1) / (kWG / 64));
+    hipLaunchKernelGGL((k_cvr_combine_long<VT, COMP>), dim3(g), dim3(kWG), 0, s, d.split_rows,
+                       d.split_ptr, d.nsplit_long, (const AC*)part, (IO*)y, (AC)alpha, (AC)beta);
+    if (int rc = hip_check(hipGetLastError(), "k_cvr_combine_long launch")) return rc;
+  }
+  if (d.nsplit > d.nsplit_long) {
+    const unsigned g = (unsigned)((d.nsplit - d.nsplit_long + kWG - 1) / kWG);
+    hipLaunchKernelGGL((k_cvr_combine<VT, COMP>), dim3(g), dim3(kWG), 0, s, d.split_rows,
+                       d.split_ptr, d.nsplit_long, d.nsplit, (const AC*)part, (IO*)y, (AC)alpha,
+                       (AC)beta);
+    if (int rc = hip_check(hipGetLastError(), "k_cvr_combine launch")) return rc;
   }
   if (d.nempty > 0) {
     hipLaunchKernelGGL((k_scale_rows<VT>), dim3((unsigned)((d.nempty + kWG - 1) / kWG)), dim3(kWG),
@@ -100,12 +156,12 @@ static int run_cvr(const rvvla_mat_s& A, double alpha, const void* x, double bet
 }
 
 int launch_cvr(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
-               hipStream_t s) {
+               hipStream_t s, void* scratch) {
   switch (A.dtype) {
-    case RVVLA_F64: return run_cvr<double>(A, alpha, x, beta, y, s);
-    case RVVLA_F32: return run_cvr<float>(A, alpha, x, beta, y, s);
-    case RVVLA_BF16: return run_cvr<bf16_t>(A, alpha, x, beta, y, s);
-    case RVVLA_F16: return run_cvr<_Float16>(A, alpha, x, beta, y, s);
+    case RVVLA_F64: return run_cvr<double>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F32: return run_cvr<float>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_BF16: return run_cvr<bf16_t>(A, alpha, x, beta, y, s, scratch);
+    case RVVLA_F16: return run_cvr<_Float16>(A, alpha, x, beta, y, s, scratch);
   }
   return RVVLA_ERR_VALUE_TYPE;
 }

@@ -41,13 +41,39 @@ def _inputs(n, m, dt, seed):
     return rng.uniform(-1, 1, n).astype(dt), rng.uniform(-1, 1, m).astype(dt)
 
 
+def _cvr_segments(rst, streams):
+    """Segments per row: rows longer than max(64, ceil(nnz / (64 streams)))
+    entries are cut into pieces of that many."""
+    nnz = int(rst[-1])
+    cap = max(64, -(-nnz // (64 * streams)))
+    return -(-np.diff(rst).astype(np.int64) // cap)
+
+
+def test_cvr_segment_counts():
+    """Records per row for rows of 3, 20,000 and 7 entries (hand-counted):
+    one stream cuts at ceil(20,010 / 64) = 313 entries (64 segments), 40
+    streams at 64 (313 segments)."""
+    lens = np.array([3, 20000, 7])
+    rst = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    col = np.concatenate([np.arange(k) for k in lens]).astype(np.int32)
+    a = R.CSR(3, 20000, rst, col, np.ones(len(col)))
+    for streams, want in ((1, [1, 64, 1]), (40, [1, 313, 1])):
+        L = R.convert("cvr", a, R.F64, f"streams={streams}")
+        rec = L.array("vPack_vec_record").view(np.int32).reshape(-1, 2)[:-1, 1]
+        assert list(np.bincount(rec, minlength=3)) == want
+        assert list(_cvr_segments(rst, streams)) == want
+
+
 @need_ref
 @pytest.mark.parametrize("lanes", [8, 64])
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_cvr_single_stream_on_reference_kernel(name, lanes, dt):
     """Single stream: the reference scalar CVR kernel reproduces the scalar
-    CSR result bit for bit (each row summed by one lane in entry order)."""
+    CSR result bit for bit on every row taken whole (one lane, entry order);
+    a row cut into segments is one sum per segment, each added into y by
+    its own record, so it meets the summation bound with two more
+    roundings per segment."""
     (rst, col, val), n = CASES[name][0](), CASES[name][1]
     m = len(rst) - 1
     L = R.convert("cvr", R.CSR(m, n, rst, col, val), R.F64 if dt == np.float64 else R.F32,
@@ -58,7 +84,15 @@ def test_cvr_single_stream_on_reference_kernel(name, lanes, dt):
     x, y0 = _inputs(n, m, dt, 1)
     y = O.cvr_spmv_ref(arr, sc, x, 1.5, -0.5, y0)
     y_ref = O.csr_spmv(rst, col, val.astype(dt), x, 1.5, -0.5, y0, use_ref=True)
-    assert np.array_equal(y, y_ref)
+    seg = _cvr_segments(rst, 1)
+    whole = seg <= 1
+    assert np.array_equal(y[whole], y_ref[whole])
+    u = 2.0 ** -53 if dt == np.float64 else 2.0 ** -24
+    v = val.astype(dt)
+    scale = 1.5 * O.csr_absdot(rst, col, v, x) + 0.5 * np.abs(y0.astype(np.float64))
+    ln = np.diff(rst).astype(np.float64)
+    err = np.abs(y.astype(np.float64) - y_ref)
+    assert np.all(err[~whole] <= (ln + 2 * seg + 2)[~whole] * u * scale[~whole])
 
 
 @pytest.mark.parametrize("streams", [1, 3, 17])
@@ -69,8 +103,8 @@ def test_cvr_invariants(name, streams):
     L = R.convert("cvr", R.CSR(m, n, rst, col, val), R.F64, f"streams={streams}")
     rec = L.array("vPack_vec_record").view(np.int32).reshape(-1, 2)
     rows = rec[:-1, 1]
-    nonempty = np.nonzero(np.diff(rst))[0]
-    assert np.array_equal(np.sort(rows), nonempty)          # every non-empty row flushed once
+    # every non-empty row recorded once per segment
+    assert np.array_equal(np.sort(rows), np.repeat(np.arange(m), _cvr_segments(rst, streams)))
     vals = L.array("vPack_vec_vals").view(np.float64)
     assert np.isclose(np.sort(np.abs(vals[vals != 0])), np.sort(np.abs(val[val != 0]))).all()
     nstreams = int(L.scalar("Nthreads"))

@@ -64,18 +64,6 @@ def case(name) -> R.CSR:
 CASES = ["rand_long", "lap64", "powerlaw", "random16_c1", "empty", "one_long_row"]
 
 
-def cvr_refuses(fmt, name):
-    """CVR cannot pack rows far longer than a stream's share (one lane would
-    run the whole row while the others idle on padding); the builder refuses
-    such layouts with RVVLA_ERR_UNSUPPORTED_SPMV."""
-    if fmt == "cvr" and name in ("powerlaw", "one_long_row"):
-        with pytest.raises(R.RvvlaError) as e:
-            R.convert(fmt, case(name), R.F64)
-        assert e.value.code == -4
-        return True
-    return False
-
-
 def run_gpu(fmt, dtype, a, x64, y064, alpha, beta, params=""):
     dev = _device()
     L = R.convert(fmt, a, dtype, params)
@@ -97,8 +85,6 @@ def inputs(a, seed=0):
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
 @pytest.mark.parametrize("name", CASES)
 def test_spmv_drop_in_precisions(fmt, dtype, name):
-    if cvr_refuses(fmt, name):
-        return
     a = case(name)
     x64, y064 = inputs(a, 1)
     dt = np.float64 if dtype == R.F64 else np.float32
@@ -115,8 +101,6 @@ def test_spmv_drop_in_precisions(fmt, dtype, name):
 @pytest.mark.parametrize("dtype", [R.BF16, R.F16])
 @pytest.mark.parametrize("name", ["rand_long", "powerlaw", "lap64", "one_long_row"])
 def test_spmv_16bit_values(fmt, dtype, name):
-    if cvr_refuses(fmt, name):
-        return
     a = case(name)
     x64, y064 = inputs(a, 2)
     alpha, beta = 1.0, 0.5
@@ -207,9 +191,7 @@ def _compensated_bound(y_exact, absdot):
                                         ("csr", "sell=1"),
                                         ("hcsr", ""),
                                         ("sell", ""), ("csr5", ""), ("lav", ""), ("vnec", ""),
-                                        ("vhcc", ""),
-                                        pytest.param("cvr", "", marks=pytest.mark.skip(
-                                            reason="CVR refuses rows this long"))])
+                                        ("vhcc", ""), ("cvr", "")])
 def test_16bit_long_row_compensated(fmt, params, dtype):
     """Rows longer than 1024 entries are summed with TwoSum compensation in
     the 16-bit value forms: the long row's error against its exact sum stays
