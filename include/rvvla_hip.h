@@ -281,10 +281,14 @@ int rvvla_spmv_noalloc(rvvla_mat_t A, double alpha, const void* x, double beta,
  * (reference include/types.h:24-27; the reference accepts the flag but
  * always computes A x): op 0 = SPARSE_OPERATION_NON_TRANSPOSE (rvvla_spmv),
  * op 1 = SPARSE_OPERATION_TRANSPOSE, y = alpha * A^T * x + beta * y with x of
- * m entries and y of n.  The transpose is implemented for CSR matrices (every
- * dtype): the first transposed product builds a CSR image of A^T on the
- * device (counting sort by column, about as much memory again as A); other
- * formats return RVVLA_ERR_UNSUPPORTED_SPMV. */
+ * m entries and y of n, every format and dtype: the first transposed
+ * product builds a CSR image of A^T on the device (counting sort by column,
+ * about as much memory again as A) -- for CSR from the matrix itself, for
+ * the other formats from the CSR they were converted from (rvvla_upload of
+ * an rvvla_convert layout keeps a reference to it; the C++ convert keeps a
+ * host copy).  Matrices uploaded from format arrays (rvvla_<fmt>_upload) or
+ * built on the device (rvvla_<fmt>_from_device) have no CSR to build it
+ * from and return RVVLA_ERR_UNSUPPORTED_SPMV. */
 int rvvla_spmv_op(rvvla_mat_t A, int op, double alpha, const void* x, double beta,
                   void* y, void* stream);
 /* Y = alpha * A * X + beta * Y for k right-hand sides; X is n x k and Y is

@@ -1366,6 +1366,12 @@ int rvvla_upload(rvvla_layout_t L, rvvla_mat_t* out) {
   return guarded([&]() {
     if (!L || !out) throw Error(RVVLA_ERR_INVALID_ARG, "null argument");
     *out = L->upload();
+    if (L->format != RVVLA_FMT_CSR && *out && L->crs) {
+      if (L->dtype == RVVLA_F64)
+        rvvla::attach_transpose_source<double>(**out, std::static_pointer_cast<const spMtxCRS<double>>(L->crs));
+      else
+        rvvla::attach_transpose_source<float>(**out, std::static_pointer_cast<const spMtxCRS<float>>(L->crs));
+    }
     return RVVLA_OK;
   });
 }
@@ -1763,6 +1769,7 @@ int rvvla_mat_read(rvvla_mat_t A, const char* field, void* dst, int64_t capacity
       : f == "flush" ? V{c.flush, c.npacks, 8} : f == "rec_rows" ? V{c.rec_rows, c.nrec, 4}
       : f == "tail_rows" ? V{c.tail_rows, S * 64, 4} : f == "col" ? V{c.col, c.npacks * 64, 4}
       : f == "val" ? V{c.val, c.npacks * 64, vb} : f == "empty_rows" ? V{c.empty_rows, c.nempty, 4}
+      : f == "split_rows" ? V{c.split_rows, c.nsplit, 4} : f == "split_ptr" ? V{c.split_ptr, 2 * c.nsplit, 4}
       : V{nullptr, -1, 0};
   } else if (A->format == RVVLA_FMT_VHCC) {
     const rvvla::DevVHCC& c = A->vhcc;
@@ -1949,13 +1956,80 @@ int rvvla_spmv(rvvla_mat_t A, double alpha, const void* x, double beta, void* y,
   return rvvla::spmv_dispatch(*A, alpha, x, beta, y, (hipStream_t)stream, nullptr);
 }
 
-// CSR image of the transpose: counting sort of the entries by column (a
-// column's entries stay in row order), values moved with them as raw
-// storage elements, adaptive blocks over the n rows of the transpose.
+// CSR image of the transpose from CSR arrays on the host: counting sort of
+// the entries by column (a column's entries stay in row order), values
+// moved with them as raw storage elements (es bytes each), adaptive blocks
+// over the n rows of the transpose.
+static void put_transposed_image(rvvla_mat_s& A, const int32_t* rst, const int32_t* col,
+                                 const unsigned char* val, size_t es) {
+  const int64_t m = A.m, n = A.n, nnz = A.nnz;
+  std::vector<int32_t> ptr((size_t)n + 1, 0), row((size_t)nnz);
+  std::vector<unsigned char> tval((size_t)nnz * es);
+  for (int64_t e = 0; e < nnz; ++e) ptr[(size_t)col[e] + 1]++;
+  for (int64_t c = 0; c < n; ++c) ptr[(size_t)c + 1] += ptr[(size_t)c];
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (int64_t r = 0; r < m; ++r)
+    for (int32_t e = rst[r]; e < rst[r + 1]; ++e) {
+      const int32_t p = fill[(size_t)col[e]]++;
+      row[(size_t)p] = (int32_t)r;
+      std::memcpy(&tval[(size_t)p * es], &val[(size_t)e * es], es);
+    }
+  const std::vector<int32_t> blocks = rvvla::csr_row_blocks(ptr.data(), 0, n);
+  rvvla::DevCSR& t = A.tcsr;
+  t.rst = A.put_vec(ptr);
+  t.col = A.put_vec(row);
+  t.val = A.put(tval.data(), tval.size());
+  t.blocks = A.put_vec(blocks);
+  t.nblocks = (int64_t)blocks.size() / 2;
+  A.has_tcsr = true;
+}
+
+extern "C++" {
+namespace rvvla {
+template <class V>
+void attach_transpose_source(rvvla_mat_s& M, std::shared_ptr<const spMtxCRS<V>> src) {
+  M.transpose_source = [src](rvvla_mat_s& A) {
+    const spMtxCRS<V>& c = *src;
+    if ((int64_t)c.m != A.m || (int64_t)c.n != A.n || (int64_t)c.nz != A.nnz)
+      throw Error(RVVLA_ERR_LOGIC, "transpose: the source CSR does not match the matrix");
+    // values in the matrix's storage type, rounded as its upload rounds them
+    const size_t es = A.dtype == RVVLA_F64 ? 8 : A.dtype == RVVLA_F32 ? 4 : 2;
+    std::vector<unsigned char> v((size_t)c.nz * es);
+    for (size_t e = 0; e < c.nz; ++e) {
+      if (A.dtype == RVVLA_F64) {
+        const double d = (double)c.Val[e];
+        std::memcpy(&v[e * 8], &d, 8);
+      } else if (A.dtype == RVVLA_F32) {
+        const float f = (float)c.Val[e];
+        std::memcpy(&v[e * 4], &f, 4);
+      } else {
+        const uint16_t h = A.dtype == RVVLA_BF16 ? f32_to_bf16((float)c.Val[e]) : f32_to_f16((float)c.Val[e]);
+        std::memcpy(&v[e * 2], &h, 2);
+      }
+    }
+    put_transposed_image(A, c.Rst, c.Col, v.data(), es);
+  };
+}
+template void attach_transpose_source<double>(rvvla_mat_s&, std::shared_ptr<const spMtxCRS<double>>);
+template void attach_transpose_source<float>(rvvla_mat_s&, std::shared_ptr<const spMtxCRS<float>>);
+}  // namespace rvvla
+}  // extern "C++"
+
+// The CSR image of A^T: a CSR matrix's own arrays copied back, or the
+// source CSR a matrix of another format was converted from.
 static int build_transposed_image(rvvla_mat_t A) {
   if (A->has_tcsr) return RVVLA_OK;
   return guarded([&]() {
-    const int64_t m = A->m, n = A->n, nnz = A->nnz;
+    if (A->format != RVVLA_FMT_CSR) {
+      if (!A->transpose_source)
+        throw Error(RVVLA_ERR_UNSUPPORTED_SPMV,
+                    "the transposed product needs the CSR this matrix was converted from: "
+                    "matrices converted from CSR (rvvla_convert, the C++ convert) keep it; "
+                    "matrices uploaded from format arrays or built on the device do not");
+      A->transpose_source(*A);
+      return RVVLA_OK;
+    }
+    const int64_t m = A->m, nnz = A->nnz;
     const size_t es = A->dtype == RVVLA_F64 ? 8 : A->dtype == RVVLA_F32 ? 4 : 2;
     std::vector<int32_t> rst((size_t)m + 1), col((size_t)nnz);
     std::vector<unsigned char> val((size_t)nnz * es);
@@ -1963,25 +2037,7 @@ static int build_transposed_image(rvvla_mat_t A) {
     if (!rc && nnz) rc = rvvla::hip_check(hipMemcpy(col.data(), A->csr.col, col.size() * 4, hipMemcpyDeviceToHost), "copy Col");
     if (!rc && nnz) rc = rvvla::hip_check(hipMemcpy(val.data(), A->csr.val, val.size(), hipMemcpyDeviceToHost), "copy Val");
     if (rc) throw Error(rc, rvvla::last_error());
-    std::vector<int32_t> ptr((size_t)n + 1, 0), row((size_t)nnz);
-    std::vector<unsigned char> tval((size_t)nnz * es);
-    for (int64_t e = 0; e < nnz; ++e) ptr[(size_t)col[(size_t)e] + 1]++;
-    for (int64_t c = 0; c < n; ++c) ptr[(size_t)c + 1] += ptr[(size_t)c];
-    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
-    for (int64_t r = 0; r < m; ++r)
-      for (int32_t e = rst[(size_t)r]; e < rst[(size_t)r + 1]; ++e) {
-        const int32_t p = fill[(size_t)col[(size_t)e]]++;
-        row[(size_t)p] = (int32_t)r;
-        std::memcpy(&tval[(size_t)p * es], &val[(size_t)e * es], es);
-      }
-    const std::vector<int32_t> blocks = rvvla::csr_row_blocks(ptr.data(), 0, n);
-    rvvla::DevCSR& t = A->tcsr;
-    t.rst = A->put_vec(ptr);
-    t.col = A->put_vec(row);
-    t.val = A->put(tval.data(), tval.size());
-    t.blocks = A->put_vec(blocks);
-    t.nblocks = (int64_t)blocks.size() / 2;
-    A->has_tcsr = true;
+    put_transposed_image(*A, rst.data(), col.data(), val.data(), es);
     return RVVLA_OK;
   });
 }
@@ -1991,10 +2047,6 @@ int rvvla_spmv_op(rvvla_mat_t A, int op, double alpha, const void* x, double bet
   if (op == 0) return rvvla_spmv(A, alpha, x, beta, y, stream);
   if (op != 1 || !A) return RVVLA_ERR_INVALID_ARG;
   if (int rc = need_device(A)) return rc;
-  if (A->format != RVVLA_FMT_CSR) {
-    rvvla::set_last_error("the transposed product is implemented for the CSR format");
-    return RVVLA_ERR_UNSUPPORTED_SPMV;
-  }
   if ((A->m > 0 && !x) || (A->n > 0 && !y)) return RVVLA_ERR_INVALID_ARG;
   if (int rc = build_transposed_image(A)) return rc;
   return rvvla::launch_csr_image(A->tcsr, A->n, A->dtype, alpha, x, beta, y, (hipStream_t)stream);

@@ -10,9 +10,11 @@
 // include/types.h:33-38): PREPARATION copies b and y to the device,
 // MULT runs the kernel, GET_RESULTS copies y back, ALL_STAGES does all
 // three.  Status codes are the RVVLA_* codes (0 = success).  The reference
-// ignores type_op; here SPARSE_OPERATION_TRANSPOSE computes A^T b for CRS
-// matrices and is answered with RVVLA_ERR_UNSUPPORTED_SPMV by the other
-// formats instead of computing the untransposed product.
+// ignores type_op; here SPARSE_OPERATION_TRANSPOSE computes A^T b: for CRS
+// from the matrix itself, for the other formats from the copy of the CRS
+// their convert keeps (a container filled some other way answers it with
+// RVVLA_ERR_UNSUPPORTED_SPMV rather than computing the untransposed
+// product).
 #include <omp.h>
 
 #include <algorithm>
@@ -46,6 +48,14 @@ int guard(F&& f) {
 template <class T>
 constexpr int dtype_of() {
   return std::is_same<T, double>::value ? RVVLA_F64 : RVVLA_F32;
+}
+
+// A converted matrix of a format other than CRS keeps a copy of its source
+// CRS for the transposed product (its CSR image of A^T is built from it on
+// first use).
+template <class T>
+void keep_source(const DeviceImage& d, const spMtxCRS<T>& s) {
+  if (d) rvvla::attach_transpose_source<T>(*d.get(), std::make_shared<const spMtxCRS<T>>(s));
 }
 
 // Device image of each container (uploaded on first use).
@@ -97,25 +107,12 @@ rvvla_mat_t image(const spMtxVNEC<T>& a) {
   return a.device.get();
 }
 
-template <class M>
-constexpr bool is_crs() {
-  return false;
-}
-template <>
-constexpr bool is_crs<spMtxCRS<double>>() {
-  return true;
-}
-template <>
-constexpr bool is_crs<spMtxCRS<float>>() {
-  return true;
-}
-
 template <class T, class M>
 sparse_matrix_status run_stage(sparse_matrix_mv_stage st, sparse_operation_t op, T alpha, const M& mat,
                                const std::vector<T>& b, T beta, std::vector<T>& y) {
   sparse_matrix_status status;
-  if (op != SPARSE_OPERATION_NON_TRANSPOSE && !(op == SPARSE_OPERATION_TRANSPOSE && is_crs<M>())) {
-    rvvla::set_last_error("SPARSE_OPERATION_TRANSPOSE is implemented for CRS matrices");
+  if (op != SPARSE_OPERATION_NON_TRANSPOSE && op != SPARSE_OPERATION_TRANSPOSE) {
+    rvvla::set_last_error("only SPARSE_OPERATION_NON_TRANSPOSE and SPARSE_OPERATION_TRANSPOSE");
     status.code = RVVLA_ERR_UNSUPPORTED_SPMV;
     return status;
   }
@@ -232,6 +229,7 @@ void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertPa
   if (!iparam(p, "device", 0) || C != 64) {
     rvvla::build_sell(d, s, C, sigma);
     d.device.reset(rvvla::upload_sell(dtype_of<T>(), d));
+    keep_source<T>(d.device, s);
     return;
   }
   const size_t m = s.m, nz = s.nz;
@@ -242,6 +240,7 @@ void convert_sell(spMtxSELL_C_Sigma<T>& d, const spMtxCRS<T>& s, const convertPa
                                   nullptr, &A);
   if (rc) throw Error(rc, rvvla_strerror(rc));
   d.device.reset(A);
+  keep_source<T>(d.device, s);
   d.m = s.m;
   d.n = s.n;
   d.nz = s.nz;
@@ -288,6 +287,7 @@ void convert_hcsr(spMtxHCSR<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   if (!iparam(p, "device", 0)) {
     rvvla::build_hcsr<T>(d, s, (size_t)R, (size_t)C, pp);
     d.device.reset(rvvla::upload_hcsr<T>(dtype_of<T>(), d));
+    keep_source<T>(d.device, s);
     return;
   }
   if (R <= 0 || C <= 0) throw Error(RVVLA_ERR_INVALID_ARG, "HCSR: R and C must be positive");
@@ -299,6 +299,7 @@ void convert_hcsr(spMtxHCSR<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   if (rc) throw Error(rc, rvvla_strerror(rc));
   d = spMtxHCSR<T>();
   d.device.reset(A);
+  keep_source<T>(d.device, s);
   d.R = (size_t)R;
   d.C = (size_t)C;
   d.m = s.m;
@@ -388,6 +389,7 @@ void convert_csr5(spMtxCSR5<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   if (!iparam(p, "device", 0)) {
     rvvla::build_csr5(d, s, omega, sigma);
     d.device.reset(rvvla::upload_csr5(dtype_of<T>(), d));
+    keep_source<T>(d.device, s);
     return;
   }
   if (omega != 64) throw Error(RVVLA_ERR_UNSUPPORTED_OMEGA, "CSR5: omega must be 64");
@@ -423,6 +425,7 @@ void convert_csr5(spMtxCSR5<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   d._temp_calibrator.assign((size_t)omp_get_max_threads() * (CSR5_CACHELINE / sizeof(T)), T(0));
   d.setCSR5Format();
   d.device.reset(H.release());
+  keep_source<T>(d.device, s);
 }
 
 // LAV keys "SIMD_Lanes" (64), "T" (double, 0.2), "segment_len"
@@ -434,6 +437,7 @@ void convert_lav(spMtxLAV<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
   if (!iparam(p, "device", 0) || lanes != 64) {
     rvvla::build_lav(d, s, lanes, Tv, seg);
     d.device.reset(rvvla::upload_lav(dtype_of<T>(), d));
+    keep_source<T>(d.device, s);
     return;
   }
   if (seg < 1) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: segment_len must be positive");
@@ -465,6 +469,7 @@ void convert_lav(spMtxLAV<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
   const std::vector<T> sval = read_vec<T, T>(H.A, "sparse_part.Val");
   d.sparse_part = spMtxCRS<T>(s.m, s.n, scol.size(), srst.data(), scol.data(), sval.data());
   d.device.reset(H.release());
+  keep_source<T>(d.device, s);
 }
 
 // CVR keys "SIMD_LEN" (64), "streams"
@@ -476,6 +481,7 @@ void convert_cvr(spMtxCVR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
   if (!iparam(p, "device", 0) || L != 64) {
     rvvla::build_cvr(d, s, L, S);
     d.device.reset(rvvla::upload_cvr(dtype_of<T>(), d, (int64_t)s.nz));
+    keep_source<T>(d.device, s);
     return;
   }
   if (S < 1) throw Error(RVVLA_ERR_INVALID_ARG, "CVR: at least one stream");
@@ -489,6 +495,15 @@ void convert_cvr(spMtxCVR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
   const std::vector<uint64_t> flush = read_vec<uint64_t, T>(H.A, "flush");
   const std::vector<int32_t> rec_rows = read_vec<int32_t, T>(H.A, "rec_rows");
   const std::vector<int32_t> tails = read_vec<int32_t, T>(H.A, "tail_rows");
+  // flush targets of cut rows name a partial-sum slot: its row
+  const std::vector<int32_t> srows = read_vec<int32_t, T>(H.A, "split_rows");
+  const std::vector<int32_t> sptr = read_vec<int32_t, T>(H.A, "split_ptr");
+  std::vector<int32_t> slot_row;
+  for (size_t i = 0; i < srows.size(); ++i) {
+    if ((size_t)sptr[2 * i + 1] > slot_row.size()) slot_row.resize((size_t)sptr[2 * i + 1], -1);
+    for (int32_t q = sptr[2 * i]; q < sptr[2 * i + 1]; ++q) slot_row[(size_t)q] = srows[i];
+  }
+  auto row_of = [&](int32_t target) { return target >= 0 ? target : slot_row[(size_t)(-target - 2)]; };
   d = spMtxCVR<T>();
   d.SIMD_LEN = 64;
   d.numRows = s.m;
@@ -508,12 +523,12 @@ void convert_cvr(spMtxCVR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
     for (int64_t k = ps[(size_t)t]; k < ps[(size_t)t + 1]; ++k)
       for (uint64_t f = flush[(size_t)k]; f; f &= f - 1) {
         d.vPack_vec_record.push_back((int)(k * 64 + __builtin_ctzll(f)));
-        d.vPack_vec_record.push_back(rec_rows[q++]);
+        d.vPack_vec_record.push_back(row_of(rec_rows[q++]));
       }
     for (int l = 0; l < 64; ++l)
-      if (tails[(size_t)t * 64 + l] >= 0) {
+      if (tails[(size_t)t * 64 + l] != -1) {
         d.vPack_vec_record.push_back((int)(ps[(size_t)t + 1] * 64 + l));
-        d.vPack_vec_record.push_back(tails[(size_t)t * 64 + l]);
+        d.vPack_vec_record.push_back(row_of(tails[(size_t)t * 64 + l]));
       }
   }
   d.vPack_vec_record.push_back((int)d.nItems);
@@ -528,6 +543,7 @@ void convert_cvr(spMtxCVR<T>& d, const spMtxCRS<T>& s, const convertParams& p) {
     for (int l = 0; l < 64; ++l) d.writeback_pos_record[2 + (size_t)l] = last < 0 ? 0 : last;
   }
   d.device.reset(H.release());
+  keep_source<T>(d.device, s);
 }
 
 // VHCC keys "vec_size" (64), "threads"
@@ -539,6 +555,7 @@ void convert_vhcc(spMtxVHCC<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   if (!iparam(p, "device", 0) || vec != 64) {
     rvvla::build_vhcc(d, s, vec, threads);
     d.device.reset(rvvla::upload_vhcc(dtype_of<T>(), d));
+    keep_source<T>(d.device, s);
     return;
   }
   if (threads < 1) throw Error(RVVLA_ERR_INVALID_ARG, "VHCC: at least one partition");
@@ -602,6 +619,7 @@ void convert_vhcc(spMtxVHCC<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   }
   veceor.push_back((uint32_t)nvec);
   d.device.reset(H.release());
+  keep_source<T>(d.device, s);
 }
 
 // VNEC key "threads" (merge-path partitions; default: as many rows+entries
@@ -613,6 +631,7 @@ void convert_vnec(spMtxVNEC<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   if (!iparam(p, "device", 0)) {
     rvvla::build_vnec(d, s, nparts);
     d.device.reset(rvvla::upload_vnec(dtype_of<T>(), d, s));
+    keep_source<T>(d.device, s);
     return;
   }
   if (nparts < 1) throw Error(RVVLA_ERR_INVALID_ARG, "VNEC: at least one partition");
@@ -644,6 +663,7 @@ void convert_vnec(spMtxVNEC<T>& d, const spMtxCRS<T>& s, const convertParams& p)
   d.use_x_indices = read_vec<int, T>(H.A, "use_x_indices");
   rvvla::vnec_select_variant(d, s);
   d.device.reset(H.release());
+  keep_source<T>(d.device, s);
 }
 
 }  // namespace

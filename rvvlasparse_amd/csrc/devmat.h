@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -306,6 +307,9 @@ struct rvvla_mat_s {
   rvvla::DevCSC csc;
   rvvla::DevCSR tcsr;  // CSR of the transpose (built on the first transposed product)
   bool has_tcsr = false;
+  // formats other than CSR: builds tcsr from the CSR the matrix was
+  // converted from (set by the conversions that have it)
+  std::function<void(rvvla_mat_s&)> transpose_source;
   // staging for rvvla_spmv_host
   void* xstage = nullptr;
   void* ystage = nullptr;

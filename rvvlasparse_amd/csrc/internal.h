@@ -119,6 +119,12 @@ template <class V>
 rvvla_mat_t upload_vnec(int dtype, const SparseMatrixLib::spMtxVNEC<V>& a,
                         const SparseMatrixLib::spMtxCRS<V>& src);
 
+// The transposed product of a matrix in another format runs on a CSR image
+// of A^T built, on its first use, from the CSR the matrix was converted
+// from: attach_transpose_source keeps `src` (shared) for that.
+template <class V>
+void attach_transpose_source(rvvla_mat_s& M, std::shared_ptr<const SparseMatrixLib::spMtxCRS<V>> src);
+
 // Adaptive CSR row blocks over rows [r0, r1) as (first, end) pairs: one
 // row of more than kLongRow entries, or up to kStreamRows rows holding at
 // most kStreamCap entries; the long-row blocks first, longest first, then

@@ -100,6 +100,34 @@ void check(const char* fmt, const char* form, const spMtxCRS<T>& a, const std::v
               ok ? "ok" : "FAIL");
 }
 
+// y = alpha * A^T b + beta * y0 against a plain transposed loop
+template <class T>
+void check_transpose(const char* fmt, const spMtxCRS<T>& a, const std::vector<T>& b,
+                     const std::vector<T>& y0, T alpha, T beta, const std::vector<T>& y,
+                     sparse_matrix_status st) {
+  const size_t n = a.n;
+  std::vector<double> ref(n, 0.0), sa(n, 0.0);
+  std::vector<int> len(n, 0);
+  for (size_t r = 0; r < a.m; ++r)
+    for (int e = a.Rst[r]; e < a.Rst[r + 1]; ++e) {
+      ref[a.Col[e]] += (double)a.Val[e] * b[r];
+      sa[a.Col[e]] += std::fabs((double)a.Val[e] * b[r]);
+      len[a.Col[e]]++;
+    }
+  const double eps = std::is_same<T, double>::value ? 1.1102230246251565e-16 : 5.960464477539063e-08;
+  double worst = 0;
+  for (size_t c = 0; c < n; ++c) {
+    const double want = alpha * ref[c] + beta * (double)y0[c];
+    const double scale = std::fabs(alpha) * sa[c] + std::fabs(beta) * std::fabs((double)y0[c]);
+    const double err = std::fabs((double)y[c] - want) / (4 * (len[c] + 2) * eps * (scale > 0 ? scale : 1));
+    if (err > worst) worst = err;
+  }
+  const bool ok = st.code == 0 && worst <= 1.0;
+  if (!ok) ++failures;
+  std::printf("%-5s %-6s A^T     status=%d err/bound=%.3f %s\n", fmt,
+              std::is_same<T, double>::value ? "double" : "float", st.code, worst, ok ? "ok" : "FAIL");
+}
+
 template <class T, template <class> class Fmt>
 void run(const char* name, const spMtxCRS<T>& a, const convertParams& P) {
   std::vector<T> b(a.n), y0(a.m);
@@ -122,14 +150,14 @@ void run(const char* name, const spMtxCRS<T>& a, const convertParams& P) {
   sparse_matrix_status s;
   s.code = s1.code ? s1.code : (s2.code ? s2.code : s3.code);
   check<T>(name, "staged", a, b, y0, alpha, beta, y, s);
-  // the transposed operation is refused (these are not CRS), not silently
-  // computed untransposed
-  auto st_t = sparse_mv<T, Fmt, true>(SPARSE_OPERATION_TRANSPOSE, alpha, d, sparse_matrix_descr{}, b,
-                                      beta, y);
-  if (st_t.code != RVVLA_ERR_UNSUPPORTED_SPMV) {
-    ++failures;
-    std::printf("%-5s transpose not refused (status %d)\n", name, st_t.code);
-  }
+  // the transposed operation: A^T b from the CRS the convert kept
+  std::vector<T> bt(a.m), yt0(a.n);
+  for (size_t i = 0; i < a.m; ++i) bt[i] = (T)std::sin(0.29 * (double)i);
+  for (size_t i = 0; i < a.n; ++i) yt0[i] = (T)std::cos(0.17 * (double)i);
+  std::vector<T> yt = yt0;
+  auto st_t = sparse_mv<T, Fmt, true>(SPARSE_OPERATION_TRANSPOSE, alpha, d, sparse_matrix_descr{}, bt,
+                                      beta, yt);
+  check_transpose<T>(name, a, bt, yt0, alpha, beta, yt, st_t);
 }
 
 // SPARSE_OPERATION_TRANSPOSE on CRS: y = alpha * A^T b + beta * y, for a

@@ -448,7 +448,7 @@ def test_device_vhcc_refuses_bad_csr():
 
 # ---------------------------------------------------------------- CVR
 CVR_FIELDS = ("pack_start", "rec_start", "flush", "rec_rows", "tail_rows", "col", "val",
-              "empty_rows")
+              "empty_rows", "split_rows", "split_ptr")
 
 
 @pytest.mark.parametrize("name", ["lap64", "powerlaw", "rand_long", "one_long_row", "empty",

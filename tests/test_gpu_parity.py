@@ -601,14 +601,60 @@ def test_spmv_transpose_16bit(dtype):
 
 
 @pytest.mark.parametrize("fmt", [f for f in FORMATS_READY if f != "csr"])
-def test_spmv_transpose_refused_elsewhere(fmt):
-    a = case("lap64")
-    M = R.convert(fmt, a, R.F64).upload()
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+@pytest.mark.parametrize("name", ["rand_long", "lap64", "powerlaw", "empty"])
+def test_spmv_transpose_formats(fmt, dtype, name):
+    """SPARSE_OPERATION_TRANSPOSE on every format converted from CSR: the
+    CSR image of A^T built from the kept source, against the oracle's CSR
+    product with the transpose (device and staged host paths)."""
+    a = case(name)
+    t = _transpose_csr(a)
     dev = _device()
+    rng = np.random.default_rng(14)
+    x64, y064 = rng.uniform(-1, 1, a.m), rng.uniform(-1, 1, a.n)
+    dt = np.float64 if dtype == R.F64 else np.float32
+    io = torch.float64 if dtype == R.F64 else torch.float32
+    print("transpose", fmt, name, dtype, flush=True)
+    M = R.convert(fmt, a, dtype).upload()
+    x = torch.from_numpy(x64).to(io).to(dev)
+    y = torch.from_numpy(y064).to(io).to(dev)
+    M.spmv_t(x, y, 0.5, -2.0)
+    torch.cuda.synchronize()
+    v, xx, y0 = t.values_for(dtype), x64.astype(dt), y064.astype(dt)
+    y_ref = O.csr_spmv(t.rst, t.col, v, xx, 0.5, -2.0, y0)
+    assert_within_bound(y.cpu().numpy(), y_ref, t.rst, t.col, v, xx, 0.5, -2.0, y0, dt,
+                        f"{fmt} A^T {name}")
+    yh = y0.copy()
+    R.check(R.lib.rvvla_spmv_host_op(M.handle, 1, 0.5, xx.ctypes.data, -2.0, yh.ctypes.data))
+    assert_within_bound(yh, y_ref, t.rst, t.col, v, xx, 0.5, -2.0, y0, dt, f"{fmt} A^T host {name}")
+
+
+@pytest.mark.parametrize("fmt", ["hcsr", "sell", "cvr"])
+def test_spmv_transpose_formats_16bit(fmt):
+    a = case("rand_long")
+    t = _transpose_csr(a)
+    dev = _device()
+    x64 = np.random.default_rng(15).uniform(-1, 1, a.m).astype(np.float32).astype(np.float64)
+    M = R.convert(fmt, a, R.BF16).upload()
+    x = torch.from_numpy(x64).float().to(dev)
+    y = torch.zeros(a.n, dtype=torch.float32, device=dev)
+    M.spmv_t(x, y)
+    y64 = O.csr_spmv(t.rst, t.col, t.val, x64, 1.0, 0.0, np.zeros(a.n))
+    rel = np.linalg.norm(y.cpu().numpy() - y64) / np.linalg.norm(y64)
+    assert rel <= 5e-3, rel
+
+
+def test_spmv_transpose_needs_source():
+    """A matrix uploaded from format arrays (no CSR kept) answers the
+    transposed product with RVVLA_ERR_UNSUPPORTED_SPMV."""
+    a = case("lap64")
+    dev = _device()
+    rst, col, val = (torch.from_numpy(v).to(dev) for v in (a.rst, a.col, a.val))
+    D = R.hcsr_from_device(R.F64, a.m, a.n, rst, col, val, 0, 0, 0.0)
     x = torch.zeros(a.m, dtype=torch.float64, device=dev)
     y = torch.zeros(a.n, dtype=torch.float64, device=dev)
     with pytest.raises(R.RvvlaError) as e:
-        M.spmv_t(x, y)
+        D.spmv_t(x, y)
     assert e.value.code == -4
 
 
