@@ -9,14 +9,20 @@ is the best FP64 format's throughput under the compulsory byte model
   B = nnz*(s_val + 4) + (m+1)*4 + n*s_x + m*s_y
 (the CSR-equivalent bytes, the same for every format).  The average kernel
 duration comes from HIP events recorded on the stream the kernels run on.
-The CPU baseline is the scalar OpenMP CSR restatement of the reference
-(oracle/), timed on this host's cores in the same run.
+The CPU baselines are the reference's own scalar OpenMP CSR (and, at the
+n = 65,536 configuration, HCSR) SpMVs compiled from its sources into
+oracle/_ref (the oracle's restatement where that library is absent), timed
+on this host's cores in the same run, with all cores and on one thread, for
+the Laplacian (FP64, FP32), the power-law matrix (FP64, FP32) and the
+n = 65,536 configuration; the power-law errors of the FP32 and 16-bit forms
+are stated against the reference's FP64 CPU product.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 2048]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import statistics
@@ -99,78 +105,129 @@ def copy_probe(torch, nbytes=1 << 30, steps=20):
     return {"bytes_moved": 2 * nbytes, "ms": round(ms, 4), "GBs": round(2 * nbytes / ms / 1e6, 1)}
 
 
-def cpu_baseline(a, x64, byts, seconds=10.0):
-    """The reference's scalar OpenMP CSR SpMV (reference
-    src/scalar/spmv_mtx_crs.cpp:22-71, compiled from its sources into
-    oracle/_ref with the reference's flags; kind "reference"), or where that
-    library is absent the oracle's restatement of it (kind "port"), FP64, on
-    the host cores, in a child process with OMP_NUM_THREADS = the physical
-    cores allowed to this process, OMP_PROC_BIND=close, OMP_PLACES=cores:
-    the median of as many full Laplacian SpMVs as fit in `seconds` (at least
-    20) after 5 warm-ups; then the same on one thread for about a third of
-    that time."""
+# CPU baseline jobs: (matrix, format, precision, seconds with all cores,
+# seconds on one thread); each is timed for at least 5 (all cores) or 3 (one
+# thread) runs
+CPU_JOBS = [("laplacian", "csr", "f64", 3.0, 1.5), ("laplacian", "csr", "f32", 2.0, 1.0),
+            ("powerlaw", "csr", "f64", 3.0, 2.0), ("powerlaw", "csr", "f32", 2.0, 1.5),
+            ("c1", "csr", "f64", 1.0, 0.5), ("c1", "hcsr", "f64", 1.0, 0.5)]
+
+
+def cpu_baseline(mats):
+    """The reference's scalar OpenMP SpMVs (reference src/scalar/spmv_mtx_crs.cpp
+    :22-71 and, for HCSR, src/scalar/crs_to_hcsr.cpp + spmv_mtx_hcsr.cpp
+    :104-134 at the reference's R = C = 2000, p = 2), compiled from its sources
+    into oracle/_ref with the reference's flags (kind "reference"), or where
+    that library is absent the oracle's CSR restatement (kind "port"; HCSR
+    jobs skipped), on the host cores, in a child process with OMP_NUM_THREADS
+    = the physical cores allowed to this process, OMP_PROC_BIND=close,
+    OMP_PLACES=cores: per job (CPU_JOBS) the median of the full SpMVs that
+    fit its budget after warm-ups, then the same on one thread.  `mats`:
+    name -> (CSR, x).  Returns the Laplacian FP64 figure as the headline, the
+    rest under "workloads", and the power-law FP64 result vector of the CPU
+    run (the reference's own FP64 CSR product) for the error figures."""
     cores = physical_cores(HOST_CPUS)
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     tmpdir = "/dev/shm" if os.path.isdir("/dev/shm") else None
     with tempfile.TemporaryDirectory(dir=tmpdir) as td:
-        path = os.path.join(td, "csr.npz")
-        np.savez(path, rst=a.rst, col=a.col, val=a.val, x=x64)
+        for name, (a, x) in mats.items():
+            for k, v in (("rst", a.rst), ("col", a.col), ("val", a.val), ("x", x)):
+                np.save(os.path.join(td, f"{name}.{k}.npy"), v)
         env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close",
                    OMP_PLACES="cores")
-        res = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-child", path,
-                              "--cpu-seconds", str(seconds)],
-                             env=env, stdout=subprocess.PIPE, text=True, timeout=600, check=True)
-    med, cnt, med1, cnt1, kind = json.loads(res.stdout.strip().splitlines()[-1])
-    return {"value": round(byts / med / 1e9, 2), "unit": "GB/s", "cores": cores, "kind": kind,
-            "sample": f"full Laplacian FP64 SpMV, median of {cnt} runs after 5 warm-ups "
-                      f"({med * 1e3:.3f} ms each, {cores} threads, one per physical core, "
-                      f"OMP_PROC_BIND=close)",
-            "single_thread": {"value": round(byts / med1 / 1e9, 2), "unit": "GB/s", "cores": 1,
-                              "sample": f"median of {cnt1} runs ({med1 * 1e3:.3f} ms each)"}}
+        res = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--cpu-child", td],
+                             env=env, stdout=subprocess.PIPE, text=True, timeout=900, check=True)
+        out = json.loads(res.stdout.strip().splitlines()[-1])
+        ypath = os.path.join(td, "powerlaw.y64.npy")
+        y_pl = np.load(ypath) if os.path.exists(ypath) else None
+    wl = {}
+    for job in out["jobs"]:
+        name, fmt, prec = job["job"]
+        a = mats[name][0]
+        sv = 8 if prec == "f64" else 4
+        byts = algo_bytes(a.m, a.n, a.nnz, sv, sv, sv)
+        wl[f"{name}/{fmt}/{prec}"] = {
+            "GBs": round(byts / job["med"] / 1e9, 2), "ms": round(job["med"] * 1e3, 4),
+            "runs": job["cnt"], "threads": cores,
+            "single_thread": {"GBs": round(byts / job["med1"] / 1e9, 2),
+                              "ms": round(job["med1"] * 1e3, 4), "runs": job["cnt1"]}}
+    head = wl["laplacian/csr/f64"]  # always present
+    lap = mats["laplacian"][0]
+    return {"value": head["GBs"], "unit": "GB/s", "cores": cores, "kind": out["kind"],
+            "sample": f"full Laplacian {int(np.sqrt(lap.m))}^2 FP64 CSR SpMV, median of "
+                      f"{head['runs']} runs after warm-ups ({head['ms']:.3f} ms each, {cores} "
+                      f"threads, one per physical core, OMP_PROC_BIND=close); other workloads "
+                      f"under 'workloads', each also on one thread",
+            "single_thread": {"value": head["single_thread"]["GBs"], "unit": "GB/s", "cores": 1,
+                              "sample": f"median of {head['single_thread']['runs']} runs "
+                                        f"({head['single_thread']['ms']:.3f} ms each)"},
+            "workloads": wl}, y_pl
 
 
-def cpu_child(path, seconds):
+def cpu_child(td):
     """The timed part of cpu_baseline (runs in the child process; no GPU)."""
     from oracle import oracle as O
     if O.orc is None:
         O.build(ref=False)
         import importlib
         importlib.reload(O)
-    d = np.load(path)
-    rst, col, val, x = (np.ascontiguousarray(d[k]) for k in ("rst", "col", "val", "x"))
-    m = len(rst) - 1
-    if O.ref is not None:  # the reference library: matrix and vectors built once
-        kind = "reference"
-        h = O.ref.ref_csr_bench_new_f64(m, len(x), len(col), rst.ctypes.data, col.ctypes.data,
-                                        val.ctypes.data, x.ctypes.data)
+    kind = "reference" if O.ref is not None else "port"
+    results = []
+    threads = int(os.environ.get("OMP_NUM_THREADS", "1"))
+    loaded = {}
+    for name, fmt, prec, secs, secs1 in CPU_JOBS:
+        if (fmt == "hcsr" and kind != "reference") or not os.path.exists(
+                os.path.join(td, f"{name}.rst.npy")):
+            continue
+        if name not in loaded:
+            loaded = {name: {k: np.load(os.path.join(td, f"{name}.{k}.npy"))
+                             for k in ("rst", "col", "val", "x")}}
+        d = loaded[name]
+        dt = np.float64 if prec == "f64" else np.float32
+        rst, col = np.ascontiguousarray(d["rst"]), np.ascontiguousarray(d["col"])
+        val, x = np.ascontiguousarray(d["val"].astype(dt)), np.ascontiguousarray(d["x"].astype(dt))
+        m = len(rst) - 1
+        if kind == "reference":
+            if fmt == "hcsr":
+                h = O.ref.ref_hcsr_bench_new_f64(m, len(x), len(col), rst.ctypes.data,
+                                                 col.ctypes.data, val.ctypes.data, x.ctypes.data,
+                                                 2000, 2000, 2.0)
+            else:
+                new = O.ref.ref_csr_bench_new_f64 if prec == "f64" else O.ref.ref_csr_bench_new_f32
+                h = new(m, len(x), len(col), rst.ctypes.data, col.ctypes.data, val.ctypes.data,
+                        x.ctypes.data)
 
-        def run():
-            O.ref.ref_csr_bench_run_f64(h)
-    else:
-        kind = "port"
-        y = np.zeros(m)
+            def run():
+                O.ref.ref_bench_run(h)
+        else:
+            y = np.zeros(m, dt)
 
-        def run():
-            O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
+            def run():
+                O.csr_spmv(rst, col, val, x, 1.0, 0.0, y)
 
-    def sample(threads, budget):
-        O.orc.orc_set_threads(threads)  # the process's OpenMP thread count
-        for _ in range(5):
-            run()
-        ts = []
-        t_end = time.perf_counter() + budget
-        while len(ts) < 20 or time.perf_counter() < t_end:
-            t0 = time.perf_counter()
-            run()
-            ts.append(time.perf_counter() - t0)
-        return statistics.median(ts), len(ts)
+        def sample(nthreads, budget, least):
+            O.orc.orc_set_threads(nthreads)  # the process's OpenMP thread count
+            for _ in range(2):
+                run()
+            ts = []
+            t_end = time.perf_counter() + budget
+            while len(ts) < least or time.perf_counter() < t_end:
+                t0 = time.perf_counter()
+                run()
+                ts.append(time.perf_counter() - t0)
+            return statistics.median(ts), len(ts)
 
-    med, cnt = sample(int(os.environ.get("OMP_NUM_THREADS", "1")), seconds)
-    med1, cnt1 = sample(1, seconds / 3)
-    if kind == "reference":
-        O.ref.ref_csr_bench_free(h)
-    print(json.dumps([med, cnt, med1, cnt1, kind]), flush=True)
+        med, cnt = sample(threads, secs, 5)
+        if kind == "reference" and (name, fmt, prec) == ("powerlaw", "csr", "f64"):
+            yv = np.ctypeslib.as_array(C.cast(O.ref.ref_bench_y(h), C.POINTER(C.c_double)), (m,))
+            np.save(os.path.join(td, "powerlaw.y64.npy"), yv.copy())
+        med1, cnt1 = sample(1, secs1, 3)
+        if kind == "reference":
+            O.ref.ref_csr_bench_free(h)
+        results.append({"job": [name, fmt, prec], "med": med, "cnt": cnt, "med1": med1,
+                        "cnt1": cnt1})
+    print(json.dumps({"kind": kind, "jobs": results}), flush=True)
 
 
 def powerlaw_leg(R, torch, steps, warmup, fmts):
@@ -178,9 +235,10 @@ def powerlaw_leg(R, torch, steps, warmup, fmts):
     to 64M entries, half within +-1024 of the diagonal, values N(0,1)): the
     FP64 formats, then FP32 and the 16-bit value forms (FP32 x/y and
     accumulation) in the best FP64 format.  Errors of the reduced forms are
-    stated against the FP64 result of the same format on the GPU (whose own
-    parity with the FP64 oracle is checked in the GPU tests): normwise
-    ||y - y64||_2 / ||y64||_2 and max-row max_i |y_i - y64_i| / ||y64||_inf."""
+    first stated against the FP64 result of the same format on the GPU;
+    main() replaces them by errors against the reference's FP64 CSR product
+    on the CPU when the CPU baseline runs.  Returns the leg's figures, the
+    matrix, x and the GPU results."""
     dev = torch.device("cuda:0")
     log("bench: generating power-law matrix")
     a = R.gen_powerlaw(4194304, 1.8, 50000, 64_000_000, 1024, seed=0)
@@ -216,16 +274,65 @@ def powerlaw_leg(R, torch, steps, warmup, fmts):
                           "frac": round(byts / ev_ms / 1e6 / PEAK_GBS, 4)}
         ys[tag] = y.double()
         del M
-    y64 = ys["f64"]
     for tag in ("f32", "bf16", "f16"):
-        d = ys[tag] - y64
-        per[best][tag]["normwise_rel_err"] = float(torch.linalg.norm(d) / torch.linalg.norm(y64))
-        per[best][tag]["maxrow_rel_err"] = float(d.abs().max() / y64.abs().max())
         per[best][tag]["speedup_vs_f64"] = round(per[best]["f64"]["ms"] / per[best][tag]["ms"], 3)
     out["formats"] = per
     out["bf16_vs_f64_speedup"] = per[best]["bf16"]["speedup_vs_f64"]
-    out["bf16_normwise_rel_err"] = per[best]["bf16"]["normwise_rel_err"]
-    return out
+    set_errors(out, {t: v.cpu().numpy() for t, v in ys.items()}, ys["f64"].cpu().numpy(),
+               "the FP64 result of the same format on the GPU")
+    return out, a, x64, {t: v.cpu().numpy() for t, v in ys.items()}
+
+
+def set_errors(out, ys, y64, against):
+    """Normwise ||y - y64||_2 / ||y64||_2 and max-row max_i |y_i - y64_i| /
+    ||y64||_inf of the FP32, BF16 and FP16 results against y64."""
+    best = out["best_format"]
+    for tag in ("f32", "bf16", "f16"):
+        d = ys[tag] - y64
+        out["formats"][best][tag]["normwise_rel_err"] = float(np.linalg.norm(d) / np.linalg.norm(y64))
+        out["formats"][best][tag]["maxrow_rel_err"] = float(np.abs(d).max() / np.abs(y64).max())
+    out["bf16_normwise_rel_err"] = out["formats"][best]["bf16"]["normwise_rel_err"]
+    out["errors_against"] = against
+
+
+def config1_leg(R, torch, steps, warmup):
+    """The reference's own CPU-path configuration on the GPU: n = 65,536, 16
+    entries per row (the diagonal and 15 distinct uniform columns), values
+    and x uniform[-1, 1], seed 0, FP64, CSR and HCSR (the reference's
+    R = C = 2000, p = 2).  Returns the timings, the matrix and x."""
+    dev = torch.device("cuda:0")
+    a = R.gen_random_rows(65536, 16, seed=0, include_diag=True)
+    x64 = R.gen_uniform(a.n, -1.0, 1.0, seed=0, purpose=3)
+    x = torch.from_numpy(x64).to(dev)
+    y = torch.zeros(a.m, dtype=torch.float64, device=dev)
+    byts = algo_bytes(a.m, a.n, a.nnz, 8, 8, 8)
+    gpu = {}
+    for fmt in ("csr", "hcsr"):
+        M = R.convert(fmt, a, R.F64).upload()
+        ev_ms, wall_ms = time_format(M, x, y, steps, warmup, torch)
+        gpu[fmt] = {"us": round(ev_ms * 1e3, 2), "wall_us": round(wall_ms * 1e3, 2),
+                    "GBs": round(byts / ev_ms / 1e6, 1)}
+        del M
+    return {"workload": f"n={a.m}, nnz={a.nnz} (diagonal + 15 uniform columns per row), FP64",
+            "gpu": gpu}, a, x64
+
+
+def read_probe(torch, nbytes=1 << 31, steps=20):
+    """Achievable read rate: torch's sum over a 2 GiB FP32 buffer (reads the
+    buffer once, writes a few bytes), HIP events."""
+    buf = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        buf.sum()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        buf.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    del buf
+    return {"bytes_read": nbytes, "ms": round(ms, 4), "GBs": round(nbytes / ms / 1e6, 1)}
 
 
 def autograd_leg(R, torch, steps, warmup):
@@ -325,10 +432,9 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy probe (counter calibration)")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_child:
-        cpu_child(args.cpu_child, args.cpu_seconds)
+        cpu_child(args.cpu_child)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -355,18 +461,24 @@ def main():
     if tf.exists():
         traffic_db = json.loads(tf.read_text())
 
+    # each format with the reference's parameters, plus HCSR with blocks of
+    # 2048 x 2048 (the grid's row length: a block-row's neighbours fall in
+    # whole blocks)
+    variants = [(f, f, "") for f in fmts]
+    if "hcsr" in fmts:
+        variants.append(("hcsr_R2048", "hcsr", "R=2048,C=2048"))
     results = {}
     for dtype, tag, s in ((R.F64, "f64", 8), (R.F32, "f32", 4)):
         io = torch.float64 if dtype == R.F64 else torch.float32
         x = torch.from_numpy(x64).to(io).to(dev)
         y = torch.zeros(a.m, dtype=io, device=dev)
         byts = algo_bytes(a.m, a.n, a.nnz, s, s, s)
-        for fmt in fmts:
-            log(f"bench: {fmt} {tag}: convert + upload")
-            M = R.convert(fmt, a, dtype).upload()
-            log(f"bench: {fmt} {tag}: timing {args.warmup}+{args.steps}")
+        for name, fmt, params in variants:
+            log(f"bench: {name} {tag}: convert + upload")
+            M = R.convert(fmt, a, dtype, params).upload()
+            log(f"bench: {name} {tag}: timing {args.warmup}+{args.steps}")
             ev_ms, wall_ms = time_format(M, x, y, args.steps, args.warmup, torch)
-            results.setdefault(fmt, {})[tag] = {
+            results.setdefault(name, {})[tag] = {
                 "ms": round(ev_ms, 5), "wall_ms": round(wall_ms, 5),
                 "GBs": round(byts / (ev_ms * 1e-3) / 1e9, 1),
                 "frac": round(byts / (ev_ms * 1e-3) / 1e9 / PEAK_GBS, 4),
@@ -395,14 +507,27 @@ def main():
         "formats": results,
     }
     line["copy_probe"] = copy_probe(torch)
+    line["read_probe"] = read_probe(torch)
+    mats = {"laplacian": (a, x64)}
+    log("bench: config-1 (n = 65,536, 16 per row) CSR and HCSR")
+    line["config1"], a1, x1 = config1_leg(R, torch, args.steps, args.warmup)
+    mats["c1"] = (a1, x1)
+    pl_ys = None
     if not args.no_powerlaw:
-        line["powerlaw"] = powerlaw_leg(R, torch, args.steps, args.warmup, fmts)
+        line["powerlaw"], apl, xpl, pl_ys = powerlaw_leg(R, torch, args.steps, args.warmup, fmts)
+        mats["powerlaw"] = (apl, xpl)
     if not args.no_extra:
         line["autograd_spmm"] = autograd_leg(R, torch, min(args.steps, 20), 3)
         line["batched"] = batch_leg(R, torch, min(args.steps, 20), 3)
     if not args.no_cpu:
-        log("bench: CPU baseline")
-        line["cpu_baseline"] = cpu_baseline(a, x64, byts64)
+        log("bench: CPU baselines")
+        line["cpu_baseline"], y_ref = cpu_baseline(mats)
+        wl = line["cpu_baseline"]["workloads"]
+        line["config1"]["cpu_reference"] = {k.split("/")[1]: v for k, v in wl.items()
+                                            if k.startswith("c1/")}
+        if pl_ys is not None and y_ref is not None:
+            set_errors(line["powerlaw"], pl_ys, y_ref,
+                       "the reference's FP64 CSR product on the CPU (oracle/_ref)")
     print(json.dumps(line), flush=True)
 
 

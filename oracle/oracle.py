@@ -95,7 +95,11 @@ _REF_SIGS = {
     "ref_sell_spmv_f32": (_i32, [_i64, _i64, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp,
                                  _vp, _vp, _f, _f, _vp]),
     "ref_csr_bench_new_f64": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "ref_csr_bench_new_f32": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "ref_hcsr_bench_new_f64": (_vp, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _i32, _i32, _d]),
     "ref_csr_bench_run_f64": (_i32, [_vp]),
+    "ref_bench_run": (_i32, [_vp]),
+    "ref_bench_y": (_vp, [_vp]),
     "ref_csr_bench_free": (None, [_vp]),
 }
 

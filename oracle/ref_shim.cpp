@@ -24,6 +24,22 @@ struct HcsrBox {
   spMtxHCSR<T> h;
 };
 
+struct RefBenchBase {
+  virtual ~RefBenchBase() = default;
+  virtual void run() = 0;
+  virtual const void* y() const = 0;
+};
+template <class T, template <class> class Fmt>
+struct RefBench : RefBenchBase {
+  Fmt<T> a;
+  std::vector<T> b, yv;
+  void run() override {
+    sparse_mv<T, Fmt, false>(SPARSE_OPERATION_NON_TRANSPOSE, (T)1, a, sparse_matrix_descr{}, b,
+                             (T)0, yv);
+  }
+  const void* y() const override { return yv.data(); }
+};
+
 }  // namespace
 
 extern "C" {
@@ -48,26 +64,51 @@ int ref_csr_spmv_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const in
   return 0;
 }
 
-// The reference CSR SpMV on a matrix and vectors built once, for timing
-// (bench.py's CPU baseline): new (copies the matrix and b in), run (one
-// y = A*b through sparse_mv<double, spMtxCRS, false>, reference
-// src/scalar/spmv_mtx_crs.cpp:22-71), free.
-struct RefCsrBench {
-  spMtxCRS<double> a;
-  std::vector<double> b, y;
-};
+// The reference CSR and HCSR SpMVs on a matrix and vectors built once, for
+// timing (bench.py's CPU baselines): new (copies the matrix and b in; HCSR:
+// converts through convert<T, spMtxHCSR, false> with R, C, p), run (one
+// y = A*b through sparse_mv<T, Fmt, false>, reference
+// src/scalar/spmv_mtx_crs.cpp:22-71 and src/scalar/spmv_mtx_hcsr.cpp
+// :104-134), y (the last result), free.
 void* ref_csr_bench_new_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
                             const double* val, const double* b) {
-  return new RefCsrBench{make_crs<double>(m, n, nnz, rst, col, val),
-                         std::vector<double>(b, b + n), std::vector<double>((size_t)m, 0.0)};
+  auto* p = new RefBench<double, spMtxCRS>;
+  p->a = make_crs<double>(m, n, nnz, rst, col, val);
+  p->b.assign(b, b + n);
+  p->yv.assign((size_t)m, 0.0);
+  return static_cast<RefBenchBase*>(p);
+}
+void* ref_csr_bench_new_f32(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                            const float* val, const float* b) {
+  auto* p = new RefBench<float, spMtxCRS>;
+  p->a = make_crs<float>(m, n, nnz, rst, col, val);
+  p->b.assign(b, b + n);
+  p->yv.assign((size_t)m, 0.0f);
+  return static_cast<RefBenchBase*>(p);
+}
+void* ref_hcsr_bench_new_f64(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col,
+                             const double* val, const double* b, int R, int C, double p) {
+  auto* q = new RefBench<double, spMtxHCSR>;
+  convertParams P;
+  P.param["R"].i = R;
+  P.param["C"].i = C;
+  P.param["p"].d = p;
+  auto crs = make_crs<double>(m, n, nnz, rst, col, val);
+  convert<double, spMtxHCSR, false>(q->a, crs, P);
+  q->b.assign(b, b + n);
+  q->yv.assign((size_t)m, 0.0);
+  return static_cast<RefBenchBase*>(q);
 }
 int ref_csr_bench_run_f64(void* h) {
-  auto* p = static_cast<RefCsrBench*>(h);
-  sparse_mv<double, spMtxCRS, false>(SPARSE_OPERATION_NON_TRANSPOSE, 1.0, p->a,
-                                     sparse_matrix_descr{}, p->b, 0.0, p->y);
+  static_cast<RefBenchBase*>(h)->run();
   return 0;
 }
-void ref_csr_bench_free(void* h) { delete static_cast<RefCsrBench*>(h); }
+int ref_bench_run(void* h) {
+  static_cast<RefBenchBase*>(h)->run();
+  return 0;
+}
+const void* ref_bench_y(void* h) { return static_cast<RefBenchBase*>(h)->y(); }
+void ref_csr_bench_free(void* h) { delete static_cast<RefBenchBase*>(h); }
 
 // reference src/scalar/crs_to_hcsr.cpp:137-169 through convert<T, spMtxHCSR, false>.
 // Returns a handle; *err = 0, or -10/-11/-12 for the three exception kinds.
