@@ -371,10 +371,14 @@ def autograd_leg(R, torch, steps, warmup):
     f_ms /= steps
     b_ms /= steps
     fb = a.nnz * 8 + (n + 1) * 4 + 2 * n * k * 4  # vals+cols, row ptr, X and Y
+    # the columns are uniform over 2^20 rows of X (268 MB, more than the
+    # caches hold): every entry gathers its own k-row of X
+    gb = a.nnz * (8 + k * 4) + (n + 1) * 4 + n * k * 4
     return {"workload": f"SpMM n={n}, nnz={a.nnz}, k={k}, FP32", "forward_ms": round(f_ms, 4),
             "backward_ms": round(b_ms, 4), "backward_over_forward": round(b_ms / f_ms, 3),
             "forward_GFLOPs": round(2 * a.nnz * k / f_ms / 1e6, 1),
-            "forward_compulsory_GBs": round(fb / f_ms / 1e6, 1)}
+            "forward_compulsory_GBs": round(fb / f_ms / 1e6, 1),
+            "forward_gather_GBs": round(gb / f_ms / 1e6, 1)}
 
 
 def batch_leg(R, torch, steps, warmup, count=512):

@@ -316,6 +316,16 @@ int rvvla_spmm_transpose(rvvla_mat_t A, const void* vals, int k, double alpha, c
                          int64_t ldx, double beta, void* Y, int64_t ldy, void* stream);
 int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy, const void* X,
                 int64_t ldx, void* out_vals, void* stream);
+/* The backward of Y = A(vals) X in one pass over A's columns (no reference
+ * counterpart): dX = A(vals)^T DY (n x k, leading dimension lddx, written,
+ * not accumulated) and dvals[e] = <DY[row_e, :], X[col_e, :]> (CSR order),
+ * each DY row gathered once for both; either output may be NULL.  Builds
+ * the cached CSC structure on first use (as rvvla_spmm_transpose).  Same
+ * results as rvvla_spmm_transpose (alpha 1, beta 0) plus rvvla_sddmm
+ * (alpha 1), up to the order of additions. */
+int rvvla_spmm_backward(rvvla_mat_t A, const void* vals, int k, const void* DY, int64_t lddy,
+                        const void* X, int64_t ldx, void* dX, int64_t lddx, void* dvals,
+                        void* stream);
 
 /* ---- batched uneven SpMM in one persistent launch ----------------------
  * No reference counterpart.  `count` CSR matrices (host arrays; values

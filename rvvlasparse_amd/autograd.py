@@ -5,9 +5,10 @@ CSC permutation used by the transposed product); ``spmm(A, theta, X)`` is a
 ``torch.autograd.Function`` whose backward computes
   dX     = A(theta)^T dY        (gather over the cached CSC permutation, no atomics)
   dtheta = <dY[row_e, :], X[col_e, :]> for every entry e   (sampled dense product)
-with hand-written HIP kernels (rvvla_spmm_values / rvvla_spmm_transpose /
-rvvla_sddmm).  FP32 and FP64.  No reference counterpart: the reference
-multiplies one vector with fixed values.
+with hand-written HIP kernels: rvvla_spmm_values forward, and one pass over
+A's columns for both gradients (rvvla_spmm_backward: each dY row gathered
+once for dX and dtheta).  FP32 and FP64.  No reference counterpart: the
+reference multiplies one vector with fixed values.
 """
 from __future__ import annotations
 
@@ -24,6 +25,7 @@ for _name, _args in {
     "rvvla_spmm_values": [_vp, _vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp],
     "rvvla_spmm_transpose": [_vp, _vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp],
     "rvvla_sddmm": [_vp, _i32, _d, _vp, _i64, _vp, _i64, _vp, _vp],
+    "rvvla_spmm_backward": [_vp, _vp, _i32, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp],
 }.items():
     _f = getattr(lib, _name)
     _f.restype, _f.argtypes = _i32, _args
@@ -83,16 +85,15 @@ class _SpMM(torch.autograd.Function):
         A = ctx.A
         dY = dY.contiguous()
         k = X.shape[1]
-        dtheta = dX = None
-        if ctx.needs_input_grad[1]:
-            dX = torch.empty_like(X)
-            check(lib.rvvla_spmm_transpose(A.handle, theta.data_ptr(), k, 1.0, dY.data_ptr(),
-                                           dY.stride(0), 0.0, dX.data_ptr(), dX.stride(0),
-                                           _stream(dY)))
-        if ctx.needs_input_grad[0]:
-            dtheta = torch.empty_like(theta)
-            check(lib.rvvla_sddmm(A.handle, k, 1.0, dY.data_ptr(), dY.stride(0), X.data_ptr(),
-                                  X.stride(0), dtheta.data_ptr(), _stream(dY)))
+        dtheta = torch.empty_like(theta) if ctx.needs_input_grad[0] else None
+        dX = torch.empty_like(X) if ctx.needs_input_grad[1] else None
+        if dtheta is not None or dX is not None:
+            check(lib.rvvla_spmm_backward(A.handle, theta.data_ptr(), k, dY.data_ptr(),
+                                          dY.stride(0), X.data_ptr(), X.stride(0),
+                                          dX.data_ptr() if dX is not None else None,
+                                          dX.stride(0) if dX is not None else k,
+                                          dtheta.data_ptr() if dtheta is not None else None,
+                                          _stream(dY)))
         return dtheta, dX, None
 
 

@@ -2124,6 +2124,25 @@ int rvvla_spmm_transpose(rvvla_mat_t A, const void* vals, int k, double alpha, c
                                    ldy, (hipStream_t)stream);
 }
 
+int rvvla_spmm_backward(rvvla_mat_t A, const void* vals, int k, const void* DY, int64_t lddy,
+                        const void* X, int64_t ldx, void* dX, int64_t lddx, void* dvals,
+                        void* stream) {
+  if (int rc = need_csr_real(A)) return rc;
+  if (k < 0 || lddy < k || ldx < k || (dX && lddx < k) ||
+      (k > 0 && A->nnz > 0 && (!DY || (dvals && !X))) || (k > 0 && A->n > 0 && dX && !DY))
+    return RVVLA_ERR_INVALID_ARG;
+  if (!dX && !dvals) return RVVLA_OK;
+  if (k == 0) {  // no columns: dX is n x 0, and every <DY[r, :], X[c, :]> is 0
+    if (dvals && A->nnz > 0)
+      return rvvla::hip_check(hipMemsetAsync(dvals, 0, (size_t)A->nnz * (A->dtype == RVVLA_F64 ? 8 : 4),
+                                             (hipStream_t)stream), "clear dvals");
+    return RVVLA_OK;
+  }
+  if (int rc = rvvla_prepare_transpose(A)) return rc;
+  return rvvla::launch_spmm_backward(*A, vals ? vals : A->csr.val, k, DY, lddy, X, ldx, dX, lddx,
+                                     dvals, (hipStream_t)stream);
+}
+
 int rvvla_sddmm(rvvla_mat_t A, int k, double alpha, const void* DY, int64_t lddy, const void* X,
                 int64_t ldx, void* out_vals, void* stream) {
   if (int rc = need_csr_real(A)) return rc;

@@ -425,6 +425,9 @@ int launch_lav(const rvvla_mat_s& A, double alpha, const void* x, double beta, v
                hipStream_t s, void* scratch = nullptr);
 int launch_vnec(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                 hipStream_t s, void* scratch = nullptr);
+int launch_spmm_backward(const rvvla_mat_s& A, const void* vals, int k, const void* DY,
+                         int64_t lddy, const void* X, int64_t ldx, void* dX, int64_t lddx,
+                         void* dvals, hipStream_t s);
 int launch_spmm_values(const rvvla_mat_s& A, bool transpose, const void* vals, int k, double alpha,
                        const void* X, int64_t ldx, double beta, void* Y, int64_t ldy,
                        hipStream_t s);

@@ -109,3 +109,38 @@ def test_spmm_transpose_and_sddmm_odd_widths(k):
     dtheta_ref, dX_ref = _oracle_grads(a, X, W)
     np.testing.assert_allclose(theta.grad.cpu().numpy(), dtheta_ref, rtol=0, atol=1e-12)
     np.testing.assert_allclose(Xt.grad.cpu().numpy(), dX_ref, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("k", [64, 16, 8, 5, 1])
+@pytest.mark.parametrize("which", ["both", "dX", "dtheta"])
+def test_spmm_backward_fused_against_oracle(dtype, k, which):
+    """rvvla_spmm_backward (one pass over the columns for dX and dtheta; k = 5
+    takes the two separate passes) against the oracle's gradients, with
+    either output alone too; rows of A with no entries in a column range and
+    empty columns included (n = 1500, 3..12 entries per row)."""
+    import ctypes as C
+    from rvvlasparse_amd.autograd import SparseStructure, lib
+    a = R.gen_random_rows_var(1500, 3, 12, seed=k)
+    rng = np.random.default_rng(k)
+    X = rng.normal(size=(a.n, k))
+    W = rng.normal(size=(a.m, k))
+    dev = torch.device("cuda:0")
+    A = SparseStructure(a.m, a.n, a.rst, a.col, dtype)
+    theta = torch.tensor(a.val, dtype=dtype, device=dev)
+    Xt = torch.tensor(X, dtype=dtype, device=dev)
+    Wt = torch.tensor(W, dtype=dtype, device=dev)
+    dX = torch.full((a.n, k), float("nan"), dtype=dtype, device=dev) if which != "dtheta" else None
+    dth = torch.full((a.nnz,), float("nan"), dtype=dtype, device=dev) if which != "dX" else None
+    print(f"fused backward {dtype} k={k} {which}", flush=True)
+    R.check(lib.rvvla_spmm_backward(A.handle, theta.data_ptr(), k, Wt.data_ptr(), k, Xt.data_ptr(), k,
+                                    dX.data_ptr() if dX is not None else None, k,
+                                    dth.data_ptr() if dth is not None else None, None))
+    torch.cuda.synchronize()
+    dtheta_ref, dX_ref = _oracle_grads(a, X, W)
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    if dX is not None:
+        np.testing.assert_allclose(dX.cpu().numpy(), dX_ref, rtol=0, atol=tol * np.abs(dX_ref).max())
+    if dth is not None:
+        np.testing.assert_allclose(dth.cpu().numpy(), dtheta_ref, rtol=0,
+                                   atol=tol * np.abs(dtheta_ref).max())
