@@ -310,7 +310,7 @@ static bool choose_xlds(const rvvla_mat_s& M, const spMtxCRS<V>& a, int request,
 }
 
 template <class V>
-rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, int sell) {
+rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, int sell, int xlcomb) {
   check_dtype_for(dtype, std::is_same<V, double>::value);
   check_csr((int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, a.Rst, a.Col);
   auto M = new_mat(RVVLA_FMT_CSR, dtype, (int64_t)a.m, (int64_t)a.n, (int64_t)a.nz);
@@ -325,13 +325,18 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, i
     const int io = dtype == RVVLA_F64 ? 8 : 4;
     const int64_t W = xl_slice_cols(io);
     // the combine's run table holds (blocks + 1) * slices offsets
-    const int64_t est_blocks = (int64_t)a.m / kXlCombRows + 2 * (int64_t)a.nz / xl_comb_pieces(io) + 2;
+    const int cwg = xlcomb > 0 ? xlcomb : kXlCombWG;
+    if (cwg != 256 && cwg != 512 && cwg != 1024)
+      throw Error(RVVLA_ERR_INVALID_ARG, "CSR: xlcomb must be 256, 512 or 1024");
+    const int64_t crows = xl_comb_rows(cwg), cpieces = xl_comb_pieces(io, cwg);
+    const int64_t est_blocks = (int64_t)a.m / crows + 2 * (int64_t)a.nz / cpieces + 2;
     if (est_blocks * (((int64_t)a.n + W - 1) / W) > (int64_t(1) << 27))
       throw Error(RVVLA_ERR_UNSUPPORTED_SPMV, "CSR: LDS-sliced image run table too large");
     const int vpw = dtype == RVVLA_F64 ? 2 : dtype == RVVLA_F32 ? 4 : 8;
-    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, W, device_cus(*M),
-                  kXlCombRows, xl_comb_pieces(io), vpw);
+    build_xlds<V>(X, (int64_t)a.m, (int64_t)a.n, a.Rst, a.Col, a.Val, W, device_cus(*M), crows,
+                  cpieces, vpw);
     DevXLds& d = M->csr.xl;
+    d.cwg = cwg;
     d.S = X.S;
     d.W = X.W;
     d.n = (int64_t)a.n;
@@ -1045,8 +1050,8 @@ template rvvla_mat_t upload_vnec<float>(int, const spMtxVNEC<float>&, const spMt
 
 #define RVVLA_INST_UPLOAD(F, T) template rvvla_mat_t F<double>(int, const T<double>&); \
   template rvvla_mat_t F<float>(int, const T<float>&);
-template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int, int, int);
-template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int, int, int);
+template rvvla_mat_t upload_csr<double>(int, const spMtxCRS<double>&, int, int, int, int);
+template rvvla_mat_t upload_csr<float>(int, const spMtxCRS<float>&, int, int, int, int);
 RVVLA_INST_UPLOAD(upload_hcsr, spMtxHCSR)
 RVVLA_INST_UPLOAD(upload_sell, spMtxSELL_C_Sigma)
 RVVLA_INST_UPLOAD(upload_csr5, spMtxCSR5)
@@ -1097,7 +1102,8 @@ void convert_into(rvvla_layout_s& L, int format, int dtype, std::shared_ptr<spMt
       const int xs = param_int(P, "xslices", -1);
       const int xl = param_int(P, "xlds", -1);
       const int sl = param_int(P, "sell", -1);
-      L.upload = [a, dtype, xs, xl, sl]() { return upload_csr<V>(dtype, *a, xs, xl, sl); };
+      const int xc = param_int(P, "xlcomb", 0);
+      L.upload = [a, dtype, xs, xl, sl, xc]() { return upload_csr<V>(dtype, *a, xs, xl, sl, xc); };
       break;
     }
     case RVVLA_FMT_HCSR: {

@@ -14,14 +14,16 @@ constexpr int kLongRow = 1024;
 
 // LDS-sliced image (csr_xlds.cpp): x slices of kXlLdsBytes in LDS (16384
 // doubles or 32768 floats), tasks of kXlTask slice-major entries (one
-// wavefront each, 8 entries a lane), combine blocks of at most kXlCombRows
-// rows holding at most xl_comb_pieces partials (64 KiB of LDS), or one row
-// with more.
+// wavefront each, 8 entries a lane).  The combine runs workgroups of cwg
+// threads (256, 512 or 1024; kXlCombWG by default) over blocks of at most
+// xl_comb_rows(cwg) rows holding at most xl_comb_pieces(acc, cwg) partials
+// (64 bytes of LDS a thread), or one row with more.
 constexpr int kXlLdsBytes = 131072;
 constexpr int kXlTask = 512;
 constexpr int xl_slice_cols(int io_bytes) { return kXlLdsBytes / io_bytes; }
-constexpr int kXlCombRows = 2048;
-constexpr int xl_comb_pieces(int acc_bytes) { return 65536 / acc_bytes; }
+constexpr int kXlCombWG = 1024;
+constexpr int xl_comb_rows(int cwg) { return 2 * cwg; }
+constexpr int xl_comb_pieces(int acc_bytes, int cwg) { return 64 * cwg / acc_bytes; }
 
 // HCSR image (hcsr_image.cpp): ints per unit record (row0, nrows, first
 // and end slot row, first and end tail entry, partial slot or -1, 0) and

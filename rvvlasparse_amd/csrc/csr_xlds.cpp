@@ -233,7 +233,7 @@ void build_xlds(XLdsHost<V>& X, int64_t m, int64_t n, const int* rst, const int*
   const int64_t nb = (int64_t)X.cblk.size() / 2;
   // ---- slot runs of each (block, slice) and the row-major places
   X.off.assign((size_t)(nb + 1) * S, -1);
-  X.lpos.assign((size_t)npieces, 0);
+  X.lpos.assign((size_t)std::max<int64_t>(npieces, 1), 0);  // never empty: the combine reads slot 0 when idle
 #pragma omp parallel for schedule(dynamic, 16)
   for (int64_t b = 0; b < nb; ++b) {
     const int64_t r0 = X.cblk[(size_t)(2 * b)], r1 = X.cblk[(size_t)(2 * b + 1)];

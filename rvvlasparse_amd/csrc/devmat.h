@@ -33,10 +33,11 @@ struct DevXSlice {
 // (one workgroup per CU, its LDS holding one slice of x) runs
 // tasks of kXlTask entries, one partial per piece; the combine adds each
 // row's partials in slice order.
-constexpr int kXcWG = 1024;  // threads of the combine kernel
-constexpr int kXcMaxSlices = kXcWG;  // slices the combine scans in LDS
+// cwg: threads of a combine workgroup (256, 512 or 1024; combine blocks
+// sized for it, see csr_blocks.h).
 struct DevXLds {
   int64_t S = 0, W = 0, npieces = 0, ntasks = 0, nunits = 0, n = 0, nwg = 0, nblocks = 0, nent = 0;
+  int cwg = kXlCombWG;
   const uint16_t* col = nullptr;
   const void* val = nullptr;
   const int32_t* task = nullptr;

@@ -99,10 +99,11 @@ void build_hcsr_image(HcsrImage<V>& im, int64_t m, int64_t n, int64_t R, int64_t
 // LDS-sliced image (-1 = chosen from the matrix, 0 = off, 1 = on; it takes
 // the place of the sliced image).  sell: the natural-order SELL-64 image
 // for matrices with even row lengths (-1 = chosen from the matrix, 0 = off,
-// 1 = on; only where no sliced image is used).
+// 1 = on; only where no sliced image is used).  xlcomb: threads of the
+// LDS-sliced image's combine workgroups (256, 512 or 1024; 0 = default).
 template <class V>
 rvvla_mat_t upload_csr(int dtype, const SparseMatrixLib::spMtxCRS<V>& a, int xslices = -1,
-                       int xlds = -1, int sell = -1);
+                       int xlds = -1, int sell = -1, int xlcomb = 0);
 template <class V>
 rvvla_mat_t upload_hcsr(int dtype, const SparseMatrixLib::spMtxHCSR<V>& a);
 template <class V>

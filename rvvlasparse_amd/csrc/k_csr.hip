@@ -198,7 +198,8 @@ __device__ __forceinline__ int lanes_below(uint64_t m) {
 // Launch shape of the slice pass: FP64 (x slice of 16384 doubles) runs 8
 // wavefronts a CU, each with 4 tasks in flight; the 32-bit-x forms run 16
 // wavefronts with one task ahead.  Both stage a task's partials in LDS, so
-// that they leave in whole lines.
+// that they leave in whole lines (writing them from the lanes straight to
+// their slots, with 16 FP64 wavefronts a CU, measured slower).
 template <class VT>
 struct XlShape {
   static constexpr bool F64 = sizeof(VT) == 8;
@@ -297,12 +298,12 @@ __device__ __forceinline__ int xl_lane_prefix(int v, int lane, int& total) {
   return s - v;
 }
 
-// One task.  The lane's walk has no branches: at every entry the running
-// sum of the lane's open piece is written to that piece's stage slot, so
-// the last write a lane makes to a slot is its complete share of the piece;
-// a piece that continues into higher lanes is written again, complete, by
-// the lane that closes it (after every walk write, so it wins).  The stage
-// then goes out in slot order.
+// One task.  The lane's walk writes the running sum of its open piece to
+// that piece's stage slot where the piece leaves the lane (the next entry
+// opens another piece, or the lane's last entry), so each write is the
+// lane's complete share of the piece; a piece that continues into higher
+// lanes is written again, complete, by the lane that closes it (after
+// every walk write, so it wins).  The stage then goes out in slot order.
 template <class VT, bool COMP>
 __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename Val<VT>::io* xs,
                                         typename Val<VT>::acc* __restrict__ part,
@@ -356,7 +357,9 @@ __device__ __forceinline__ void xl_task(const XlTaskData<VT>& d, const typename 
     seen = seen || f;
     slot += f ? 1 : 0;
     run.add(prod[k]);
-    stg[xl_swz(slot)] = run.value();
+    // the piece's share leaves when the next entry opens another piece
+    // (or at the lane's last entry)
+    if (k == 7 || fl[k < 7 ? k + 1 : 7]) stg[xl_swz(slot)] = run.value();
   }
   // segmented inclusive scan of the open sums (lane 0 always holds a mark)
   const Ac S = wave_seg_scan(run, __ballot(seen), lane);
@@ -433,20 +436,25 @@ __global__ __launch_bounds__(XlShape<VT>::WG) void k_xl_slices(
   }
 }
 
-// Combine of the LDS-sliced image: block b (rows r0 .. r1-1) takes from
-// each slice p its run of slots off[b S + p] .. off[(b+1) S + p] - 1.  The
-// run lengths are scanned in LDS, numbering the block's partials 0 .. cnt-1
-// in slice order; thread t takes partials t, t + kXcWG, ... (each one's run
-// found by a two-level ballot search, all loads issued together) and puts
-// each at its row-major place lpos in LDS.  Then each row's
+// Combine of the LDS-sliced image, WG threads a block: block b (rows
+// r0 .. r1-1, at most 2 WG of them) takes from each slice p its run of
+// slots off[b S + p] .. off[(b+1) S + p] - 1.  The run lengths are scanned
+// in LDS, numbering the block's partials 0 .. cnt-1 in slice order; each
+// non-empty run marks its first partial in a bitmap and records itself as
+// the run of every 64-partial word it begins.  Thread t takes partials
+// t, t + WG, ...: a wavefront's 64 partials are one word, so a lane's run
+// is the word's first run plus the marks up to it (a bit count), and its
+// slot that run's offset from it; all loads are issued together.  Each
+// partial goes to its row-major place lpos in LDS; then each row's
 // partials are added in slice order (a thread per row, a wavefront for a
-// row of more than 32 pieces) and alpha and beta applied.  A block of one
+// row of more than 32 pieces) and alpha and beta applied.  The run bounds
+// are loaded first and the rows' piece bounds with them.  A block of one
 // row with more pieces than LDS holds adds its runs in place.  Consecutive
 // blocks run on one XCD, so the lines their runs share stay in that XCD's
 // L2.
-template <class A, bool COMP>
+template <class A, bool COMP, int WG>
 __device__ __forceinline__ void xc_finish(Acc<A, COMP> a, A* wsum, int lane, int wave) {
-  constexpr int NW = kXcWG / 64;
+  constexpr int NW = WG / 64;
   a = group_reduce(a, 64);
   if (lane == 0) {
     wsum[wave] = a.s;
@@ -454,18 +462,20 @@ __device__ __forceinline__ void xc_finish(Acc<A, COMP> a, A* wsum, int lane, int
   }
 }
 
-template <class A, class IO, bool COMP>
-__global__ __launch_bounds__(kXcWG) void k_xl_combine(
+template <class A, class IO, bool COMP, int WG>
+__global__ __launch_bounds__(WG) void k_xl_combine(
     const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cblk,
     const int32_t* __restrict__ off, const uint16_t* __restrict__ lpos,
     const A* __restrict__ part, IO* __restrict__ y, int64_t S, int nb, A alpha, A beta) {
   using Ac = Acc<A, COMP>;
-  constexpr int CP = xl_comb_pieces(sizeof(A));
-  constexpr int NW = kXcWG / 64;
-  constexpr int PT = CP / kXcWG;
+  constexpr int CP = xl_comb_pieces(sizeof(A), WG);
+  constexpr int NW = WG / 64;
+  constexpr int PT = CP / WG;
+  constexpr int RT = xl_comb_rows(WG) / WG;  // rows of a thread
   __shared__ A prod[CP];
-  __shared__ int pre[kXcMaxSlices + 1];
-  __shared__ int beg[kXcMaxSlices];
+  __shared__ uint32_t bits[CP / 32];  // run starts over the block's partials
+  __shared__ int rw[CP / 64];         // run of each 64-partial word's first
+  __shared__ int delta[WG];           // slot - partial index of each run
   __shared__ A wsum[2 * NW];
   __shared__ int wtot[NW];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -473,17 +483,31 @@ __global__ __launch_bounds__(kXcWG) void k_xl_combine(
   const int per = (nb + 7) / 8;
   const int b = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
   if (b >= nb) return;
-  const int r0 = cblk[2 * b], r1 = cblk[2 * b + 1];
-  const int base = rowptr[r0], cnt = rowptr[r1] - base;
   const int32_t* o0 = off + (int64_t)b * S;
   const int32_t* o1 = o0 + S;
+  // thread t: slice t's run (independent of the rows: loaded first, from
+  // a clamped index so that no branch waits for it)
+  const bool scan = S <= WG;
+  const int pt = scan ? min(tid, (int)S - 1) : 0;
+  const int a0 = o0[pt], a1 = o1[pt];
+  const int r0 = cblk[2 * b], r1 = cblk[2 * b + 1];
+  const int base = rowptr[r0], cnt = rowptr[r1] - base;
+  // the piece bounds of the thread's rows r0 + tid + j WG (rows past r1
+  // read r1's), loaded ahead of the partials
+  int rs[RT], re[RT];
+#pragma unroll
+  for (int j = 0; j < RT; ++j) {
+    const int r = r0 + tid + j * WG;
+    rs[j] = rowptr[min(r, r1)];
+    re[j] = rowptr[min(r + 1, r1)];
+  }
   if (cnt > CP) {
     Ac a;
     for (int64_t p = wave; p < S; p += NW) {
-      const int a1 = o1[p];
-      for (int i = o0[p] + lane; i < a1; i += 64) a.add(part[i]);
+      const int e1 = o1[p];
+      for (int i = o0[p] + lane; i < e1; i += 64) a.add(part[i]);
     }
-    xc_finish<A, COMP>(a, wsum, lane, wave);
+    xc_finish<A, COMP, WG>(a, wsum, lane, wave);
     __syncthreads();
     if (tid == 0) {
       Ac t;
@@ -499,55 +523,50 @@ __global__ __launch_bounds__(kXcWG) void k_xl_combine(
     }
     return;
   }
-  if (S > kXcMaxSlices) {  // too many slices to scan in LDS: a wavefront per run
+  if (!scan) {  // too many slices to scan in LDS: a wavefront per run
     for (int64_t p = wave; p < S; p += NW) {
-      const int a0 = o0[p], a1 = o1[p];
-      for (int i = a0 + lane; i < a1; i += 64) prod[lpos[i]] = part[i];
+      const int e0 = o0[p], e1 = o1[p];
+      for (int i = e0 + lane; i < e1; i += 64) prod[lpos[i]] = part[i];
     }
   } else {
-    // run starts and the scan of the run lengths (thread t: slice t)
-    int l0 = 0;
-    if (tid < S) {
-      const int a0 = o0[tid];
-      beg[tid] = a0;
-      l0 = o1[tid] - a0;
-    }
-    int inc = l0;  // inclusive scan across the workgroup
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += o;
-    }
-    if (lane == 63) wtot[wave] = inc;
+    for (int i = tid; i < CP / 32; i += WG) bits[i] = 0u;
+    // scan across the workgroup of (run length | non-empty << 16)
+    const int l0 = tid < S ? a1 - a0 : 0;
+    const int own = l0 | (l0 > 0 ? 1 << 16 : 0);
+    int wsum;
+    const int wex = xl_lane_prefix(own, lane, wsum);
+    if (lane == 0) wtot[wave] = wsum;
     __syncthreads();
     int wbase = 0;
     for (int w = 0; w < wave; ++w) wbase += wtot[w];
-    if (tid < S) pre[tid] = wbase + inc - l0;
-    if (tid == 0) pre[S] = cnt;
+    // a non-empty run: its first partial index q0 and its rank c among the
+    // non-empty runs; its start marked, the words it begins recorded
+    const int ex = wbase + wex;
+    const int q0 = ex & 0xFFFF, c = ex >> 16;
+    if (l0 > 0) {
+      delta[c] = a0 - q0;
+      atomicOr(&bits[q0 >> 5], 1u << (q0 & 31));
+      for (int w = (q0 + 63) >> 6; w * 64 < q0 + l0; ++w) rw[w] = c;
+    }
     __syncthreads();
-    // partial q = tid + j kXcWG: the run of the wavefront's first q by a
-    // two-level ballot search over the scan (16 runs, then 1), each lane's
-    // own run by walking on from there; all loads issued together
+    // partial q = qc + lane, qc = wave 64 + j WG (a whole 64-partial word):
+    // its run is the word's first run plus the run starts up to q
     A v[PT];
     int at[PT];
 #pragma unroll
     for (int j = 0; j < PT; ++j) {
-      const int qc = wave * 64 + j * kXcWG;  // uniform
+      const int qc = wave * 64 + j * WG;  // uniform
       v[j] = A(0);
       at[j] = -1;
       if (qc < cnt) {
-        const int i1 = lane * 16;
-        const uint64_t b1 = __ballot(i1 <= (int)S && pre[min(i1, (int)S)] <= qc);
-        const int L1 = 63 - __clzll((long long)b1);  // lane 0 always qualifies (pre[0] = 0)
-        const int i2 = L1 * 16 + (lane & 15);
-        const uint64_t b2 = __ballot(lane < 16 && i2 < (int)S && pre[min(i2, (int)S)] <= qc);
-        int lo = 63 - __clzll((long long)b2);
-        lo = L1 * 16 + lo;
+        const int w = qc >> 6;
+        const uint64_t m = (((uint64_t)bits[2 * w + 1] << 32) | bits[2 * w]) & ~1ull;
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const int run = rw[w] + below + (int)((m >> lane) & 1ull);
         const int q = qc + lane;
         if (q < cnt) {
-          int nxt = pre[lo + 1];
-          while (q >= nxt) nxt = pre[++lo + 1];
-          const int slot = beg[lo] + (q - pre[lo]);
+          const int slot = q + delta[run];
           v[j] = part[slot];
           at[j] = lpos[slot];
         }
@@ -558,13 +577,10 @@ __global__ __launch_bounds__(kXcWG) void k_xl_combine(
       if (at[j] >= 0) prod[at[j]] = v[j];
   }
   __syncthreads();
-  for (int rb = r0; rb < r1; rb += kXcWG) {
-    const int r = rb + tid;
-    int k0 = 0, k1 = 0;
-    if (r < r1) {
-      k0 = rowptr[r] - base;
-      k1 = rowptr[r + 1] - base;
-    }
+#pragma unroll
+  for (int j = 0; j < RT; ++j) {
+    const int r = r0 + tid + j * WG;
+    const int k0 = rs[j] - base, k1 = re[j] - base;
     const bool is_long = r < r1 && k1 - k0 > 32;
     if (r < r1 && !is_long) {
       Ac s;
@@ -580,7 +596,7 @@ __global__ __launch_bounds__(kXcWG) void k_xl_combine(
       for (int k = rlo + lane; k < rhi; k += 64) a.add(prod[k]);
       a = group_reduce(a, 64);
       if (lane == 0) {
-        const int rr = rb + (tid - lane) + l;
+        const int rr = r0 + (tid - lane) + j * WG + l;
         y[rr] = alpha * a.value() + scaled_y(beta, y, rr);
       }
     }
@@ -624,25 +640,30 @@ static int run_csr(const DevCSR& d, int64_t m, double alpha, const void* x, doub
     if (m == 0) return RVVLA_OK;
     if (xl.nunits > 0) {
       if (comp)
-        hipLaunchKernelGGL((k_xl_slices<VT, true>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s, xl.col,
-                           (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg, (const IO*)x,
-                           xl.n, part);
+        hipLaunchKernelGGL((k_xl_slices<VT, true>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0,
+                           s, xl.col, (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg,
+                           (const IO*)x, xl.n, part);
       else
-        hipLaunchKernelGGL((k_xl_slices<VT, false>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG), 0, s,
-                           xl.col, (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg,
+        hipLaunchKernelGGL((k_xl_slices<VT, false>), dim3((unsigned)xl.nwg), dim3(XlShape<VT>::WG),
+                           0, s, xl.col, (const VT*)xl.val, (const int4*)xl.task, xl.units, xl.wg,
                            (const IO*)x, xl.n, part);
       if (int rc = hip_check(hipGetLastError(), "k_xl_slices launch")) return rc;
     }
     const int nb = (int)xl.nblocks;
     const unsigned grid = (unsigned)(8 * ((nb + 7) / 8));
-    if (comp)
-      hipLaunchKernelGGL((k_xl_combine<AC, IO, true>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
-                         xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha,
-                         (AC)beta);
-    else
-      hipLaunchKernelGGL((k_xl_combine<AC, IO, false>), dim3(grid), dim3(kXcWG), 0, s, xl.rowptr,
-                         xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha,
-                         (AC)beta);
+    auto comb = [&](auto wg) {
+      constexpr int WG = decltype(wg)::value;
+      if (comp)
+        hipLaunchKernelGGL((k_xl_combine<AC, IO, true, WG>), dim3(grid), dim3(WG), 0, s, xl.rowptr,
+                           xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha, (AC)beta);
+      else
+        hipLaunchKernelGGL((k_xl_combine<AC, IO, false, WG>), dim3(grid), dim3(WG), 0, s,
+                           xl.rowptr, xl.cblk, xl.off, xl.lpos, part, (IO*)y, xl.S, nb, (AC)alpha,
+                           (AC)beta);
+    };
+    if (xl.cwg == 256) comb(std::integral_constant<int, 256>());
+    else if (xl.cwg == 512) comb(std::integral_constant<int, 512>());
+    else comb(std::integral_constant<int, 1024>());
     return hip_check(hipGetLastError(), "k_xl_combine launch");
   }
   if (d.xs.P > 0) {

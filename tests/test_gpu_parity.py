@@ -288,24 +288,27 @@ def test_csr_sell_image_nonfinite_x(dtype):
     assert np.allclose(y[fin], y_ref[fin])
 
 
+@pytest.mark.parametrize("cwg", [256, 512, 1024])
 @pytest.mark.parametrize("name", CASES + ["wide"])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
-def test_csr_lds_slices(name, dtype):
+def test_csr_lds_slices(name, dtype, cwg):
     """The LDS-sliced CSR image (x slices held in LDS, a partial per
     (row, slice) pair, partials added in slice order) against the oracle,
-    including long pairs summed by a whole wavefront and empty rows."""
+    including long pairs summed by a whole wavefront and empty rows, with
+    each combine workgroup width."""
     a = case(name) if name != "wide" else _wide()
-    print("csr xlds", name, dtype, flush=True)
+    print("csr xlds", name, dtype, cwg, flush=True)
     x64, y064 = inputs(a, 8)
     dt = np.float64 if dtype == R.F64 else np.float32
+    params = f"xlds=1,xlcomb={cwg}"
     for alpha, beta in ((1.0, 0.0), (-0.5, 2.0)):
-        y = run_gpu("csr", dtype, a, x64, y064, alpha, beta, "xlds=1")
+        y = run_gpu("csr", dtype, a, x64, y064, alpha, beta, params)
         v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
         y_ref = O.csr_spmv(a.rst, a.col, v, x, alpha, beta, y0)
         assert_within_bound(y, y_ref, a.rst, a.col, v, x, alpha, beta, y0, dt,
                             f"csr xlds/{name}/a={alpha},b={beta}")
         # the order of the additions is fixed by the data: the same bits again
-        assert np.array_equal(y, run_gpu("csr", dtype, a, x64, y064, alpha, beta, "xlds=1"))
+        assert np.array_equal(y, run_gpu("csr", dtype, a, x64, y064, alpha, beta, params))
 
 
 def _wide():
@@ -323,8 +326,9 @@ def _wide():
     return _CACHE["wide"]
 
 
+@pytest.mark.parametrize("cwg", [256, 1024])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
-def test_csr_lds_slices_many_pairs(dtype):
+def test_csr_lds_slices_many_pairs(dtype, cwg):
     """A row with an entry in each of 5000 slices of 16384 columns (more
     partials than a combine block holds: summed by the whole workgroup)
     between short rows, against the oracle."""
@@ -338,7 +342,7 @@ def test_csr_lds_slices_many_pairs(dtype):
     print("csr xlds many pairs", dtype, flush=True)
     x64, y064 = inputs(a, 14)
     dt = np.float64 if dtype == R.F64 else np.float32
-    y = run_gpu("csr", dtype, a, x64, y064, 0.5, -1.0, "xlds=1")
+    y = run_gpu("csr", dtype, a, x64, y064, 0.5, -1.0, f"xlds=1,xlcomb={cwg}")
     v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
     y_ref = O.csr_spmv(a.rst, a.col, v, x, 0.5, -1.0, y0)
     assert_within_bound(y, y_ref, a.rst, a.col, v, x, 0.5, -1.0, y0, dt, "csr xlds many pairs")

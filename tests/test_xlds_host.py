@@ -31,9 +31,10 @@ def _w(dtype):
     return 16384 if dtype == R.F64 else 32768
 
 
-def _cap(dtype):
-    """Combine blocks: at most 2048 rows and 64 KiB of partials."""
-    return 2048, (8192 if dtype == R.F64 else 16384)
+def _cap(dtype, cwg=1024):
+    """Combine blocks for cwg-thread workgroups: at most 2 cwg rows and
+    64 cwg bytes of partials (FP64 partials for FP64, FP32 otherwise)."""
+    return 2 * cwg, 64 * cwg // (8 if dtype == R.F64 else 4)
 
 
 def _view(M, dtype):
@@ -115,11 +116,11 @@ def _many_pieces(W):
     return rst, np.concatenate(cols).astype(np.int32), np.linspace(-1, 1, 9002), n
 
 
-def _check_structure(a, v, dtype, rst, col):
+def _check_structure(a, v, dtype, rst, col, cwg=1024):
     W = _w(dtype)
     S = (a.n + W - 1) // W
     m, nnz = a.m, a.nnz
-    rows_cap, cap = _cap(dtype)
+    rows_cap, cap = _cap(dtype, cwg)
     tasks = v["task"].reshape(-1, 4)
     npieces = len(v["lpos"])
     marks = v["mark"]
@@ -172,15 +173,17 @@ def _check_structure(a, v, dtype, rst, col):
             assert sorted(v["lpos"][slots].tolist()) == list(range(cnt))
 
 
+@pytest.mark.parametrize("cwg", [256, 512, 1024])
 @pytest.mark.parametrize("name", list(CASES) + ["many_pieces"])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
-def test_xlds_layout_and_product(name, dtype):
+def test_xlds_layout_and_product(name, dtype, cwg):
     W = _w(dtype)
     rst, col, val, n = CASES[name]() if name != "many_pieces" else _many_pieces(W)
     a = R.CSR(len(rst) - 1, n, rst, col, val)
-    M = R.convert("csr", a, dtype, "xlds=1").upload()
+    params = "xlds=1" if cwg == 1024 else f"xlds=1,xlcomb={cwg}"
+    M = R.convert("csr", a, dtype, params).upload()
     v = _view(M, dtype)
-    _check_structure(a, v, dtype, rst, col)
+    _check_structure(a, v, dtype, rst, col, cwg)
     rng = np.random.default_rng(5)
     dt = np.float64 if dtype == R.F64 else np.float32
     x = rng.uniform(-1, 1, n).astype(dt)
@@ -188,7 +191,7 @@ def test_xlds_layout_and_product(name, dtype):
     vals = a.values_for(dtype)
     S = (n + W - 1) // W
     part = _slice_pass(v, n, W, x.astype(np.float64))
-    y = _combine(v, a.m, S, part, 1.25, -0.5, y0.astype(np.float64), _cap(dtype)[1])
+    y = _combine(v, a.m, S, part, 1.25, -0.5, y0.astype(np.float64), _cap(dtype, cwg)[1])
     y_ref = O.csr_spmv(a.rst, a.col, vals, x, 1.25, -0.5, y0)
     assert_within_bound(y.astype(dt), y_ref, a.rst, a.col, vals, x, 1.25, -0.5, y0, dt,
                         f"xlds {name}")
@@ -207,6 +210,13 @@ def test_xlds_pieces_cut_by_tasks():
     # pieces: row 0 (1), row 1 cut at 512, 1024, 1536 (4), row 2 (1)
     assert np.array_equal(v["rowptr"], [0, 1, 5, 6])
     assert np.flatnonzero(v["mark"][:2004]).tolist() == [0, 3, 512, 1024, 1536, 2003]
+
+
+def test_xlds_combine_width_refused():
+    rst, col, val = laplacian(32)
+    a = R.CSR(len(rst) - 1, 1024, rst, col, val)
+    with pytest.raises(R.RvvlaError):
+        R.convert("csr", a, R.F64, "xlds=1,xlcomb=300").upload()
 
 
 def test_xlds_empty_matrix():
