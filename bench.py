@@ -317,22 +317,30 @@ def config1_leg(R, torch, steps, warmup):
             "gpu": gpu}, a, x64
 
 
-def read_probe(torch, nbytes=1 << 31, steps=20):
-    """Achievable read rate: torch's sum over a 2 GiB FP32 buffer (reads the
-    buffer once, writes a few bytes), HIP events."""
+def read_probe(R, torch, nbytes=1 << 31, steps=20):
+    """Achievable streaming-read rate: the library's read probe (16 workgroups
+    a CU, each streaming its own contiguous share with 16-byte nontemporal
+    loads) over a 2 GiB buffer, HIP events on the launch stream."""
     buf = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    sink = torch.zeros(1 << 16, dtype=torch.int32, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+
+    def go():
+        R.check(R.lib.rvvla_read_probe(buf.data_ptr(), nbytes, sink.data_ptr(), s))
+
     for _ in range(3):
-        buf.sum()
+        go()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(steps):
-        buf.sum()
+        go()
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
-    del buf
-    return {"bytes_read": nbytes, "ms": round(ms, 4), "GBs": round(nbytes / ms / 1e6, 1)}
+    del buf, sink
+    return {"bytes_read": nbytes, "ms": round(ms, 4), "GBs": round(nbytes / ms / 1e6, 1),
+            "kernel": "k_read_probe"}
 
 
 def autograd_leg(R, torch, steps, warmup):
@@ -511,7 +519,7 @@ def main():
         "formats": results,
     }
     line["copy_probe"] = copy_probe(torch)
-    line["read_probe"] = read_probe(torch)
+    line["read_probe"] = read_probe(R, torch)
     mats = {"laplacian": (a, x64)}
     log("bench: config-1 (n = 65,536, 16 per row) CSR and HCSR")
     line["config1"], a1, x1 = config1_leg(R, torch, args.steps, args.warmup)

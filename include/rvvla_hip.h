@@ -75,6 +75,14 @@ int         rvvla_version(void);
  * "vhcc", "cvr"); NULL for an unknown code. */
 const char* rvvla_format_name(int format);
 
+/* Measurement helper (no reference counterpart): reads `bytes` (a multiple
+ * of 16, 16-byte aligned device buffer) once with streaming loads on
+ * `stream`; `sink` is a device buffer of 16 * CUs uint32 that the kernel
+ * writes only for a fold value the caller's data will not produce.  Time it
+ * with events on `stream`: the device's streaming-read rate, the ceiling of
+ * a read-mostly SpMV. */
+int rvvla_read_probe(const void* buf, int64_t bytes, void* sink, void* stream);
+
 /* ---- conversion (host) -----------------------------------------------
  * Replaces convert<T, MtxDist, true>(dist, src, params)
  * (reference include/convert.h:30-31; the only reference converter is

@@ -93,6 +93,7 @@ _sigs = {
     "rvvla_spmv_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp, _vp]),
     "rvvla_spmv_host_op": (_i32, [_vp, _i32, _d, _vp, _d, _vp]),
     "rvvla_spmm": (_i32, [_vp, _i32, _d, _vp, _i64, _d, _vp, _i64, _vp]),
+    "rvvla_read_probe": (_i32, [_vp, _i64, _vp, _vp]),
     "rvvla_spmv_host": (_i32, [_vp, _d, _vp, _d, _vp]),
     "rvvla_gen_laplacian5": (_i32, [_i64, C.POINTER(GenCSR)]),
     "rvvla_gen_random_rows": (_i32, [_i64, _i32, _i32, _i32, _d, C.c_uint64, C.POINTER(GenCSR)]),

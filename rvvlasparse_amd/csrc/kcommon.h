@@ -88,6 +88,21 @@ __device__ __forceinline__ uint32_t ldu(const uint32_t* p, int64_t i) {
   else return p[i];
 }
 
+// CUs of the current device (256 when it cannot be read), looked up once
+// per device.
+static inline int device_cu_count() {
+  static int cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = __atomic_load_n(&cache[dev], __ATOMIC_RELAXED);
+  if (c == 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0)
+      c = 256;
+    __atomic_store_n(&cache[dev], c, __ATOMIC_RELAXED);
+  }
+  return c;
+}
+
 // Nontemporal streaming of the matrix arrays is on unless the environment
 // sets RVVLA_NT=0 (for comparisons).
 static inline bool nt_enabled() {

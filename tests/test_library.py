@@ -166,3 +166,12 @@ def test_generators():
     assert expect <= near.sum() <= expect + 0.01 * pl.nnz
     u = R.gen_uniform(10000, -1, 1, seed=4)
     assert -1 <= u.min() and u.max() < 1 and abs(u.mean()) < 0.05
+
+
+def test_read_probe_refuses_bad_buffers():
+    """The read probe checks its arguments before touching the device."""
+    import ctypes
+    bad = [(None, 1024), (ctypes.c_void_p(16), 8), (ctypes.c_void_p(16), 1000),
+           (ctypes.c_void_p(8), 1024)]
+    for buf, nbytes in bad:
+        assert R.lib.rvvla_read_probe(buf, nbytes, ctypes.c_void_p(64), None) == -10  # RVVLA_ERR_INVALID_ARG
