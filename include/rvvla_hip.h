@@ -106,6 +106,23 @@ int rvvla_layout_array(rvvla_layout_t L, const char* field, const void** data,
 int rvvla_layout_scalar(rvvla_layout_t L, const char* field, double* value);
 int rvvla_layout_free(rvvla_layout_t L);
 
+/* Autotuner (no reference counterpart: the reference leaves the choice of
+ * format and of SELL sigma, CSR5 sigma and HCSR R and C to the caller).
+ * Times candidate images of the CSR matrix on the current device -- CSR
+ * (its chosen image, the plain kernel), SELL (sigma 1, 64, 4096, default),
+ * CSR5 (sigma 4, 8, 16), HCSR (the reference's R = C defaults, 2048^2,
+ * 4096^2, 256 x 8192, 1024 x 16384), LAV, VNEC, VHCC and CVR with their
+ * defaults -- and writes the fastest one's format code to *format and its
+ * convertParams text to `out` (`len` bytes).  params: "formats=a+b+..."
+ * (default all eight), "reps=N" (timed products a candidate, default 20),
+ * "cache=<path>" (a text file of earlier choices, read first and rewritten).
+ * Choices are cached for the process by the row-length histogram (rows
+ * in power-of-two length buckets to 1/20, the max/mean bucket), the
+ * power-of-two buckets of m and n, the dtype and the format list. */
+int rvvla_autotune(int dtype, int64_t m, int64_t n, int64_t nnz, const int32_t* rst,
+                   const int32_t* col, const void* val, const char* params, int* format,
+                   char* out, int len);
+
 /* ---- upload (host -> device, once) ------------------------------------ */
 int rvvla_upload(rvvla_layout_t L, rvvla_mat_t* out);
 /* Direct upload of spMtxCRS fields (reference include/CRS.h:13-20). */

@@ -397,6 +397,18 @@ def convert(fmt: str | int, a: CSR, dtype: int = F64, params: str = "") -> Layou
     return Layout(fmt, a, dtype, params)
 
 
+def autotune(a: CSR, dtype: int = F64, params: str = "") -> tuple[str, str]:
+    """The fastest (format name, convertParams text) for `a` on the current
+    device among the autotuner's candidates (rvvla_autotune); params
+    "formats=a+b", "reps=N", "cache=<path>"."""
+    vals = a.values_for(dtype)
+    fmt, out = C.c_int(), C.create_string_buffer(256)
+    check(lib.rvvla_autotune(dtype, a.m, a.n, a.nnz, _ptr(a.rst), _ptr(a.col), _ptr(vals),
+                             params.encode(), C.byref(fmt), out, 256))
+    name = [k for k, v in FORMATS.items() if v == fmt.value][0]
+    return name, out.value.decode()
+
+
 for _name, (_res, _args) in {
     "rvvla_batch_create": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _cp, C.POINTER(_vp)]),
     "rvvla_batch_block": (_i32, [_vp, _i32, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_i64),
@@ -406,6 +418,8 @@ for _name, (_res, _args) in {
     "rvvla_batch_capture": (_i32, [_vp, _d, _d]),
     "rvvla_batch_replay": (_i32, [_vp, _vp]),
     "rvvla_batch_free": (_i32, [_vp]),
+    "rvvla_autotune": (_i32, [_i32, _i64, _i64, _i64, _vp, _vp, _vp, _cp, C.POINTER(_i32), _cp,
+                              _i32]),
 }.items():
     _f = getattr(lib, _name)
     _f.restype, _f.argtypes = _res, _args

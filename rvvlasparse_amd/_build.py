@@ -22,7 +22,7 @@ BUILD = ROOT / "_build"
 LIB = PKG / "librvvla_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HIP_HOST = ("capi.cpp", "batch_host.cpp", "cxx_api.cpp")  # host code that calls the HIP runtime
+HIP_HOST = ("capi.cpp", "batch_host.cpp", "cxx_api.cpp", "autotune.cpp")  # host code that calls the HIP runtime
 HIP_SOURCES = sorted(CSRC.glob("*.hip")) + [CSRC / f for f in HIP_HOST]
 HOST_SOURCES = sorted(p for p in CSRC.glob("*.cpp") if p.name not in HIP_HOST)
 

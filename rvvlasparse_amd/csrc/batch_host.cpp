@@ -46,30 +46,6 @@ constexpr int kBatchGrid = 2048;  // persistent workgroups (4 wavefronts each)
 std::mutex g_tune_mu;
 std::map<std::string, Choice> g_tune_cache;
 
-std::string tune_key(int64_t m, const int* rst, int k) {
-  int hist[32] = {0};
-  int64_t mx = 0, tot = 0;
-  for (int64_t r = 0; r < m; ++r) {
-    const int64_t l = rst[r + 1] - rst[r];
-    int b = 0;
-    while ((int64_t(1) << b) <= l && b < 31) ++b;
-    hist[b]++;
-    mx = std::max(mx, l);
-    tot += l;
-  }
-  std::ostringstream os;
-  int kp = 1;
-  while (kp < k) kp <<= 1;
-  os << "k" << kp << ":";
-  for (int b = 0; b < 32; ++b) {
-    const int q = m ? (int)std::lround(20.0 * hist[b] / (double)m) : 0;
-    if (q) os << b << "=" << q << ",";
-  }
-  const double ratio = tot ? (double)mx * (double)m / (double)tot : 1.0;
-  os << "r" << (int)std::ceil(std::log2(std::max(1.0, ratio)));
-  return os.str();
-}
-
 }  // namespace
 
 struct rvvla_batch_s {
@@ -344,7 +320,7 @@ rvvla_batch_t create(int dtype, int count, const int64_t* m, const int64_t* n,
     throw Error(RVVLA_ERR_INVALID_ARG, "batch: wave_items must be in [1, 1024]");
   for (int i = 0; i < count; ++i) {
     const HostMat<V>& h = hm[(size_t)i];
-    const std::string key = tune_key(h.m, h.rst, h.k);
+    const std::string key = rvvla::row_length_key(h.m, h.rst, h.k);
     Choice c;
     if (force_fmt >= 0) {
       c.fmt = force_fmt;

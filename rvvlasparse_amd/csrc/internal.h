@@ -27,6 +27,10 @@ const std::string& last_error();
 // Validates CSR arrays (shape, monotone row pointers, columns in range);
 // throws Error(RVVLA_ERR_INVALID_ARG).
 void check_csr_public(int64_t m, int64_t n, int64_t nnz, const int* rst, const int* col);
+// Key of a matrix's row-length histogram (autotune.cpp): rows in
+// power-of-two length buckets rounded to 1/20, the max/mean bucket and the
+// right-hand-side count rounded up to a power of two.
+std::string row_length_key(int64_t m, const int* rst, int k);
 
 // convertParams from "k=v,k=v" text.
 SparseMatrixLib::convertParams parse_params(const char* text);
