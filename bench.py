@@ -443,6 +443,8 @@ def main():
                     help="skip the autograd-SpMM and batched legs")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy probe (counter calibration)")
+    ap.add_argument("--only", choices=("batch", "autograd"), default=None,
+                    help="run only the batched or the autograd-SpMM leg (for profiles)")
     ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_child:
@@ -461,6 +463,13 @@ def main():
     dev = torch.device("cuda:0")
     if args.copy_only:
         print(json.dumps({"copy_probe": copy_probe(torch, steps=args.steps)}), flush=True)
+        return
+    if args.only == "batch":
+        print(json.dumps({"batched": batch_leg(R, torch, args.steps, args.warmup)}), flush=True)
+        return
+    if args.only == "autograd":
+        print(json.dumps({"autograd_spmm": autograd_leg(R, torch, args.steps, args.warmup)}),
+              flush=True)
         return
     g = args.grid
     log(f"bench: generating Laplacian {g}x{g}")
@@ -520,6 +529,11 @@ def main():
     }
     line["copy_probe"] = copy_probe(torch)
     line["read_probe"] = read_probe(R, torch)
+    log("bench: autotuner on the Laplacian (FP64)")
+    t0 = time.perf_counter()
+    tf_, tp_ = R.autotune(a, R.F64, "formats=csr+sell+hcsr+csr5,reps=20")
+    line["autotune"] = {"laplacian/f64": f"{tf_}:{tp_}", "candidates": "csr, sell, hcsr, csr5",
+                        "seconds": round(time.perf_counter() - t0, 1)}
     mats = {"laplacian": (a, x64)}
     log("bench: config-1 (n = 65,536, 16 per row) CSR and HCSR")
     line["config1"], a1, x1 = config1_leg(R, torch, args.steps, args.warmup)
