@@ -4,11 +4,20 @@
 // line per (format, type, stage form) with the largest row error against a
 // plain CSR loop, scaled by the row's sum |a||x| + |beta||y|, and exits 1 if
 // any exceeds 4*(len+2)*eps.  Run on a machine with a HIP device.
+//
+//   api_drop_in [G] [dump-dir]
+//
+// G: the Laplacian grid (default 100).  With a dump directory every product
+// is also written there, one file each (operands, result and the matrix, see
+// dump()), for a checker outside the program to compare with its own
+// reference.
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rvvla_hip.h"
@@ -64,6 +73,37 @@ spMtxCRS<T> ragged(int g) {
 }
 
 static int failures = 0;
+static const char* g_dump = nullptr;
+static int g_dumped = 0;
+
+// One product as a file: int64 m, n, nnz, element bytes, transposed (0/1);
+// double alpha, beta; int32 Rst[m+1], Col[nnz]; T Val[nnz]; T b, y0, y
+// (b of n entries and y0, y of m, or the other way round when transposed).
+template <class T>
+void dump(const char* fmt, const char* form, const spMtxCRS<T>& a, const std::vector<T>& b,
+          const std::vector<T>& y0, T alpha, T beta, const std::vector<T>& y, bool transposed) {
+  if (!g_dump) return;
+  char path[1024];
+  std::snprintf(path, sizeof path, "%s/%03d_%s_%s_%s.bin", g_dump, g_dumped++, fmt,
+                std::is_same<T, double>::value ? "double" : "float", form);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) {
+    std::printf("cannot write %s\n", path);
+    ++failures;
+    return;
+  }
+  const int64_t hdr[5] = {(int64_t)a.m, (int64_t)a.n, (int64_t)a.nz, (int64_t)sizeof(T), transposed ? 1 : 0};
+  const double ab[2] = {(double)alpha, (double)beta};
+  std::fwrite(hdr, sizeof hdr, 1, f);
+  std::fwrite(ab, sizeof ab, 1, f);
+  std::fwrite(a.Rst, sizeof(int), a.m + 1, f);
+  std::fwrite(a.Col, sizeof(int), a.nz, f);
+  std::fwrite(a.Val, sizeof(T), a.nz, f);
+  std::fwrite(b.data(), sizeof(T), b.size(), f);
+  std::fwrite(y0.data(), sizeof(T), y0.size(), f);
+  std::fwrite(y.data(), sizeof(T), y.size(), f);
+  std::fclose(f);
+}
 
 static void report(const char* fmt, bool dbl, const char* what, bool same) {
   if (!same) ++failures;
@@ -95,6 +135,7 @@ void check(const char* fmt, const char* form, const spMtxCRS<T>& a, const std::v
   }
   const bool ok = st.code == 0 && worst <= 1.0;
   if (!ok) ++failures;
+  if (st.code == 0) dump<T>(fmt, form, a, b, y0, alpha, beta, y, false);
   std::printf("%-5s %-6s %-7s status=%d err/bound=%.3f %s\n", fmt,
               std::is_same<T, double>::value ? "double" : "float", form, st.code, worst,
               ok ? "ok" : "FAIL");
@@ -124,6 +165,7 @@ void check_transpose(const char* fmt, const spMtxCRS<T>& a, const std::vector<T>
   }
   const bool ok = st.code == 0 && worst <= 1.0;
   if (!ok) ++failures;
+  if (st.code == 0) dump<T>(fmt, "transposed", a, b, y0, alpha, beta, y, true);
   std::printf("%-5s %-6s A^T     status=%d err/bound=%.3f %s\n", fmt,
               std::is_same<T, double>::value ? "double" : "float", st.code, worst, ok ? "ok" : "FAIL");
 }
@@ -362,6 +404,7 @@ void all_formats(int g) {
 
 int main(int argc, char** argv) {
   const int g = argc > 1 ? std::atoi(argv[1]) : 100;
+  if (argc > 2) g_dump = argv[2];
   all_formats<double>(g);
   all_formats<float>(g);
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
