@@ -18,7 +18,10 @@
 //    is fixed by the structure and found at upload), tail rows are summed
 //    as CSR, empty rows get alpha*0 + beta*y.  Segments spanning more than
 //    kCsr5Serial tiles or running into the tail, and tail rows longer than
-//    kCsr5Serial entries, are left to k_csr5_long (one wavefront each).
+//    kCsr5Serial entries, take a wavefront each (csr5_long_item) in extra
+//    workgroups of the same launch.
+//  * With 32-bit accumulation and sigma 8 or 16, each XCD walks a
+//    contiguous run of tiles.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
   using Ac = Acc<A, COMP>;
   constexpr int TPB = SIG > 0 ? 1 : kWG / 64;  // tiles per workgroup
   __shared__ A xb[SIG > 0 ? 64 * (SIG + 1) : 1];
-  const int64_t t = (int64_t)blockIdx.x * TPB + (threadIdx.x >> 6);
+  const int64_t t = SIG > 0 && sizeof(A) < 8 ? (int64_t)xcd_run_index((int)blockIdx.x, (int)gridDim.x) : (int64_t)blockIdx.x * TPB + (threadIdx.x >> 6);
   if (t >= a.ntiles) return;  // uniform per wavefront (per workgroup when SIG > 0)
   const int lane = threadIdx.x & 63;
   const int sigma = a.sigma, f0 = 32 - a.bit_y - a.bit_seg;
@@ -199,52 +202,17 @@ __global__ __launch_bounds__(SIG > 0 ? 64 : kWG) void k_csr5_tiles(
   }
 }
 
-template <class VT, bool COMP>
-__global__ __launch_bounds__(kWG) void k_csr5_fixup(
-    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
-    typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
-    const typename Val<VT>::acc* __restrict__ last_part, const int32_t* __restrict__ last_row,
-    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
-  using A = typename Val<VT>::acc;
-  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
-  const int64_t ntail = a.m - a.first_tail_row;
-  if (i < a.ntiles) {
-    const int32_t r = last_row[i];
-    if (r < 0) return;
-    const uint32_t ch = (uint32_t)a.chain[i];
-    const int64_t q1 = ch & 0x7FFFFFFFu;
-    if ((ch >> 31) || q1 - i - 1 > kCsr5Serial) return;  // k_csr5_long's
-    Acc<A, COMP> s;
-    s.add(last_part[i]);
-    for (int64_t q = i + 1; q < q1; ++q) s.add(first_part[q]);
-    y[r] = alpha * s.value() + scaled_y(beta, y, r);
-  } else if (i < a.ntiles + ntail) {
-    const int64_t r = a.first_tail_row + (i - a.ntiles);
-    const int64_t e0 = a.rst[r], e1 = a.rst[r + 1];
-    if (e0 == e1) return;  // empty rows are in the empty list
-    if (e1 - e0 > kCsr5Serial) return;  // k_csr5_long's
-    Acc<A, COMP> s;
-    for (int64_t e = e0; e < e1; ++e) s.add(Val<VT>::get(val, e) * (A)x[a.col[e]]);
-    y[r] = alpha * s.value() + scaled_y(beta, y, r);
-  } else if (i < a.ntiles + ntail + a.nempty) {
-    const int64_t r = a.empty_rows[i - a.ntiles - ntail];
-    y[r] = scaled_y(beta, y, r);
-  }
-}
-
 // One long segment or long tail row per wavefront: an open segment adds
 // the last part of its tile, the leading parts of the tiles it spans and
 // (if it runs into the tail) its tail entries, lanes striding and then a
 // butterfly; a tail row is summed the same way.
 template <class VT, bool COMP>
-__global__ __launch_bounds__(kWG) void k_csr5_long(
-    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+__device__ __forceinline__ void csr5_long_item(
+    const Csr5Args& a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
     const typename Val<VT>::acc* __restrict__ last_part, const int32_t* __restrict__ last_row,
-    typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta, int64_t w) {
   using A = typename Val<VT>::acc;
-  const int64_t w = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
-  if (w >= a.nlong) return;
   const int lane = threadIdx.x & 63;
   const int32_t code = a.long_fix[w];
   Acc<A, COMP> s;
@@ -268,6 +236,44 @@ __global__ __launch_bounds__(kWG) void k_csr5_long(
   if (lane == 0) y[r] = alpha * s.value() + scaled_y(beta, y, r);
 }
 
+template <class VT, bool COMP>
+__global__ __launch_bounds__(kWG) void k_csr5_fixup(
+    Csr5Args a, const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    typename Val<VT>::io* __restrict__ y, const typename Val<VT>::acc* __restrict__ first_part,
+    const typename Val<VT>::acc* __restrict__ last_part, const int32_t* __restrict__ last_row,
+    typename Val<VT>::acc alpha, typename Val<VT>::acc beta, int64_t fix_blocks) {
+  using A = typename Val<VT>::acc;
+  if ((int64_t)blockIdx.x >= fix_blocks) {  // the long items, a wavefront each
+    const int64_t w = (blockIdx.x - fix_blocks) * (kWG / 64) + (threadIdx.x >> 6);
+    if (w < a.nlong) csr5_long_item<VT, COMP>(a, val, x, y, first_part, last_part, last_row, alpha, beta, w);
+    return;
+  }
+  const int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x;
+  const int64_t ntail = a.m - a.first_tail_row;
+  if (i < a.ntiles) {
+    const int32_t r = last_row[i];
+    if (r < 0) return;
+    const uint32_t ch = (uint32_t)a.chain[i];
+    const int64_t q1 = ch & 0x7FFFFFFFu;
+    if ((ch >> 31) || q1 - i - 1 > kCsr5Serial) return;  // a long item
+    Acc<A, COMP> s;
+    s.add(last_part[i]);
+    for (int64_t q = i + 1; q < q1; ++q) s.add(first_part[q]);
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
+  } else if (i < a.ntiles + ntail) {
+    const int64_t r = a.first_tail_row + (i - a.ntiles);
+    const int64_t e0 = a.rst[r], e1 = a.rst[r + 1];
+    if (e0 == e1) return;  // empty rows are in the empty list
+    if (e1 - e0 > kCsr5Serial) return;  // a long item
+    Acc<A, COMP> s;
+    for (int64_t e = e0; e < e1; ++e) s.add(Val<VT>::get(val, e) * (A)x[a.col[e]]);
+    y[r] = alpha * s.value() + scaled_y(beta, y, r);
+  } else if (i < a.ntiles + ntail + a.nempty) {
+    const int64_t r = a.empty_rows[i - a.ntiles - ntail];
+    y[r] = scaled_y(beta, y, r);
+  }
+}
+
 template <class VT>
 static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double beta, void* y,
                     hipStream_t s, void* scratch) {
@@ -283,10 +289,10 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
   int32_t* lr = (int32_t*)(lp + d.ntiles);
   if (d.ntiles > 0) {
     if (d.sigma == 16) {
-      hipLaunchKernelGGL((k_csr5_tiles<VT, 16, COMP>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 16, COMP>), dim3((unsigned)(sizeof(AC) < 8 ? xcd_grid(d.ntiles) : d.ntiles)), dim3(64), 0, s, a,
                          (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     } else if (d.sigma == 8) {
-      hipLaunchKernelGGL((k_csr5_tiles<VT, 8, COMP>), dim3((unsigned)d.ntiles), dim3(64), 0, s, a,
+      hipLaunchKernelGGL((k_csr5_tiles<VT, 8, COMP>), dim3((unsigned)(sizeof(AC) < 8 ? xcd_grid(d.ntiles) : d.ntiles)), dim3(64), 0, s, a,
                          (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
     } else {
       const unsigned g1 = (unsigned)((d.ntiles + kWG / 64 - 1) / (kWG / 64));
@@ -295,18 +301,13 @@ static int run_csr5(const rvvla_mat_s& A, double alpha, const void* x, double be
     }
     if (int rc = hip_check(hipGetLastError(), "k_csr5_tiles launch")) return rc;
   }
+  // the fix-ups (a thread each) and the long items (a wavefront each) in one launch
   const int64_t nfix = d.ntiles + (A.m - d.first_tail_row) + d.nempty;
-  if (nfix > 0) {
-    const unsigned g2 = (unsigned)((nfix + kWG - 1) / kWG);
-    hipLaunchKernelGGL((k_csr5_fixup<VT, COMP>), dim3(g2), dim3(kWG), 0, s, a, (const VT*)d.val,
-                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
-    if (int rc = hip_check(hipGetLastError(), "k_csr5_fixup launch")) return rc;
-  }
-  if (d.nlong > 0) {
-    const unsigned g3 = (unsigned)((d.nlong + kWG / 64 - 1) / (kWG / 64));
-    hipLaunchKernelGGL((k_csr5_long<VT, COMP>), dim3(g3), dim3(kWG), 0, s, a, (const VT*)d.val,
-                       (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta);
-    return hip_check(hipGetLastError(), "k_csr5_long launch");
+  const int64_t g2 = (nfix + kWG - 1) / kWG, g3 = (d.nlong + kWG / 64 - 1) / (kWG / 64);
+  if (g2 + g3 > 0) {
+    hipLaunchKernelGGL((k_csr5_fixup<VT, COMP>), dim3((unsigned)(g2 + g3)), dim3(kWG), 0, s, a,
+                       (const VT*)d.val, (const IO*)x, (IO*)y, fp, lp, lr, (AC)alpha, (AC)beta, g2);
+    return hip_check(hipGetLastError(), "k_csr5_fixup launch");
   }
   return RVVLA_OK;
 }

@@ -12,7 +12,8 @@
 // adds a row's slots in order; empty rows come from a separate list.  The
 // records are re-laid at upload as one 64-bit flush mask per pack (a
 // uniform scalar load) plus the flush targets in record order; a lane finds
-// its target by counting the flushing lanes below it.
+// its target by counting the flushing lanes below it.  With 32-bit
+// accumulation each XCD walks a contiguous run of streams.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -26,7 +27,7 @@ __global__ __launch_bounds__(kWG) void k_cvr(
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ part,
     int64_t nstreams, typename Val<VT>::acc alpha, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
-  const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
+  const int64_t t = __builtin_amdgcn_readfirstlane((int)((sizeof(A) < 8 ? xcd_run_index((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= nstreams) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (uint64_t(1) << lane) - 1;
@@ -128,7 +129,8 @@ static int run_cvr(const rvvla_mat_s& A, double alpha, const void* x, double bet
   const DevCVR& d = A.cvr;
   AC* part = (AC*)(scratch ? scratch : d.part);
   if (d.nstreams > 0) {
-    const unsigned g = (unsigned)((d.nstreams + kWG / 64 - 1) / (kWG / 64));
+    const unsigned g0 = (unsigned)((d.nstreams + kWG / 64 - 1) / (kWG / 64));
+    const unsigned g = sizeof(AC) < 8 ? (unsigned)xcd_grid(g0) : g0;
     hipLaunchKernelGGL((k_cvr<VT, COMP>), dim3(g), dim3(kWG), 0, s, d.pack_start, d.rec_start,
                        d.flush, d.rec_rows, d.tail_rows, d.col, (const VT*)d.val, (const IO*)x,
                        (IO*)y, part, d.nstreams, (AC)alpha, (AC)beta);

@@ -10,9 +10,13 @@
 // gfx950 form:
 //  * vectors without a row end add into lane-private sums (no cross-lane
 //    work); the next row-end vector folds them into its carry;
-//  * the segmented scan's masks follow from the row-end mask by bit
-//    arithmetic (lane j adds lane j-2^s iff j-2^s >= its segment start), so
-//    only the 64-bit row-end mask per vector is read;
+//  * the segmented scan's segments follow from the row-end mask alone (a
+//    lane's segment starts after the highest row end below it), so only the
+//    64-bit row-end mask per vector is read; the scan runs on DPP row
+//    shifts inside each row of 16 lanes plus three row carries (no LDS
+//    permutes);
+//  * with 32-bit accumulation, each XCD walks a contiguous run of
+//    partitions (xcd_run_index);
 //  * instead of the reference's overflow rows and atomic fix-up, a partition
 //    stores the partial of the row it continues (first_part) and of the row
 //    it hands on (fwd_part); one owner thread per handed-on row adds the
@@ -43,11 +47,10 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
                                                typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   using Ac = Acc<A, COMP>;
-  const int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6)));
+  const int64_t t = __builtin_amdgcn_readfirstlane((int)((sizeof(A) < 8 ? xcd_run_index((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x) * (kWG / 64) + (threadIdx.x >> 6)));
   if (t >= a.nparts) return;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (uint64_t(1) << lane) - 1;
-  const uint64_t upto = below | (uint64_t(1) << lane);
   const int64_t v0 = a.start_vec[t], v1 = a.start_vec[t + 1];
   int64_t rank = a.rbase[t];
   const bool has_fwd = a.fwd_row[t] >= 0;
@@ -99,14 +102,7 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
       Ac sum;
       sum.add(p[u]);
       if (lane == 0) sum.merge(carry);
-      const uint64_t heads = 1u | (m << 1);
-      const int hp = 63 - __builtin_clzll(heads & upto);  // this lane's segment start
-      for (int d = 1; d < 64; d <<= 1) {
-        Ac o;
-        o.s = __shfl_up(sum.s, d, 64);
-        if constexpr (COMP) o.c = __shfl_up(sum.c, d, 64);
-        if (lane - d >= hp) sum.merge(o);
-      }
+      sum = wave_seg_scan(sum, 1u | (m << 1), lane);
       if ((m >> lane) & 1u) {
         const uint32_t r = rr[u];
         const int64_t e = (vb + u) * 64 + lane;
@@ -118,10 +114,7 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
           y[r] = alpha * sum.value() + scaled_y(beta, y, r);
         }
       }
-      Ac last;
-      last.s = __shfl(sum.s, 63, 64);
-      if constexpr (COMP) last.c = __shfl(sum.c, 63, 64);
-      carry = (m >> 63) ? Ac() : last;
+      carry = (m >> 63) ? Ac() : acc_readlane(sum, 63);
     }
   }
 }
@@ -163,7 +156,8 @@ static int run_vhcc(const rvvla_mat_s& A, double alpha, const void* x, double be
   AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
   if (d.nparts > 0) {
-    const unsigned g = (unsigned)((d.nparts + kWG / 64 - 1) / (kWG / 64));
+    const unsigned g0 = (unsigned)((d.nparts + kWG / 64 - 1) / (kWG / 64));
+    const unsigned g = sizeof(AC) < 8 ? (unsigned)xcd_grid(g0) : g0;
     hipLaunchKernelGGL((k_vhcc<VT, COMP>), dim3(g), dim3(kWG), 0, s, a, (const VT*)d.val, (const IO*)x,
                        (IO*)y, fp, wp, (AC)alpha, (AC)beta);
     if (int rc = hip_check(hipGetLastError(), "k_vhcc launch")) return rc;

@@ -11,13 +11,15 @@
 // are lane shapes of a CPU SIMD unit; on gfx950 all three run this kernel.
 // Rows crossing partitions are finished by one owner thread that adds the
 // pieces in partition order (deterministic, no atomics):
-// y = alpha*sum + beta*y.
+// y = alpha*sum + beta*y.  A workgroup issues all of its partition's loads
+// (column list, entries, row bounds) before the x gathers; with 32-bit
+// accumulation each XCD walks a contiguous run of partitions.
 #include "kcommon.h"
 
 namespace rvvla {
 
-static_assert(vnec_items(4) <= 8 * kWG && vnec_items(8) <= 8 * kWG,
-              "k_vnec stages at most 8 entries per thread");
+static_assert(vnec_items(4) % kWG == 0 && vnec_items(8) % kWG == 0,
+              "k_vnec stages a whole number of columns and entries per thread");
 
 template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_vnec(
@@ -26,49 +28,51 @@ __global__ __launch_bounds__(kWG) void k_vnec(
     const uint16_t* __restrict__ ecr, const int32_t* __restrict__ rst,
     const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc* __restrict__ first_part,
-    typename Val<VT>::acc* __restrict__ fwd_part, int64_t m, typename Val<VT>::acc alpha,
+    typename Val<VT>::acc* __restrict__ fwd_part, int64_t nparts, int64_t m, typename Val<VT>::acc alpha,
     typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   constexpr int kItems = vnec_items((int)sizeof(A));
   __shared__ A xs[kItems];
   __shared__ A prod[kItems];
-  const int t = blockIdx.x, tid = threadIdx.x;
+  const int t = sizeof(A) < 8 ? xcd_run_index((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int tid = threadIdx.x;
+  if (t >= nparts) return;
   const int x0 = cx[t], x1 = cx[t + 1], y0 = cy[t], y1 = cy[t + 1];
   const int k0 = nec_ptr[t], nec = nec_ptr[t + 1] - k0;
-  // four loads in flight per thread in each staging loop
-  {
-    int k = tid;
-    for (; k + 3 * kWG < nec; k += 4 * kWG) {
-      const int u0 = use_x[k0 + k], u1 = use_x[k0 + k + kWG];
-      const int u2 = use_x[k0 + k + 2 * kWG], u3 = use_x[k0 + k + 3 * kWG];
-      const A a0 = (A)x[u0], a1 = (A)x[u1], a2 = (A)x[u2], a3 = (A)x[u3];
-      xs[k] = a0;
-      xs[k + kWG] = a1;
-      xs[k + 2 * kWG] = a2;
-      xs[k + 3 * kWG] = a3;
-    }
-    for (; k < nec; k += kWG) xs[k] = (A)x[use_x[k0 + k]];
+  constexpr int KU = kItems / kWG;
+  // every load of the partition in flight together: its column list, its
+  // entries, the thread's row bounds; then the x gathers
+  int u[KU];
+#pragma unroll
+  for (int j = 0; j < KU; ++j) {
+    const int k = tid + j * kWG;
+    u[j] = k < nec ? use_x[k0 + k] : -1;
+  }
+  const int n = y1 - y0;  // <= kItems
+  A v[KU];
+  uint16_t c[KU];
+#pragma unroll
+  for (int j = 0; j < KU; ++j) {
+    const int e = y0 + tid + j * kWG;
+    c[j] = e < y1 ? ecr[e] : (uint16_t)0;
+    v[j] = e < y1 ? Val<VT>::get(val, e) : A(0);
   }
   const int nrows = x1 - x0 + (x1 < m ? 1 : 0);
-  // this thread's row bounds (thread-per-row case), fetched before the barriers
   int lo = 0, hi = 0;
   if (nrows > kWG / 2 && tid < nrows) {
     const int r = x0 + tid;
     lo = max(rst[r], y0) - y0;
     hi = min(rst[r + 1], y1) - y0;
   }
-  const int n = y1 - y0;  // <= kItems <= 8 * kWG
-  A v[8];
-  uint16_t c[8];
+  A gx[KU];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int e = y0 + tid + j * kWG;
-    c[j] = e < y1 ? ecr[e] : (uint16_t)0;
-    v[j] = e < y1 ? Val<VT>::get(val, e) : A(0);
-  }
+  for (int j = 0; j < KU; ++j) gx[j] = u[j] >= 0 ? (A)x[u[j]] : A(0);
+#pragma unroll
+  for (int j = 0; j < KU; ++j)
+    if (u[j] >= 0) xs[tid + j * kWG] = gx[j];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 8; ++j)
+  for (int j = 0; j < KU; ++j)
     if (tid + j * kWG < n) prod[tid + j * kWG] = v[j] * xs[c[j]];
   __syncthreads();
   auto finish = [&](int r, A s) {
@@ -141,9 +145,9 @@ static int run_vnec(const rvvla_mat_s& A, double alpha, const void* x, double be
   constexpr bool COMP = sizeof(VT) == 2;  // 16-bit values: TwoSum throughout
   AC* fp = (AC*)(scratch ? scratch : d.scratch);
   AC* wp = fp + d.nparts;
-  hipLaunchKernelGGL((k_vnec<VT, COMP>), dim3((unsigned)d.nparts), dim3(kWG), 0, s, d.cx, d.cy,
+  hipLaunchKernelGGL((k_vnec<VT, COMP>), dim3((unsigned)(sizeof(AC) < 8 ? xcd_grid(d.nparts) : d.nparts)), dim3(kWG), 0, s, d.cx, d.cy,
                      d.nec_ptr, d.use_x, d.ecr, d.rst, (const VT*)d.val, (const IO*)x, (IO*)y, fp,
-                     wp, A.m, (AC)alpha, (AC)beta);
+                     wp, d.nparts, A.m, (AC)alpha, (AC)beta);
   if (int rc = hip_check(hipGetLastError(), "k_vnec launch")) return rc;
   hipLaunchKernelGGL((k_vnec_fixup<VT, COMP>), dim3((unsigned)((d.nparts + kWG - 1) / kWG)), dim3(kWG),
                      0, s, d.cx, d.cy, d.rst, (IO*)y, fp, wp, d.nparts, A.m, (AC)alpha, (AC)beta);
