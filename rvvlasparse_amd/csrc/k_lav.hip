@@ -34,7 +34,26 @@ __device__ __forceinline__ typename Val<VT>::acc lav_sparse_row(
     const LavSparse<VT>& sp, int e0, int e1, const typename Val<VT>::io* __restrict__ x) {
   using A = typename Val<VT>::acc;
   A s = 0;
-  for (int e = e0; e < e1; ++e) s += Val<VT>::get(sp.val, e) * (A)x[sp.col[e]];
+  // eight entries' loads, then their gathers, in flight together; added in order
+  for (int e = e0; e < e1; e += 8) {
+    int cc[8];
+    A vv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      cc[j] = 0;
+      vv[j] = A(0);
+      if (e + j < e1) {
+        cc[j] = sp.col[e + j];
+        vv[j] = Val<VT>::get(sp.val, e + j);
+      }
+    }
+    A p[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) p[j] = e + j < e1 ? vv[j] * (A)x[cc[j]] : A(0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (e + j < e1) s += p[j];
+  }
   return s;
 }
 
