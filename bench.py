@@ -440,7 +440,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-powerlaw", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the autograd-SpMM and batched legs")
+                    help="skip the autotuner, autograd-SpMM and batched legs")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB copy probe (counter calibration)")
     ap.add_argument("--only", choices=("batch", "autograd"), default=None,
@@ -529,11 +529,12 @@ def main():
     }
     line["copy_probe"] = copy_probe(torch)
     line["read_probe"] = read_probe(R, torch)
-    log("bench: autotuner on the Laplacian (FP64)")
-    t0 = time.perf_counter()
-    tf_, tp_ = R.autotune(a, R.F64, "reps=20")
-    line["autotune"] = {"laplacian/f64": f"{tf_}:{tp_}", "candidates": "all eight formats",
-                        "seconds": round(time.perf_counter() - t0, 1)}
+    if not args.no_extra:
+        log("bench: autotuner on the Laplacian (FP64)")
+        t0 = time.perf_counter()
+        tf_, tp_ = R.autotune(a, R.F64, "reps=20")
+        line["autotune"] = {"laplacian/f64": f"{tf_}:{tp_}", "candidates": "all eight formats",
+                            "seconds": round(time.perf_counter() - t0, 1)}
     mats = {"laplacian": (a, x64)}
     log("bench: config-1 (n = 65,536, 16 per row) CSR and HCSR")
     line["config1"], a1, x1 = config1_leg(R, torch, args.steps, args.warmup)
