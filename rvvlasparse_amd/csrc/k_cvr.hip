@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kWG) void k_cvr(
   int32_t rb = rec_start[t];
   Acc<A, COMP> acc;  // 16-bit values (COMP): TwoSum compensated
   auto write = [&](int32_t r) {
-    if (r >= 0) y[r] = alpha * acc.value() + scaled_y(beta, y, r);
+    if (r >= 0) put_y(y, r, alpha * acc.value() + scaled_y(beta, y, r));
     else part[-(int64_t)r - 2] = acc.value();
   };
   // kBatch packs at a time: masks, flushed rows, columns, values and the

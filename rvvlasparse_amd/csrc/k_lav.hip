@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
     a0.merge(a2);
     const A dense = a0.value();
     if constexpr (FUSED)
-      y[r] = alpha * (lav_sparse_row(sp, s0, s1, x) + dense) + scaled_y(beta, y, r);
+      put_y(y, r, alpha * (lav_sparse_row(sp, s0, s1, x) + dense) + scaled_y(beta, y, r));
     else
       y[r] = (A)y[r] + alpha * dense;
   }

@@ -63,7 +63,7 @@ __device__ __forceinline__ void sell_slice(
     a0 += !BOUND || j < len ? p : A(0);
   }
   const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
-  y[r] = alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r);
+  put_y(y, r, alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r));
 }
 
 // CHUNK: a grid of at most 8 workgroups a CU, each walking its own

@@ -114,6 +114,18 @@ static inline int device_cu_count() {
 __device__ __forceinline__ int xcd_run_index(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
 constexpr int64_t xcd_grid(int64_t g) { return (g + 7) / 8 * 8; }
 
+// y[i] = v, FP64 results with a nontemporal store.  Used where a
+// wavefront writes whole runs of consecutive rows (SELL, CVR, fused LAV):
+// measured 1-3 % faster there on the 2048^2 Laplacian, and slower where
+// lanes write scattered rows (CSR5 +25 %, VHCC +4 %) or in the LDS-sliced
+// combine (+2 %).
+template <class IO, class A>
+__device__ __forceinline__ void put_y(IO* __restrict__ y, int64_t i, A v) {
+  const IO o = (IO)v;
+  if constexpr (sizeof(IO) == 8) __builtin_nontemporal_store(o, y + i);
+  else y[i] = o;
+}
+
 // Nontemporal streaming of the matrix arrays is on unless the environment
 // sets RVVLA_NT=0 (for comparisons).
 static inline bool nt_enabled() {
