@@ -20,7 +20,8 @@
 // The order of every addition is fixed by the image, so results repeat
 // bit for bit.  16-bit values accumulate with TwoSum compensation.  A
 // slice cut into several units leaves one partial sum per row and unit,
-// summed in unit order by k_hcsr_combine.
+// summed in unit order by k_hcsr_combine.  With 32-bit accumulation each
+// XCD walks a contiguous run of units.
 #include "kcommon.h"
 
 namespace rvvla {
@@ -148,7 +149,9 @@ __global__ __launch_bounds__(kWG) void k_hcsr_slices(
   __shared__ A ls[NW][64];
   __shared__ A lc[COMP ? NW : 1][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t u = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * NW + wv));
+  // 32-bit accumulation: each XCD walks a contiguous run of units
+  const int wg = sizeof(A) < 8 ? xcd_run_index((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+  const int64_t u = __builtin_amdgcn_readfirstlane((int)(wg * NW + wv));
   if (u >= nunits) return;
   const int32_t* U = unit + u * kHcUnitInts;
   const int32_t row0 = U[0], nrows = U[1], s0 = U[2], s1 = U[3], te0 = U[4], te1 = U[5],
@@ -262,7 +265,8 @@ static int run_hcsr(const rvvla_mat_s& A, double alpha, const void* x, double be
   const DevHCSR& d = A.hcsr;
   AC* part = (AC*)(scratch ? scratch : d.part);
   if (d.nunits > 0) {
-    const unsigned g = (unsigned)((d.nunits + kWG / 64 - 1) / (kWG / 64));
+    const int64_t g0 = (d.nunits + kWG / 64 - 1) / (kWG / 64);
+    const unsigned g = (unsigned)(sizeof(AC) < 8 ? xcd_grid(g0) : g0);
     hipLaunchKernelGGL((k_hcsr_slices<VT, COMP>), dim3(g), dim3(kWG), 0, s, d.unit, d.nunits,
                        d.dxoff, d.dcol, (const VT*)d.dval, d.tcol, d.trow,
                        (const VT*)d.tval, (const IO*)x, (IO*)y, part, (AC)alpha, (AC)beta,

@@ -109,8 +109,8 @@ static inline int device_cu_count() {
 // the contiguous run [k G/8, (k+1) G/8), so that neighbouring work units,
 // which share lines of x and of the streamed arrays, meet in one L2.
 // Measured on the 2048^2 Laplacian it pays for 32-bit accumulation (VNEC,
-// VHCC, CVR, CSR5: 6-15 %) and costs FP64 up to 10 %, so the launchers
-// use it for 32-bit accumulators only.
+// VHCC, CVR, CSR5, HCSR: 5-15 %) and is neutral or costs FP64 up to 10 %,
+// so the launchers use it for 32-bit accumulators only.
 __device__ __forceinline__ int xcd_run_index(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
 constexpr int64_t xcd_grid(int64_t g) { return (g + 7) / 8 * 8; }
 
