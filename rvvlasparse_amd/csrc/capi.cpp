@@ -948,6 +948,10 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
   }
   std::vector<int64_t> sptr((size_t)S);
   std::vector<V> vals(a.val);
+  // lanes of each step: 64 up to the chunk's full-mask offset, the step's
+  // mask after it (the kernels drop the products of the other lanes, so a
+  // padding slot never multiplies x, not even an Inf)
+  std::vector<uint16_t> lanes((size_t)total, (uint16_t)L);
   for (int64_t g = 0; g < S; ++g) {
     sptr[(size_t)g] = (int64_t)a.segment_ptr[(size_t)g];
     for (int64_t c = 0; c < cc; ++c) {
@@ -958,6 +962,8 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
         throw Error(RVVLA_ERR_INVALID_ARG, "LAV: chunk offsets out of range");
       for (int64_t k = kf; k < k1; ++k) {
         const int64_t step = sptr[(size_t)g] + k;
+        if (a.mask[(size_t)step] > L) throw Error(RVVLA_ERR_INVALID_ARG, "LAV: mask out of range");
+        lanes[(size_t)step] = (uint16_t)a.mask[(size_t)step];
         for (int64_t i = a.mask[(size_t)step]; i < L; ++i) vals[(size_t)(step * L + i)] = V(0);
       }
     }
@@ -983,6 +989,8 @@ rvvla_mat_t upload_lav(int dtype, const spMtxLAV<V>& a) {
                                         S > 0 ? (size_t)(cc * L) * 4 : 0);
   d.col = M->put_vec(a.col_id);
   d.val = put_values(*M, dtype, vals.data(), vals.size());
+  d.mask = M->put_vec(lanes);
+  d.cfull = M->put_vec(a.chunk_offsets_full_mask);
   attach_lav_pieces(*M, S, cc, a.chunk_offsets.data(), sptr.data());
   return M.release();
 }
@@ -1549,6 +1557,8 @@ int rvvla_lav_from_device(int dtype, int64_t m, int64_t n, int64_t nnz, const in
       d.val = h.dval;
       d.ref_cfull = h.cfull;
       d.ref_mask = h.mask;
+      d.mask = h.mask;  // 64 for the full steps, the lane count for the others
+      d.cfull = h.cfull;
       d.total_steps = h.total;
       std::vector<int32_t> coff((size_t)(h.S * (h.cc + 1)));
       std::vector<int64_t> sptr((size_t)h.S + 1);

@@ -199,6 +199,8 @@ struct DevLAV {
   const uint32_t* out_order = nullptr; // nchunks * 64 rows (shared by all segments)
   const uint32_t* col = nullptr;
   const void* val = nullptr;
+  const int32_t* cfull = nullptr;      // nseg * nchunks: first step of each chunk that is not full
+  const uint16_t* mask = nullptr;      // valid lanes of each step (64 when full)
   int64_t nwhole = 0;                  // chunks run whole
   const int32_t* whole = nullptr;      // their ids (nullptr: all chunks, in order)
   int64_t npieces = 0, nsplit = 0;

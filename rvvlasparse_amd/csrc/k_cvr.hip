@@ -54,19 +54,28 @@ __global__ __launch_bounds__(kWG) void k_cvr(
       c[u] = in ? col[e] : 0u;
       v[u] = in ? Val<VT>::get(val, e) : A(0);
     }
+    // a pack past the stream's end adds zero (its x gather is column 0,
+    // which may hold Inf): the gathers are all taken first (the empty asm),
+    // so that the select does not move any of them under a branch
+    A g[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
-      p[u] = v[u] * (A)x[c[u]];
+      g[u] = (A)x[c[u]];
       rr[u] = ((mk[u] >> lane) & 1u) ? rec_rows[rb + __popcll(mk[u] & below)] : 0;
       rb += __popcll(mk[u]);
     }
+    static_assert(kBatch == 8, "eight gathers kept live");
+    __asm__ volatile("" : "+v"(g[0]), "+v"(g[1]), "+v"(g[2]), "+v"(g[3]), "+v"(g[4]), "+v"(g[5]),
+                     "+v"(g[6]), "+v"(g[7]));
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) p[u] = kb + u < k1 ? v[u] * g[u] : A(0);
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {
       if ((mk[u] >> lane) & 1u) {
         write(rr[u]);
         acc = Acc<A, COMP>();
       }
-      acc.add(p[u]);  // zero past the stream's last pack
+      acc.add(p[u]);
     }
   }
   const int32_t r = tail_rows[t * 64 + lane];

@@ -10,15 +10,23 @@
 // y = alpha * (dense + sparse) + beta * y once (no CSR pass, no second
 // pass over y).  Because the library sorts rows
 // once for all segments, a wavefront walks chunk c through every segment,
-// keeps the lane sums in registers and adds alpha * sum into y once.  Slots
-// past a step's mask were zeroed at upload, so all 64 lanes load every step
-// (one contiguous access per step) without per-lane predicates.  Chunks
+// keeps the lane sums in registers and adds alpha * sum into y once.  All
+// 64 lanes load every step (one contiguous access per step); the products
+// of lanes past the step's lane count (its mask) are dropped by a select,
+// so a padding slot never multiplies x (0 * Inf would be NaN).  Chunks
 // longer than kLavPiece steps are cut into pieces (k_lav_pieces) whose
 // partials k_lav_combine adds (a workgroup per chunk), so no wavefront
 // walks a chunk of thousands of steps alone.
 #include "kcommon.h"
 
 namespace rvvla {
+
+// The four gathers of a step group taken together before the lane selects
+// (so that no select moves a gather under a branch of its own).
+template <class A>
+__device__ __forceinline__ void lav_keep(A& g0, A& g1, A& g2, A& g3) {
+  __asm__ volatile("" : "+v"(g0), "+v"(g1), "+v"(g2), "+v"(g3));
+}
 
 // The sparse_part CSR arrays, for the fused form.
 template <class VT>
@@ -62,7 +70,8 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
     const int32_t* __restrict__ coff, const int64_t* __restrict__ sptr, int64_t nseg,
     int64_t nchunks, const int32_t* __restrict__ whole, int64_t nwhole,
     const uint32_t* __restrict__ out_order, const uint32_t* __restrict__ col,
-    const VT* __restrict__ val, const typename Val<VT>::io* __restrict__ x,
+    const VT* __restrict__ val, const int32_t* __restrict__ cfull,
+    const uint16_t* __restrict__ lanes, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, int64_t m, typename Val<VT>::acc alpha,
     LavSparse<VT> sp, typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
@@ -86,7 +95,10 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
     const int64_t base = sptr[g];
     int64_t k = base + coff[g * (nchunks + 1) + c];
     const int64_t k1 = base + coff[g * (nchunks + 1) + c + 1];
-    for (; k + 3 < k1; k += 4) {
+    // steps before kf hold all 64 lanes; from kf on, the products of lanes
+    // past the step's lane count are dropped
+    const int64_t kf = base + cfull[g * nchunks + c];
+    for (; k + 3 < kf; k += 4) {
       const int64_t e0 = k * 64 + lane;
       const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
       const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
@@ -96,9 +108,15 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
       a2.add(v2 * (A)x[c2]);
       a3.add(v3 * (A)x[c3]);
     }
-    for (; k < k1; ++k) {
+    for (; k < kf; ++k) {
       const int64_t e0 = k * 64 + lane;
       a0.add(Val<VT>::get(val, e0) * (A)x[col[e0]]);
+    }
+    for (; k < k1; ++k) {
+      const int64_t e0 = k * 64 + lane;
+      A p = Val<VT>::get(val, e0) * (A)x[col[e0]];
+      keep_live(p);
+      a0.add(lane < lanes[k] ? p : A(0));
     }
   }
   if (rp < m) {
@@ -120,7 +138,7 @@ __global__ __launch_bounds__(kWG) void k_lav_dense(
 template <class VT, bool COMP>
 __global__ __launch_bounds__(kWG) void k_lav_pieces(
     const int64_t* __restrict__ pc_k0, const int64_t* __restrict__ pc_k1, int64_t npieces,
-    const uint32_t* __restrict__ col, const VT* __restrict__ val,
+    const uint32_t* __restrict__ col, const VT* __restrict__ val, const uint16_t* __restrict__ lanes,
     const typename Val<VT>::io* __restrict__ x, typename Val<VT>::acc* __restrict__ part) {
   using A = typename Val<VT>::acc;
   const int64_t q = (int64_t)blockIdx.x * (kWG / 64) + (threadIdx.x >> 6);
@@ -134,14 +152,19 @@ __global__ __launch_bounds__(kWG) void k_lav_pieces(
     const uint32_t c0 = col[e0], c1 = col[e0 + 64], c2 = col[e0 + 128], c3 = col[e0 + 192];
     const A v0 = Val<VT>::get(val, e0), v1 = Val<VT>::get(val, e0 + 64);
     const A v2 = Val<VT>::get(val, e0 + 128), v3 = Val<VT>::get(val, e0 + 192);
-    a0.add(v0 * (A)x[c0]);
-    a1.add(v1 * (A)x[c1]);
-    a2.add(v2 * (A)x[c2]);
-    a3.add(v3 * (A)x[c3]);
+    const int n0 = lanes[k], n1 = lanes[k + 1], n2 = lanes[k + 2], n3 = lanes[k + 3];
+    A g0 = (A)x[c0], g1 = (A)x[c1], g2 = (A)x[c2], g3 = (A)x[c3];
+    lav_keep(g0, g1, g2, g3);
+    a0.add(lane < n0 ? v0 * g0 : A(0));
+    a1.add(lane < n1 ? v1 * g1 : A(0));
+    a2.add(lane < n2 ? v2 * g2 : A(0));
+    a3.add(lane < n3 ? v3 * g3 : A(0));
   }
   for (; k < k1; ++k) {
     const int64_t e0 = k * 64 + lane;
-    a0.add(Val<VT>::get(val, e0) * (A)x[col[e0]]);
+    A p = Val<VT>::get(val, e0) * (A)x[col[e0]];
+    keep_live(p);
+    a0.add(lane < lanes[k] ? p : A(0));
   }
   a0.merge(a1);
   a2.merge(a3);
@@ -193,7 +216,8 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
 #define RVVLA_LAV_DENSE(F)                                                                      \
   hipLaunchKernelGGL((k_lav_dense<VT, F, COMP>), dim3(g), dim3(kWG), 0, s, d.coff, d.sptr,     \
                      d.nseg, d.nchunks, d.whole, d.nwhole, d.out_order, d.col,                \
-                     (const VT*)d.val, (const IO*)x, (IO*)y, A.m, (AC)alpha, sp, (AC)beta)
+                     (const VT*)d.val, d.cfull, d.mask, (const IO*)x, (IO*)y, A.m, (AC)alpha, sp, \
+                     (AC)beta)
     if (d.fused) RVVLA_LAV_DENSE(true);
     else RVVLA_LAV_DENSE(false);
 #undef RVVLA_LAV_DENSE
@@ -201,8 +225,8 @@ static int run_lav(const rvvla_mat_s& A, double alpha, const void* x, double bet
   }
   if (d.nsplit > 0) {
     hipLaunchKernelGGL((k_lav_pieces<VT, COMP>), dim3((unsigned)((d.npieces + W - 1) / W)), dim3(kWG),
-                       0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, (const IO*)x,
-                       part);
+                       0, s, d.pc_k0, d.pc_k1, d.npieces, d.col, (const VT*)d.val, d.mask,
+                       (const IO*)x, part);
     if (int rc = hip_check(hipGetLastError(), "k_lav_pieces launch")) return rc;
     if (d.fused)
       hipLaunchKernelGGL((k_lav_combine<VT, true, COMP>), dim3((unsigned)d.nsplit), dim3(kWG), 0, s,

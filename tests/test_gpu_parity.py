@@ -288,6 +288,58 @@ def test_csr_sell_image_nonfinite_x(dtype):
     assert np.allclose(y[fin], y_ref[fin])
 
 
+def _nonfinite_case():
+    """3000 x 3000, no empty rows: 1..40 entries a row, columns drawn from
+    1..n-1 except that every 97th row also holds column 0; two rows of 1700
+    entries (long-row paths) that avoid column 0; positive values."""
+    rng = np.random.default_rng(21)
+    m = n = 3000
+    rows = []
+    for r in range(m):
+        k = 1700 if r in (500, 2222) else int(rng.integers(1, 41))
+        c = rng.choice(np.arange(1, n), k, replace=False)
+        if r % 97 == 0:
+            c[0] = 0
+        rows.append(np.sort(c))
+    rst = np.zeros(m + 1, np.int32)
+    rst[1:] = np.cumsum([len(c) for c in rows])
+    col = np.concatenate(rows).astype(np.int32)
+    return R.CSR(m, n, rst, col, rng.uniform(0.5, 1.5, len(col)))
+
+
+@pytest.mark.parametrize("fmt", FORMATS_READY)
+@pytest.mark.parametrize("dtype", [R.F64, R.F32])
+def test_nonfinite_x_all_formats(fmt, dtype):
+    """x[0] = +Inf: the rows that hold column 0 (positive values) give +Inf
+    and every other row is finite and within the bound of the oracle, as in
+    the reference CSR product: no format lets a padding slot, a lane past its
+    stream or a batch past its partition multiply the Inf (0 * Inf = NaN)."""
+    if "nonfinite" not in _CACHE:
+        _CACHE["nonfinite"] = _nonfinite_case()
+    a = _CACHE["nonfinite"]
+    x64, _ = inputs(a, 5)
+    x64[0] = np.inf
+    dt = np.float64 if dtype == R.F64 else np.float32
+    touches = np.zeros(a.m, bool)
+    rows = np.repeat(np.arange(a.m), np.diff(a.rst))
+    touches[rows[a.col == 0]] = True
+    for alpha, beta in ((1.0, 0.0), (0.5, -1.5)):
+        y064 = np.random.default_rng(6).uniform(-1, 1, a.m)
+        y = run_gpu(fmt, dtype, a, x64, y064, alpha, beta)
+        assert not np.isnan(y).any(), f"{fmt}: NaN in rows {np.flatnonzero(np.isnan(y))[:8]}"
+        assert np.array_equal(np.isposinf(y), touches), f"{fmt}: Inf rows differ"
+        v, x, y0 = a.values_for(dtype), x64.astype(dt), y064.astype(dt)
+        keep = ~touches
+        xf = x.copy()
+        xf[0] = 0  # the rows kept never read it
+        y_ref = O.csr_spmv(a.rst, a.col, v, xf, alpha, beta, y0)
+        sub = np.flatnonzero(keep)
+        rs = np.concatenate([[0], np.cumsum(np.diff(a.rst)[sub])]).astype(np.int32)
+        idx = np.concatenate([np.arange(a.rst[r], a.rst[r + 1]) for r in sub])
+        assert_within_bound(y[sub], y_ref[sub], rs, a.col[idx], v[idx], xf, alpha, beta, y0[sub],
+                            dt, f"{fmt}/nonfinite/{dt.__name__}/a={alpha},b={beta}")
+
+
 @pytest.mark.parametrize("cwg", [256, 512, 1024])
 @pytest.mark.parametrize("name", CASES + ["wide"])
 @pytest.mark.parametrize("dtype", [R.F64, R.F32])
