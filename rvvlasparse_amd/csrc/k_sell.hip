@@ -22,6 +22,36 @@ namespace rvvla {
 // gives the length), so that padding slots never multiply x, as in the CSR
 // product; the SELL format itself multiplies its zero padding like the
 // reference's SELL kernels.
+// A slice of exactly W steps (W <= 8): every column and value load, then
+// every gather, with no predicate between them; the products are added as
+// the general loop adds them (slot j to a[j mod 4] in whole groups of four,
+// the rest to a0).
+template <class VT, bool NT, bool BOUND, int W>
+__device__ __forceinline__ void sell_fixed(int64_t idx, int height, int len,
+                                           const uint32_t* __restrict__ col,
+                                           const VT* __restrict__ val,
+                                           const typename Val<VT>::io* __restrict__ x,
+                                           typename Val<VT>::acc (&a)[4]) {
+  using A = typename Val<VT>::acc;
+  uint32_t c[W];
+  A v[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    c[j] = ldu<NT>(col, idx + j * (int64_t)height);
+    v[j] = ldv<NT>(val, idx + j * (int64_t)height);
+  }
+  A p[W];
+#pragma unroll
+  for (int j = 0; j < W; ++j) p[j] = v[j] * (A)x[c[j]];
+  constexpr int WB = W / 4 * 4;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const A q = !BOUND || j < len ? p[j] : A(0);
+    if (j < WB) a[j % 4] += q;
+    else a[0] += q;
+  }
+}
+
 template <class VT, bool IDENT, bool NT, bool BOUND>
 __device__ __forceinline__ void sell_slice(
     int64_t slice, int lane, const int32_t* __restrict__ cl, const int32_t* __restrict__ cs,
@@ -40,6 +70,23 @@ __device__ __forceinline__ void sell_slice(
   A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   int len = width;  // BOUND: the row's own entries
   if constexpr (BOUND) len = rst[srow + lane + 1] - rst[srow + lane];
+  if (width <= 8) {  // uniform: one straight-line batch
+    A a[4] = {A(0), A(0), A(0), A(0)};
+    switch (width) {
+      case 1: sell_fixed<VT, NT, BOUND, 1>(idx, height, len, col, val, x, a); break;
+      case 2: sell_fixed<VT, NT, BOUND, 2>(idx, height, len, col, val, x, a); break;
+      case 3: sell_fixed<VT, NT, BOUND, 3>(idx, height, len, col, val, x, a); break;
+      case 4: sell_fixed<VT, NT, BOUND, 4>(idx, height, len, col, val, x, a); break;
+      case 5: sell_fixed<VT, NT, BOUND, 5>(idx, height, len, col, val, x, a); break;
+      case 6: sell_fixed<VT, NT, BOUND, 6>(idx, height, len, col, val, x, a); break;
+      case 7: sell_fixed<VT, NT, BOUND, 7>(idx, height, len, col, val, x, a); break;
+      case 8: sell_fixed<VT, NT, BOUND, 8>(idx, height, len, col, val, x, a); break;
+      default: break;  // width 0: an empty slice
+    }
+    const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
+    put_y(y, r, alpha * ((a[0] + a[1]) + (a[2] + a[3])) + scaled_y(beta, y, r));
+    return;
+  }
   int j = 0;
   for (; j + 3 < width; j += 4) {
     // all column and value loads first, then the four gathers
