@@ -362,6 +362,16 @@ rvvla_mat_t upload_csr(int dtype, const spMtxCRS<V>& a, int xslices, int xlds, i
   if (P == 0 && choose_csr_sell(a, sell)) {
     spMtxSELL_C_Sigma<V> S;
     build_sell<V>(S, a, 64, 1);
+    // padding slots (past a row's own entries) carry kSellPadBit in their
+    // column, so the kernel drops their products without reading Rst
+    for (int64_t i = 0; i < (int64_t)S.cnt_b; ++i) {
+      const int64_t h = S.vBs[(size_t)i], w = S.vCl[(size_t)i], c0 = S.vCs[(size_t)i];
+      const int64_t r0 = S.vSBs[(size_t)i];
+      for (int64_t k = 0; k < h; ++k) {
+        const int64_t len = a.Rst[r0 + k + 1] - a.Rst[r0 + k];
+        for (int64_t j = len; j < w; ++j) S.vCol[(size_t)(c0 + j * h + k)] |= kSellPadBit;
+      }
+    }
     fill_sell(*M, dtype, S, false);
     M->csr.sell = true;
   }

@@ -112,6 +112,9 @@ struct DevSELL {
   void* part = nullptr;               // npieces * 64 partial sums
 };
 constexpr int kSellPiece = 64;
+// The SELL image of a CSR matrix marks its padding slots (past a row's own
+// entries) by this bit of the column; the SELL format keeps plain columns.
+constexpr uint32_t kSellPadBit = 0x80000000u;
 
 // CSR5 (omega = 64): tiles as the host container holds them, plus the
 // empty-row list and per-tile scratch (leading / trailing partial sums and

@@ -18,16 +18,17 @@
 namespace rvvla {
 
 // One slice by one wavefront.  BOUND (the SELL image of a CSR matrix,
-// rows in natural order): a lane adds only the entries of its row (rst
-// gives the length), so that padding slots never multiply x, as in the CSR
-// product; the SELL format itself multiplies its zero padding like the
-// reference's SELL kernels.
+// rows in natural order): a lane adds only the entries of its row (its
+// padding slots carry kSellPadBit in the column, whose product is dropped by
+// a select), so that padding never multiplies x, as in the CSR product; the
+// SELL format itself multiplies its zero padding like the reference's SELL
+// kernels.
 // A slice of exactly W steps (W <= 8): every column and value load, then
 // every gather, with no predicate between them; the products are added as
 // the general loop adds them (slot j to a[j mod 4] in whole groups of four,
 // the rest to a0).
 template <class VT, bool NT, bool BOUND, int W>
-__device__ __forceinline__ void sell_fixed(int64_t idx, int height, int len,
+__device__ __forceinline__ void sell_fixed(int64_t idx, int height,
                                            const uint32_t* __restrict__ col,
                                            const VT* __restrict__ val,
                                            const typename Val<VT>::io* __restrict__ x,
@@ -42,11 +43,11 @@ __device__ __forceinline__ void sell_fixed(int64_t idx, int height, int len,
   }
   A p[W];
 #pragma unroll
-  for (int j = 0; j < W; ++j) p[j] = v[j] * (A)x[c[j]];
+  for (int j = 0; j < W; ++j) p[j] = v[j] * (A)x[BOUND ? c[j] & ~kSellPadBit : c[j]];
   constexpr int WB = W / 4 * 4;
 #pragma unroll
   for (int j = 0; j < W; ++j) {
-    const A q = !BOUND || j < len ? p[j] : A(0);
+    const A q = !BOUND || !(c[j] & kSellPadBit) ? p[j] : A(0);
     if (j < WB) a[j % 4] += q;
     else a[0] += q;
   }
@@ -59,7 +60,7 @@ __device__ __forceinline__ void sell_slice(
     const uint32_t* __restrict__ col, const VT* __restrict__ val,
     const uint32_t* __restrict__ perm, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, typename Val<VT>::acc alpha,
-    typename Val<VT>::acc beta, const int32_t* __restrict__ rst) {
+    typename Val<VT>::acc beta) {
   using A = typename Val<VT>::acc;
   const int height = bs[slice], width = cl[slice];
   if (width > kSellPiece) return;  // a wide slice: run as pieces
@@ -68,19 +69,17 @@ __device__ __forceinline__ void sell_slice(
   if (lane >= height) return;
   int64_t idx = base + lane;
   A a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  int len = width;  // BOUND: the row's own entries
-  if constexpr (BOUND) len = rst[srow + lane + 1] - rst[srow + lane];
   if (width <= 8) {  // uniform: one straight-line batch
     A a[4] = {A(0), A(0), A(0), A(0)};
     switch (width) {
-      case 1: sell_fixed<VT, NT, BOUND, 1>(idx, height, len, col, val, x, a); break;
-      case 2: sell_fixed<VT, NT, BOUND, 2>(idx, height, len, col, val, x, a); break;
-      case 3: sell_fixed<VT, NT, BOUND, 3>(idx, height, len, col, val, x, a); break;
-      case 4: sell_fixed<VT, NT, BOUND, 4>(idx, height, len, col, val, x, a); break;
-      case 5: sell_fixed<VT, NT, BOUND, 5>(idx, height, len, col, val, x, a); break;
-      case 6: sell_fixed<VT, NT, BOUND, 6>(idx, height, len, col, val, x, a); break;
-      case 7: sell_fixed<VT, NT, BOUND, 7>(idx, height, len, col, val, x, a); break;
-      case 8: sell_fixed<VT, NT, BOUND, 8>(idx, height, len, col, val, x, a); break;
+      case 1: sell_fixed<VT, NT, BOUND, 1>(idx, height, col, val, x, a); break;
+      case 2: sell_fixed<VT, NT, BOUND, 2>(idx, height, col, val, x, a); break;
+      case 3: sell_fixed<VT, NT, BOUND, 3>(idx, height, col, val, x, a); break;
+      case 4: sell_fixed<VT, NT, BOUND, 4>(idx, height, col, val, x, a); break;
+      case 5: sell_fixed<VT, NT, BOUND, 5>(idx, height, col, val, x, a); break;
+      case 6: sell_fixed<VT, NT, BOUND, 6>(idx, height, col, val, x, a); break;
+      case 7: sell_fixed<VT, NT, BOUND, 7>(idx, height, col, val, x, a); break;
+      case 8: sell_fixed<VT, NT, BOUND, 8>(idx, height, col, val, x, a); break;
       default: break;  // width 0: an empty slice
     }
     const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
@@ -98,16 +97,19 @@ __device__ __forceinline__ void sell_slice(
     const A v3 = ldv<NT>(val, idx + 3 * (int64_t)height);
     // the gathers run for every slot (padding holds a valid column); BOUND
     // then drops a padding product (possibly 0 * Inf) by a select
-    const A p0 = v0 * (A)x[c0], p1 = v1 * (A)x[c1], p2 = v2 * (A)x[c2], p3 = v3 * (A)x[c3];
-    a0 += !BOUND || j < len ? p0 : A(0);
-    a1 += !BOUND || j + 1 < len ? p1 : A(0);
-    a2 += !BOUND || j + 2 < len ? p2 : A(0);
-    a3 += !BOUND || j + 3 < len ? p3 : A(0);
+    constexpr uint32_t K = BOUND ? ~kSellPadBit : ~0u;
+    const A p0 = v0 * (A)x[c0 & K], p1 = v1 * (A)x[c1 & K], p2 = v2 * (A)x[c2 & K],
+            p3 = v3 * (A)x[c3 & K];
+    a0 += !BOUND || !(c0 & kSellPadBit) ? p0 : A(0);
+    a1 += !BOUND || !(c1 & kSellPadBit) ? p1 : A(0);
+    a2 += !BOUND || !(c2 & kSellPadBit) ? p2 : A(0);
+    a3 += !BOUND || !(c3 & kSellPadBit) ? p3 : A(0);
     idx += 4 * (int64_t)height;
   }
   for (; j < width; ++j, idx += height) {
-    const A p = ldv<NT>(val, idx) * (A)x[ldu<NT>(col, idx)];
-    a0 += !BOUND || j < len ? p : A(0);
+    const uint32_t c = ldu<NT>(col, idx);
+    const A p = ldv<NT>(val, idx) * (A)x[BOUND ? c & ~kSellPadBit : c];
+    a0 += !BOUND || !(c & kSellPadBit) ? p : A(0);
   }
   const uint32_t r = IDENT ? (uint32_t)(srow + lane) : perm[srow + lane];
   put_y(y, r, alpha * ((a0 + a1) + (a2 + a3)) + scaled_y(beta, y, r));
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kWG) void k_sell64(
     const uint32_t* __restrict__ col, const VT* __restrict__ val,
     const uint32_t* __restrict__ perm, const typename Val<VT>::io* __restrict__ x,
     typename Val<VT>::io* __restrict__ y, int64_t nslices, typename Val<VT>::acc alpha,
-    typename Val<VT>::acc beta, const int32_t* __restrict__ rst) {
+    typename Val<VT>::acc beta) {
   const int lane = threadIdx.x & 63;
   // the slice index is uniform across the wavefront: keep it (and the slice
   // descriptor loads) scalar
@@ -141,13 +143,13 @@ __global__ __launch_bounds__(kWG) void k_sell64(
       const int64_t slice = g * (kWG / 64) + wave;
       if (slice >= nslices) break;
       sell_slice<VT, IDENT, NT, BOUND>(slice, lane, cl, cs, bs, sbs, col, val, perm, x, y, alpha,
-                                       beta, rst);
+                                       beta);
     }
   } else {
     const int64_t slice = (int64_t)blockIdx.x * (kWG / 64) + wave;
     if (slice >= nslices) return;
     sell_slice<VT, IDENT, NT, BOUND>(slice, lane, cl, cs, bs, sbs, col, val, perm, x, y, alpha,
-                                     beta, rst);
+                                     beta);
   }
 }
 
@@ -237,7 +239,7 @@ static int run_sell(const rvvla_mat_s& A, double alpha, const void* x, double be
 #define RVVLA_SELL_LAUNCH(ID, NT_, B_)                                                          \
   hipLaunchKernelGGL((k_sell64<VT, ID, NT_, B_, CHUNK>), dim3((unsigned)grid), dim3(kWG), 0, s, \
                      d.cl, d.cs, d.bs, d.sbs, d.col, (const VT*)d.val, d.perm, (const IO*)x,    \
-                     (IO*)y, d.nslices, (AC)alpha, (AC)beta, A.csr.rst)
+                     (IO*)y, d.nslices, (AC)alpha, (AC)beta)
   if (bound) {  // natural row order: the identity permutation
     if (nt) RVVLA_SELL_LAUNCH(true, true, true);
     else RVVLA_SELL_LAUNCH(true, false, true);
