@@ -516,6 +516,9 @@ def main():
                 "frac": round(r["GBs"] / PEAK_GBS, 4),
                 "traffic": tr["bytes_per_launch"] if tr else None,
                 "algorithmic_bytes": byts64, "kernel": best}
+    if tr:  # the rate of the bytes the kernel actually moves (HCSR's 16-bit columns move fewer)
+        roofline["traffic_GBs"] = round(tr["bytes_per_launch"] / (r["ms"] * 1e-3) / 1e9, 1)
+        roofline["traffic_frac"] = round(roofline["traffic_GBs"] / PEAK_GBS, 4)
     line = {
         "metric": METRIC, "value": r["GBs"], "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["wall_ms"],

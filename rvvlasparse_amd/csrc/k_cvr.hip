@@ -46,13 +46,23 @@ __global__ __launch_bounds__(kWG) void k_cvr(
     uint32_t c[kBatch];
     A v[kBatch], p[kBatch];
     int32_t rr[kBatch];
+    if (kb + kBatch <= k1) {  // a whole batch: straight-line loads
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const bool in = kb + u < k1;
-      const int64_t e = (kb + u) * 64 + lane;
-      mk[u] = in ? flush[kb + u] : 0ull;
-      c[u] = in ? col[e] : 0u;
-      v[u] = in ? Val<VT>::get(val, e) : A(0);
+      for (int u = 0; u < kBatch; ++u) {
+        const int64_t e = (kb + u) * 64 + lane;
+        mk[u] = flush[kb + u];
+        c[u] = col[e];
+        v[u] = Val<VT>::get(val, e);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const bool in = kb + u < k1;
+        const int64_t e = (kb + u) * 64 + lane;
+        mk[u] = in ? flush[kb + u] : 0ull;
+        c[u] = in ? col[e] : 0u;
+        v[u] = in ? Val<VT>::get(val, e) : A(0);
+      }
     }
     // a pack past the stream's end adds zero (its x gather is column 0,
     // which may hold Inf): the gathers are all taken first (the empty asm),

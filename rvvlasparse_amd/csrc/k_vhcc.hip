@@ -68,13 +68,23 @@ __global__ __launch_bounds__(kWG) void k_vhcc(VhccArgs a, const VT* __restrict__
     A vv[kBatch], p[kBatch];
     uint64_t mk[kBatch];
     uint32_t rr[kBatch];
+    if (vb + kBatch <= v1) {  // a whole batch: straight-line loads
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const bool in = vb + u < v1;
-      const int64_t e = (vb + u) * 64 + lane;
-      c[u] = in ? colp[e] : 0u;
-      vv[u] = in ? Val<VT>::get(val, e) : A(0);
-      mk[u] = in ? eorp[vb + u] : 0ull;
+      for (int u = 0; u < kBatch; ++u) {
+        const int64_t e = (vb + u) * 64 + lane;
+        c[u] = colp[e];
+        vv[u] = Val<VT>::get(val, e);
+        mk[u] = eorp[vb + u];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const bool in = vb + u < v1;
+        const int64_t e = (vb + u) * 64 + lane;
+        c[u] = in ? colp[e] : 0u;
+        vv[u] = in ? Val<VT>::get(val, e) : A(0);
+        mk[u] = in ? eorp[vb + u] : 0ull;
+      }
     }
     int64_t rk = rank;
 #pragma unroll
