@@ -149,15 +149,17 @@ def _check_structure(a, v, dtype, rst, col, cwg=1024):
     # units: contiguous per workgroup, tiling every task, slices ascending
     units = v["units"].reshape(-1, 3)
     wg = v["wg"]
-    assert len(wg) == CUS + 1 and wg[0] == 0 and wg[-1] == len(units) and np.all(np.diff(wg) >= 0)
+    # FP64 splits its tasks over four workgroups a CU, the 32-bit-x forms one
+    nwg = CUS * (4 if dtype == R.F64 else 1)
+    assert len(wg) == nwg + 1 and wg[0] == 0 and wg[-1] == len(units) and np.all(np.diff(wg) >= 0)
     assert units[0, 1] == 0 and units[-1, 2] == len(tasks) and np.all(units[1:, 1] == units[:-1, 2])
     assert np.all(units[:, 2] > units[:, 1])
     for u0, u1 in zip(wg[:-1], wg[1:]):
         assert np.all(np.diff(units[u0:u1, 0]) > 0), "one unit per slice in a workgroup"
     for p, t0, t1 in units:
         assert np.all(tasks[t0:t1, 3] == p)
-    share = [tasks[units[wg[w], 1]:units[wg[w + 1] - 1, 2], 1].sum() for w in range(CUS) if wg[w + 1] > wg[w]]
-    assert max(share) <= -(-nnz // CUS) + TASK, "an equal share of the entries per workgroup"
+    share = [tasks[units[wg[w], 1]:units[wg[w + 1] - 1, 2], 1].sum() for w in range(nwg) if wg[w + 1] > wg[w]]
+    assert max(share) <= -(-nnz // nwg) + TASK, "an equal share of the entries per workgroup"
     # combine blocks: cover the rows in order, within the limits; the places
     cb = v["cblk"].reshape(-1, 2)
     rp = v["rowptr"]
