@@ -518,7 +518,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": r["GBs"], "peak": PEAK_GBS, "unit": "GB/s",
                 "frac": round(r["GBs"] / PEAK_GBS, 4),
                 "traffic": tr["bytes_per_launch"] if tr else None,
-                "algorithmic_bytes": byts64, "kernel": best}
+                "algorithmic_bytes": byts64, "kernel": best,
+                # the same model for every format: a format that stores less
+                # than CSR (HCSR's 16-bit columns) can pass the peak on it;
+                # traffic_GBs is the rate of the bytes it actually moves
+                "bytes_model": "CSR-equivalent: nnz*(s_val+4) + (m+1)*4 + n*s_x + m*s_y"}
     if tr:  # the rate of the bytes the kernel actually moves (HCSR's 16-bit columns move fewer)
         roofline["traffic_GBs"] = round(tr["bytes_per_launch"] / (r["ms"] * 1e-3) / 1e9, 1)
         roofline["traffic_frac"] = round(roofline["traffic_GBs"] / PEAK_GBS, 4)
